@@ -1,0 +1,233 @@
+#!/usr/bin/env python3
+"""McSART benchmark on one MI355X.
+
+Workload: the 128^3 motion-compensated McSART scan (volume 128^3 float32 at 2 mm, detector
+256x192 at 1.5 mm, 200 views over 360 deg, 10 outer iterations of one image sweep and one motion
+sweep, lambda 1, nonnegativity clamp).  One step is one complete reconstruction of that scan,
+with the scan's measured projections, initial image and 5DCT fields resident in HBM before the
+timed region starts.  Synthetic data: seeded ellipsoid phantom, breathing trace and Gaussian-blob
+v/f fields; the measured projections are made on the GPU with this library's warp and projector
+and Poisson noise at 1e5 photons (the FP64 CPU path is far too slow to synthesise them here).
+
+Output: one JSON line with scans/hour (value), per-scan latency p50/p99, the dominant kernel's
+roofline figures (HIP events on the stream the kernels run on) and the CPU baseline (the FP64
+OpenMP oracle timed on a bounded sample of the same scan on this host).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def gpu_measurements(spec, data, photons_seed):
+    """Measured projections p_i = P(W(phantom; alpha_i v + beta_i f)) with Poisson noise, made on
+    the GPU with the library's own operators."""
+    from mcsart_amd.ops import DeviceArray, host_deform, host_projection
+    from mcsart_amd.synth import poisson_noise
+    g = spec.geometry
+    d_ph = DeviceArray.from_host(data.phantom)
+    d_w = DeviceArray.empty(g.vol_shape)
+    d_f = [DeviceArray.from_host(f) for f in data.fields] if data.fields else None
+    d_p = DeviceArray.empty(g.det_shape)
+    out = np.empty((spec.n_views,) + g.det_shape, np.float32)
+    for i, a in enumerate(spec.angles):
+        src = d_ph
+        if d_f is not None:
+            host_deform(d_w, d_ph, g.nx, g.ny, g.nz, float(data.alpha[i]), float(data.beta[i]), *d_f)
+            src = d_w
+        host_projection(d_p, src, float(a), g.SO, g.SD, g.da, g.na, g.ai, g.db, g.nb, g.bi,
+                        g.nx, g.ny, g.nz)
+        out[i] = d_p.to_host()
+    for x in [d_ph, d_w, d_p] + (d_f or []):
+        x.close()
+    if spec.noise_photons:
+        out = poisson_noise(out, photons_seed, photons=spec.noise_photons, dx_mm=spec.vx_mm)
+    return out
+
+
+def algorithmic_bytes(spec, kind):
+    """HBM bytes one launch must move at minimum (4-byte floats).
+    projector, image sweep : read x (N) and p (D), write (r, P1) (2D)      -> 4N + 12D
+    projector, motion sweep: read g, q (2N) and p (D), write (r, Pq) (2D)  -> 8N + 12D
+    backprojector          : read (r, P1) (2D), write u (N)                -> 4N + 8D
+    """
+    g = spec.geometry
+    N, D = g.nvox, g.ndet
+    if kind == "proj_image":
+        return 4 * N + 12 * D
+    if kind == "proj_motion":
+        return 8 * N + 12 * D
+    if kind == "back":
+        return 4 * N + 8 * D
+    raise KeyError(kind)
+
+
+def launches_per_scan(spec):
+    n_img = sum(1 for o in spec.op_iter if o == 1)
+    n_mot = len(spec.op_iter) - n_img
+    stride = 1 if spec.motion else 10
+    mot_views = len(range(1, spec.n_views, stride))
+    return n_img * spec.n_views, n_mot * mot_views
+
+
+def cpu_baseline(spec, data, proj, sample_views):
+    """FP64 oracle (OpenMP) on the first `sample_views` views of the same scan: one image sweep
+    and one motion sweep, scaled to a whole scan."""
+    import oracle
+    g = spec.geometry
+    V = sample_views
+    ang = spec.angles[:V]
+    al, be = data.alpha[:V], data.beta[:V]
+    p = proj[:V]
+    zero = np.zeros(g.vol_shape)
+    t0 = time.perf_counter()
+    oracle.aumisart(zero, p, g, ang, al, be, [1], fields=data.fields, nonneg=True)
+    t_img = (time.perf_counter() - t0) / V
+    t0 = time.perf_counter()
+    oracle.aumisart(zero, p, g, ang, al, be, [0], fields=data.fields, nonneg=True)
+    t_mot = (time.perf_counter() - t0) / (V - 1)
+    n_img_views, n_mot_views = launches_per_scan(spec)
+    sec_per_scan = n_img_views * t_img + n_mot_views * t_mot
+    return {
+        "value": 3600.0 / sec_per_scan,
+        "unit": "scans/hour",
+        "cores": oracle.num_threads(),
+        "kind": "port",
+        "sample": (f"FP64 oracle, {spec.name} geometry, first {V} views: one image sweep "
+                   f"({t_img:.3f} s/view) and one motion sweep ({t_mot:.3f} s/view), scaled to "
+                   f"{n_img_views} image-sweep + {n_mot_views} motion-sweep views per scan"),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--cpu-sample-views", type=int, default=12)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
+              "(scans are independent: more GPUs would run more scans, see DESIGN.md)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    from mcsart_amd import configs
+    from mcsart_amd.configs import make_scan_inputs
+    from mcsart_amd.driver import Context, Job
+
+    spec = configs.c2_mcsart128()
+    g = spec.geometry
+    log(f"[bench] workload {spec.name}: {g.nx}x{g.ny}x{g.nz}, det {g.na}x{g.nb}, V={spec.n_views}, "
+        f"op_iter={spec.op_iter}")
+    data = make_scan_inputs(spec)
+    ctx = Context(0)
+    proj = gpu_measurements(spec, data, spec.seed + 404)
+    log("[bench] measured projections ready")
+
+    job = Job(ctx, g, spec.angles, data.alpha, data.beta, spec.op_iter, lam=spec.lam,
+              motion=spec.motion, nonneg=spec.nonneg)
+    job.set_projections(proj)
+    job.set_image(None)
+    job.set_fields(data.fields)
+
+    for w in range(args.warmup):
+        job.reset()
+        job.run()
+        ctx.synchronize()
+        log(f"[bench] warmup {w + 1}/{args.warmup} done")
+
+    ctx.enable_timing(True)
+    ctx.reset_timing()
+    ctx.synchronize()
+    lat = []
+    t0 = time.perf_counter()
+    for s in range(args.steps):
+        ts = time.perf_counter()
+        job.reset()
+        job.run()
+        ctx.synchronize()
+        lat.append(time.perf_counter() - ts)
+        log(f"[bench] step {s + 1}/{args.steps}: {lat[-1] * 1e3:.1f} ms")
+    ctx.synchronize()
+    t1 = time.perf_counter()
+    elapsed = t1 - t0
+
+    img = job.image()
+    finite = float(np.isfinite(img).mean())
+
+    # dominant-kernel roofline from the HIP events recorded around every launch
+    times = {k: ctx.kernel_time(k) for k in ("projector", "backprojector", "warp", "motion")}
+    n_img, n_mot = launches_per_scan(spec)
+    K = args.steps
+    proj_bytes = K * (n_img * algorithmic_bytes(spec, "proj_image") +
+                      n_mot * algorithmic_bytes(spec, "proj_motion"))
+    back_bytes = K * (n_img + n_mot) * algorithmic_bytes(spec, "back")
+    cand = {
+        "projector": (times["projector"][0], times["projector"][1], proj_bytes),
+        "backprojector": (times["backprojector"][0], times["backprojector"][1], back_bytes),
+    }
+    dom = max(cand, key=lambda k: cand[k][0])
+    ms, launches, nbytes = cand[dom]
+    achieved = nbytes / (ms * 1e-3) / 1e9
+    roofline = {
+        "bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+        "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+        "traffic": None,
+        "avg_launch_us": round(ms * 1e3 / max(launches, 1), 2),
+        "bytes_per_launch": int(nbytes / max(launches, 1)),
+    }
+    kernel_ms = {k: {"total_ms": round(v[0], 2), "launches": v[1],
+                     "avg_us": round(v[0] * 1e3 / max(v[1], 1), 2)} for k, v in times.items()}
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        log("[bench] timing the FP64 CPU oracle on a bounded sample ...")
+        cpu = cpu_baseline(spec, data, proj, args.cpu_sample_views)
+
+    lat_ms = np.array(lat) * 1e3
+    value = 3600.0 * K / elapsed
+    out = {
+        "metric": "McSART scans/hour (128^3 motion-compensated scan, 200 views, 10 outer iterations)",
+        "value": round(value, 2),
+        "unit": "scans/hour",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": args.warmup,
+        "ms_per_step": round(elapsed * 1e3 / K, 2),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded phantom, trace, fields; projections made on GPU + Poisson noise)",
+        "config": {"workload": spec.name, "volume": [g.nx, g.ny, g.nz], "detector": [g.na, g.nb],
+                   "views": spec.n_views, "op_iter": "[1,0]x10", "lambda": spec.lam,
+                   "nonneg": spec.nonneg, "jobs_in_flight": 1},
+        "latency_ms": {"p50": round(float(np.percentile(lat_ms, 50)), 2),
+                       "p99": round(float(np.percentile(lat_ms, 99)), 2)},
+        "kernels": kernel_ms,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "output_finite_fraction": finite,
+    }
+    print(json.dumps(out), flush=True)
+    job.close()
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

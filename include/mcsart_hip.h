@@ -1,0 +1,241 @@
+/*
+ * libmcsart_hip -- MI355X (gfx950) implementation of the McSART motion-compensated SART loop.
+ *
+ * C ABI.  Plain pointers and sizes only.  Every function returns an int status (MCSART_OK == 0,
+ * negative on error); mcsart_last_error() gives a thread-local message for the last failure.
+ *
+ * Array layout is the reference's MATLAB column-major FP32 layout:
+ *   volume      img[ix + iy*nx + iz*nx*ny]                (reference src/cu_projection.cu:87)
+ *   projection  proj[ia + ib*na]                          (src/cu_projection.cu:20)
+ *   stack       proj[ia + ib*na + iv*na*nb]               (src/AUMISART.cu:203)
+ * Geometry is in voxel units (SO, SD, da, db pre-divided by the voxel size) and ai/bi are the
+ * reference's internal detector offsets, user offset - (n/2 - 0.5) (src/entry_AUMISART.cu:76-94).
+ *
+ * Operators (section 2) keep the argument order of the reference's host_* functions and take
+ * DEVICE pointers; they are synchronous: they run on a library-owned stream of the current
+ * device and wait for it (the reference ends every call with cudaDeviceSynchronize()).  Outputs of
+ * the projector and backprojector are overwritten, not accumulated.  The drivers (section 3) take
+ * HOST pointers like the reference's host_AUMISART.  The job API (section 4) keeps a scan's
+ * state resident in device memory and runs asynchronously on a context's stream.
+ */
+#ifndef MCSART_HIP_H
+#define MCSART_HIP_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------------------------------------
+ * 1. Status, device memory and contexts (no counterpart in the reference, which reports no errors
+ *    and uses the CUDA runtime directly).
+ * ------------------------------------------------------------------------------------------- */
+enum {
+    MCSART_OK = 0,
+    MCSART_EINVAL = -1,  /* invalid argument (bad size, nx != ny where required, null pointer) */
+    MCSART_ENOMEM = -2,  /* device allocation failed or HBM budget exceeded */
+    MCSART_EHIP = -3,    /* HIP runtime error */
+    MCSART_ENOTSUP = -4, /* unsupported configuration */
+    MCSART_EBUSY = -5    /* resource in use */
+};
+
+const char *mcsart_status_string(int status);
+const char *mcsart_last_error(void);
+int mcsart_version(void);
+
+int mcsart_device_count(int *count);
+int mcsart_set_device(int device);
+int mcsart_malloc(void **dptr, size_t bytes);
+int mcsart_free(void *dptr);
+int mcsart_memcpy_htod(void *dst, const void *src, size_t bytes);
+int mcsart_memcpy_dtoh(void *dst, const void *src, size_t bytes);
+int mcsart_memcpy_dtod(void *dst, const void *src, size_t bytes);
+int mcsart_memset(void *dptr, int value, size_t bytes);
+int mcsart_device_synchronize(void);
+int mcsart_mem_info(size_t *free_bytes, size_t *total_bytes);
+
+typedef struct mcsart_ctx mcsart_ctx;
+int mcsart_ctx_create(mcsart_ctx **ctx, int device);
+int mcsart_ctx_destroy(mcsart_ctx *ctx);
+int mcsart_ctx_synchronize(mcsart_ctx *ctx);
+
+/* ---------------------------------------------------------------------------------------------
+ * 2. Operators: the reference's host_* functions (include/cu_*.h), same argument order.
+ * ------------------------------------------------------------------------------------------- */
+
+/* Replaces host_projection (include/cu_projection.h:7, src/cu_projection.cu:7-13).
+ * Cone-beam footprint-weighted forward projection of one view; d_proj (na*nb) is overwritten. */
+int mcsart_projection(float *d_proj, const float *d_img, float angle, float SO, float SD, float da,
+                      int na, float ai, float db, int nb, float bi, int nx, int ny, int nz);
+
+/* Replaces host_backprojection (include/cu_backprojection.h:7, src/cu_backprojection.cu:8-46).
+ * Voxel-driven footprint backprojection of one view; d_img (nx*ny*nz) is overwritten. */
+int mcsart_backprojection(float *d_img, const float *d_proj, float angle, float SO, float SD,
+                          float da, int na, float ai, float db, int nb, float bi, int nx, int ny,
+                          int nz);
+
+/* Replaces host_deform (include/cu_deform.h:8, src/cu_deform.cu:3-47): d_img1 = trilinear
+ * clamp-to-edge warp of d_img by the DVF m_c = alpha_c*volume + beta_c*flow.  nx == ny.
+ * In-place (d_img1 == d_img) is allowed, as in the reference (src/AUMISART.cu:228). */
+int mcsart_deform(float *d_img1, const float *d_img, int nx, int ny, int nz, float volume,
+                  float flow, const float *alpha_x, const float *alpha_y, const float *alpha_z,
+                  const float *beta_x, const float *beta_y, const float *beta_z);
+
+/* Replaces host_deform2 (include/cu_deform.h:9, src/cu_deform.cu:49-64,92-113): first-order
+ * Taylor warp with the reference's swapped boundary checks; reads past the end of the volume
+ * contribute zero.  In-place allowed. */
+int mcsart_deform2(float *d_img1, const float *d_img, int nx, int ny, int nz, float volume,
+                   float flow, const float *alpha_x, const float *alpha_y, const float *alpha_z,
+                   const float *beta_x, const float *beta_y, const float *beta_z);
+
+/* Replaces host_update_udvf (include/cu_update_udvf.h:8, src/cu_update_udvf.cu:3-62). */
+int mcsart_update_udvf(float *alpha_x, float *alpha_y, float *alpha_z, float *beta_x,
+                       float *beta_y, float *beta_z, const float *img, const float *img0,
+                       float volume_diff, float flow_diff, int nx, int ny, int nz, int iter);
+
+/* Replaces host_add (include/cu_add.h:4, src/cu_add.cu:2-18): img1 += weight * img. */
+int mcsart_add(float *img1, const float *img, int nx, int ny, int nz, float weight);
+
+/* Replaces host_add2 (include/cu_add.h:8, src/cu_add.cu:20-58):
+ * img1[ix,iy,iz] += img[iy,ix,iz] * weight * D_ind(img0)[ix,iy,iz]; nx == ny. */
+int mcsart_add2(float *img1, const float *img, int nx, int ny, int nz, const float *img0,
+                float weight, int ind);
+
+/* Replaces host_initial (include/cu_initial.h:5, src/cu_initial.cu:2-17). */
+int mcsart_initial(float *img, int nx, int ny, int nz, float value);
+
+/* Replaces host_initial2 (include/cu_initial.h:8, src/cu_initial.cu:20-49):
+ * img = (D1 + D2 + D3)(img0) * (volume + flow). */
+int mcsart_initial2(float *img, int nx, int ny, int nz, const float *img0, float volume,
+                    float flow);
+
+/* Replaces host_division (include/cu_division.h:5, src/cu_division.cu:2-25): img1 /= img with
+ * NaN and +-Inf mapped to 0.  In-place (img1 == img) allowed. */
+int mcsart_division(float *img1, const float *img, int nx, int ny, int nz);
+
+/* Replaces host_diff (src/mex_diff.cu:5,67-112): backward difference along ind (1, 2, 3), zero
+ * at the first and last index. */
+int mcsart_diff(float *img1, const float *img, int nx, int ny, int nz, int ind);
+
+/* Volume form of the __device__ diff_x/diff_y/diff_z (include/cu_diff.h:4-6,
+ * src/cu_diff.cu:2-33): forward difference along ind, zero at the last index. */
+int mcsart_diff_forward(float *img1, const float *img, int nx, int ny, int nz, int ind);
+
+/* Fused SART steps (device pointers, synchronous).  Each is one kernel that replaces a sequence
+ * of reference operator calls; the drivers are built from them.
+ *
+ * mcsart_sart_residual: fuses src/AUMISART.cu:194-209 (image sweep) and :246-263 (motion sweep).
+ *   d_rp (2*na*nb floats) receives interleaved pairs (r, n) with r = meas - P(img) and
+ *   n = P(1) when d_img2 is NULL (image sweep), else n = P(img2) (motion sweep, where img is
+ *   x - img0 so that r = meas - P(x) + P(img0) and img2 is q).  *d_norm (one device float)
+ *   receives ||r|| / ||meas||, the reference's per-view residual ratio (:209, :263). */
+int mcsart_sart_residual(float *d_rp, float *d_norm, const float *d_img, const float *d_img2,
+                         const float *d_meas, float angle, float SO, float SD, float da, int na,
+                         float ai, float db, int nb, float bi, int nx, int ny, int nz);
+
+/* mcsart_sart_backprojection: u = B(r) / B(n) from the pairs of mcsart_sart_residual, NaN/Inf
+ * mapped to 0; fuses src/AUMISART.cu:212-218 and :265-270 (two backprojections, one division). */
+int mcsart_sart_backprojection(float *d_u, const float *d_rp, float angle, float SO, float SD,
+                               float da, int na, float ai, float db, int nb, float bi, int nx,
+                               int ny, int nz);
+
+/* mcsart_motion_prep: fuses src/AUMISART.cu:252-258 and :267.  With
+ * img0 = sum over (alpha_y,D1),(alpha_x,D2),(alpha_z,D3) of add2(alpha_c, vd) plus the same with
+ * beta and fd:  d_g = img - img0  and  d_q = (D1+D2+D3)(img) * (-vd - fd).  nx == ny. */
+int mcsart_motion_prep(float *d_g, float *d_q, const float *d_img, const float *alpha_x,
+                       const float *alpha_y, const float *alpha_z, const float *beta_x,
+                       const float *beta_y, const float *beta_z, float vd, float fd, int nx,
+                       int ny, int nz);
+
+/* mcsart_motion_update: fuses src/AUMISART.cu:272-277: alpha_c += add2(u, wv, D_c img) and
+ * beta_c += add2(u, wf, D_c img) with the pairs (y,1), (x,2), (z,3).  nx == ny. */
+int mcsart_motion_update(float *alpha_x, float *alpha_y, float *alpha_z, float *beta_x,
+                         float *beta_y, float *beta_z, const float *d_u, const float *d_img,
+                         float wv, float wf, int nx, int ny, int nz);
+
+/* ---------------------------------------------------------------------------------------------
+ * 3. Drivers: host_AUMISART (include/AUMISART.h:17,19), host pointers, same argument order.
+ * ------------------------------------------------------------------------------------------- */
+
+/* Overload 1, static SART (src/AUMISART.cu:3-141).  Outputs: h_outimg (nx*ny*nz),
+ * h_outproj (na*nb*n_views, last image sweep's residuals), h_outnorm (n_iter*n_views residual
+ * ratios), h_outalphax (nx*ny*nz).  outIter, dx and err_weights are accepted and unused, as in
+ * the reference. */
+int mcsart_aumisart(float *h_outimg, float *h_outproj, float *h_outnorm, float *h_outalphax,
+                    const float *h_img, const float *h_proj, int nx, int ny, int nz, int na, int nb,
+                    int outIter, int n_views, int n_iter, const int *op_iter, float da, float db,
+                    float ai, float bi, float SO, float SD, float dx, float lambda,
+                    const float *volumes, const float *flows, const float *err_weights,
+                    const float *angles);
+
+/* Overload 2, motion-compensated SART (src/AUMISART.cu:143-313), with the six 5DCT fields. */
+int mcsart_aumisart_motion(float *h_outimg, float *h_outproj, float *h_outnorm,
+                           float *h_outalphax, const float *h_img, const float *h_proj, int nx,
+                           int ny, int nz, int na, int nb, int outIter, int n_views, int n_iter,
+                           const int *op_iter, float da, float db, float ai, float bi, float SO,
+                           float SD, float dx, float lambda, const float *volumes,
+                           const float *flows, const float *err_weights, const float *angles,
+                           const float *ax, const float *ay, const float *az, const float *bx,
+                           const float *by, const float *bz);
+
+/* ---------------------------------------------------------------------------------------------
+ * 4. Device-resident jobs (no counterpart in the reference: its driver re-uploads each measured
+ *    view and downloads each residual every view, src/AUMISART.cu:203,211).
+ * ------------------------------------------------------------------------------------------- */
+typedef struct {
+    int nx, ny, nz; /* volume; nx == ny */
+    int na, nb;     /* detector */
+    float da, db;   /* detector pitch, voxel units */
+    float ai, bi;   /* internal detector offsets */
+    float SO, SD;   /* source-isocentre and source-detector distances, voxel units */
+} mcsart_geom;
+
+typedef struct {
+    int motion;          /* 1: overload 2 (warped image sweep, view stride 1 in motion sweeps);
+                            0: overload 1 (no warps, motion sweeps step views by 10) */
+    int nonneg;          /* clamp the image at zero after every view update (not in reference) */
+    int keep_residual;   /* store each image-sweep view's residual (h_outproj) */
+    float lambda;        /* SART relaxation */
+} mcsart_opts;
+
+typedef struct mcsart_job mcsart_job;
+
+/* Bytes of device memory a job of this shape holds (admission control). */
+size_t mcsart_job_bytes(const mcsart_geom *g, int n_views, int n_iter, int keep_residual);
+
+/* Allocate a job and upload its inputs.  fields may be NULL (all six start at zero). */
+int mcsart_job_create(mcsart_ctx *ctx, const mcsart_geom *g, int n_views, const float *angles,
+                      const float *volumes, const float *flows, const mcsart_opts *opts,
+                      int n_iter, const int *op_iter, mcsart_job **job);
+int mcsart_job_set_projections(mcsart_job *job, const float *h_proj);   /* host (V*D) */
+int mcsart_job_set_projections_device(mcsart_job *job, const float *d_proj);
+int mcsart_job_set_image(mcsart_job *job, const float *h_img);          /* host, NULL -> zero */
+int mcsart_job_set_fields(mcsart_job *job, const float *const h_fields[6]); /* ax ay az bx by bz */
+/* Restore the image and fields last given to set_image / set_fields (kept resident in device
+ * memory) and clear the norms, so the same scan can be reconstructed again; enqueued on the
+ * context stream. */
+int mcsart_job_reset(mcsart_job *job);
+/* Enqueue the whole schedule (n_iter sweeps) on the context stream; returns immediately. */
+int mcsart_job_run(mcsart_job *job);
+/* Enqueue a single sweep it (0 <= it < n_iter). */
+int mcsart_job_run_sweep(mcsart_job *job, int it);
+int mcsart_job_get_image(mcsart_job *job, float *h_img);
+int mcsart_job_get_norms(mcsart_job *job, float *h_norm);                /* n_iter*n_views */
+int mcsart_job_get_residual(mcsart_job *job, float *h_proj);             /* n_views*na*nb */
+int mcsart_job_get_fields(mcsart_job *job, float *const h_fields[6]);
+/* Device pointers of the job's resident state (for zero-copy callers and tests). */
+int mcsart_job_device_image(mcsart_job *job, float **d_img);
+int mcsart_job_destroy(mcsart_job *job);
+
+/* Per-kernel timing of the last run(s) on this context, from HIP events recorded on the stream
+ * the kernels run on.  kind: 0 projector, 1 backprojector, 2 warp, 3 motion elementwise. */
+int mcsart_ctx_enable_timing(mcsart_ctx *ctx, int enable);
+int mcsart_ctx_kernel_time(mcsart_ctx *ctx, int kind, double *total_ms, long long *launches);
+int mcsart_ctx_reset_timing(mcsart_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MCSART_HIP_H */

@@ -1,0 +1,456 @@
+// Host driver: device-resident McSART jobs and the host_AUMISART drop-ins.
+//
+// Reference: both overloads of host_AUMISART (src/AUMISART.cu:3-141 static, :143-313 motion).
+// The reference allocates per call, uploads each measured view and downloads each residual every
+// view, synchronises the device after every kernel and reads two cuBLAS norms back to the host
+// per view.  A job here uploads its measured projections, image and fields once, keeps them in
+// HBM for the whole schedule, and enqueues each view as four kernels on the context's stream
+// with no host round trip: the norms are reduced on the device into a per-view array.
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "internal.hpp"
+
+using namespace mcsart;
+
+struct mcsart_job {
+    mcsart_ctx *ctx = nullptr;
+    Geom g{};
+    int V = 0;
+    int n_iter = 0;
+    std::vector<int> op_iter;
+    mcsart_opts opts{};
+    std::vector<float> angles, volumes, flows;
+    std::vector<ViewTrig> trig;
+    float *d_proj = nullptr;     // V*D measured projections
+    float *d_img = nullptr;      // N
+    float *d_fields = nullptr;   // 6N: ax ay az bx by bz
+    float *d_init = nullptr;     // 7N: initial image and fields (mcsart_job_reset)
+    float *d_w1 = nullptr;       // N: warped image (image sweep) / g (motion sweep)
+    float *d_w2 = nullptr;       // N: q then u
+    float2 *d_rp = nullptr;      // D
+    float *d_norm = nullptr;     // n_iter*V
+    float *d_resid = nullptr;    // V*D (keep_residual)
+    double *d_partials = nullptr;
+    unsigned *d_counter = nullptr;
+    void *block = nullptr;       // single allocation backing all of the above
+    size_t bytes = 0;
+};
+
+namespace {
+
+size_t align_up(size_t v) { return (v + 255) & ~(size_t)255; }
+
+struct Layout {
+    size_t proj, img, fields, init, w1, w2, rp, norm, resid, partials, counter, total;
+};
+
+Layout layout(const Geom &g, int V, int n_iter, int keep_residual) {
+    const size_t N = (size_t)g.nvox(), D = (size_t)g.ndet();
+    Layout L{};
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        const size_t o = off;
+        off += align_up(bytes);
+        return o;
+    };
+    L.proj = take(sizeof(float) * D * V);
+    L.img = take(sizeof(float) * N);
+    L.fields = take(sizeof(float) * N * 6);
+    L.init = take(sizeof(float) * N * 7);
+    L.w1 = take(sizeof(float) * N);
+    L.w2 = take(sizeof(float) * N);
+    L.rp = take(sizeof(float2) * D);
+    L.norm = take(sizeof(float) * (size_t)n_iter * V);
+    L.resid = take(keep_residual ? sizeof(float) * D * V : 0);
+    L.partials = take(sizeof(double) * 2 * (size_t)projector_partials(g));
+    L.counter = take(sizeof(unsigned));
+    L.total = off;
+    return L;
+}
+
+int check_job(mcsart_job *job) {
+    if (!job) return fail(MCSART_EINVAL, "null job");
+    return MCSART_OK;
+}
+
+Fields fields_of(const mcsart_job *j) {
+    const size_t N = (size_t)j->g.nvox();
+    const float *f = j->d_fields;
+    return Fields{f, f + N, f + 2 * N, f + 3 * N, f + 4 * N, f + 5 * N};
+}
+
+// Image sweep (src/AUMISART.cu:182-235; static form :40-65).
+int image_sweep(mcsart_job *j, int it) {
+    const Geom &g = j->g;
+    const long long N = g.nvox(), D = g.ndet();
+    hipStream_t st = j->ctx->stream;
+    const Fields f = fields_of(j);
+    const bool motion = j->opts.motion != 0;
+    int rc;
+    for (int iv = 0; iv < j->V; ++iv) {
+        const float *src = j->d_img;
+        if (motion) {   // x_i = W(x; v_i - v_0, f_i - f_0)  (:193)
+            TimedRegion tr(j->ctx, TK_WARP);
+            rc = launch_warp(j->d_w1, j->d_img, g.nx, g.ny, g.nz, j->volumes[iv] - j->volumes[0],
+                             j->flows[iv] - j->flows[0], f, st);
+            if (rc) return rc;
+            src = j->d_w1;
+        }
+        {   // r = p_i - P x_i, P(1), rho_i  (:194-211)
+            ProjArgs a{};
+            a.vol0 = src;
+            a.meas = j->d_proj + D * iv;
+            a.rp = j->d_rp;
+            a.resid_copy = j->d_resid ? j->d_resid + D * iv : nullptr;
+            a.partials = j->d_partials;
+            a.counter = j->d_counter;
+            a.norm_out = j->d_norm + (long long)it * j->V + iv;
+            TimedRegion tr(j->ctx, TK_PROJ);
+            rc = launch_projector(g, j->trig[iv], PROJ_IMAGE, a, st);
+            if (rc) return rc;
+        }
+        {   // u = B(r) / B(P1)  (:212-218)
+            TimedRegion tr(j->ctx, TK_BACK);
+            rc = launch_backprojector(g, j->trig[iv], BACK_RATIO, nullptr, j->d_rp, j->d_w2, st);
+            if (rc) return rc;
+        }
+        {   // x += lambda * W(u; v_0 - v_i, f_0 - f_i)  (:222-223)
+            TimedRegion tr(j->ctx, TK_WARP);
+            if (motion)
+                rc = launch_warp_add(j->d_img, j->d_w2, g.nx, g.ny, g.nz,
+                                     j->volumes[0] - j->volumes[iv], j->flows[0] - j->flows[iv],
+                                     f, j->opts.lambda, j->opts.nonneg, st);
+            else
+                rc = launch_add_clamp(j->d_img, j->d_w2, N, j->opts.lambda, j->opts.nonneg, st);
+            if (rc) return rc;
+        }
+    }
+    return MCSART_OK;
+}
+
+// Motion (udvf) sweep (src/AUMISART.cu:236-288; static overload :66-118 steps views by 10).
+int motion_sweep(mcsart_job *j, int it) {
+    const Geom &g = j->g;
+    const long long D = g.ndet(), N = g.nvox();
+    hipStream_t st = j->ctx->stream;
+    float *fl[6];
+    for (int c = 0; c < 6; ++c) fl[c] = j->d_fields + c * N;
+    const float *cfl[6] = {fl[0], fl[1], fl[2], fl[3], fl[4], fl[5]};
+    const int stride = j->opts.motion ? 1 : 10;
+    int rc;
+    for (int iv = 1; iv < j->V; iv += stride) {
+        const float vd = j->volumes[iv] - j->volumes[iv - 1];   // :249
+        const float fd = j->flows[iv] - j->flows[iv - 1];       // :250
+        {
+            TimedRegion tr(j->ctx, TK_MOTION);
+            rc = launch_motion_prep(j->d_w1, j->d_w2, j->d_img, cfl, g.nx, g.ny, g.nz, vd, fd, st);
+            if (rc) return rc;
+        }
+        {   // r = p_i - P(x - img0), P(q), rho_i  (:246-263)
+            ProjArgs a{};
+            a.vol0 = j->d_w1;
+            a.vol1 = j->d_w2;
+            a.meas = j->d_proj + D * iv;
+            a.rp = j->d_rp;
+            a.partials = j->d_partials;
+            a.counter = j->d_counter;
+            a.norm_out = j->d_norm + (long long)it * j->V + iv;
+            TimedRegion tr(j->ctx, TK_PROJ);
+            rc = launch_projector(g, j->trig[iv], PROJ_MOTION, a, st);
+            if (rc) return rc;
+        }
+        {   // u = B(r) / B(P q)  (:265-270)
+            TimedRegion tr(j->ctx, TK_BACK);
+            rc = launch_backprojector(g, j->trig[iv], BACK_RATIO, nullptr, j->d_rp, j->d_w2, st);
+            if (rc) return rc;
+        }
+        {   // fields += u^T * (-vd | -fd) * D_c x  (:272-277)
+            TimedRegion tr(j->ctx, TK_MOTION);
+            rc = launch_motion_update(fl, j->d_w2, j->d_img, g.nx, g.ny, g.nz,
+                                      j->volumes[iv - 1] - j->volumes[iv],
+                                      j->flows[iv - 1] - j->flows[iv], st);
+            if (rc) return rc;
+        }
+    }
+    return MCSART_OK;
+}
+
+Geom geom_of(const mcsart_geom *g) {
+    return Geom{g->nx, g->ny, g->nz, g->na, g->nb, g->da, g->db, g->ai, g->bi, g->SO, g->SD};
+}
+
+}  // namespace
+
+extern "C" size_t mcsart_job_bytes(const mcsart_geom *g, int n_views, int n_iter,
+                                   int keep_residual) {
+    if (!g || n_views <= 0 || n_iter < 0) return 0;
+    return layout(geom_of(g), n_views, n_iter, keep_residual).total;
+}
+
+extern "C" int mcsart_job_create(mcsart_ctx *ctx, const mcsart_geom *gp, int n_views,
+                                 const float *angles, const float *volumes, const float *flows,
+                                 const mcsart_opts *opts, int n_iter, const int *op_iter,
+                                 mcsart_job **out) {
+    if (!out) return fail(MCSART_EINVAL, "null job pointer");
+    *out = nullptr;
+    if (!ctx || !gp || !angles || !volumes || !flows || !opts)
+        return fail(MCSART_EINVAL, "null argument");
+    if (n_views <= 0) return fail(MCSART_EINVAL, "n_views must be positive");
+    if (n_iter < 0 || (n_iter > 0 && !op_iter)) return fail(MCSART_EINVAL, "bad op_iter");
+    const Geom g = geom_of(gp);
+    int rc = validate_geom(g);
+    if (rc) return rc;
+    rc = validate_volume(g.nx, g.ny, g.nz, true);
+    if (rc) return rc;
+    MCSART_HIP_TRY(hipSetDevice(ctx->device));
+    const Layout L = layout(g, n_views, n_iter, opts->keep_residual);
+    auto *j = new mcsart_job();
+    j->ctx = ctx;
+    j->g = g;
+    j->V = n_views;
+    j->n_iter = n_iter;
+    j->op_iter.assign(op_iter, op_iter + n_iter);
+    j->opts = *opts;
+    j->angles.assign(angles, angles + n_views);
+    j->volumes.assign(volumes, volumes + n_views);
+    j->flows.assign(flows, flows + n_views);
+    for (int i = 0; i < n_views; ++i) j->trig.push_back(view_trig(angles[i]));
+    if (hipMalloc(&j->block, L.total) != hipSuccess) {
+        (void)hipGetLastError();
+        delete j;
+        return fail(MCSART_ENOMEM, "job allocation of " + std::to_string(L.total) + " bytes failed");
+    }
+    j->bytes = L.total;
+    char *b = (char *)j->block;
+    j->d_proj = (float *)(b + L.proj);
+    j->d_img = (float *)(b + L.img);
+    j->d_fields = (float *)(b + L.fields);
+    j->d_init = (float *)(b + L.init);
+    j->d_w1 = (float *)(b + L.w1);
+    j->d_w2 = (float *)(b + L.w2);
+    j->d_rp = (float2 *)(b + L.rp);
+    j->d_norm = (float *)(b + L.norm);
+    j->d_resid = opts->keep_residual ? (float *)(b + L.resid) : nullptr;
+    j->d_partials = (double *)(b + L.partials);
+    j->d_counter = (unsigned *)(b + L.counter);
+    // everything starts at zero: image, fields, norms, residuals, reduction counter
+    if (hipMemsetAsync(j->block, 0, L.total, ctx->stream) != hipSuccess ||
+        hipStreamSynchronize(ctx->stream) != hipSuccess) {
+        hipFree(j->block);
+        delete j;
+        return fail(MCSART_EHIP, "job initialisation failed");
+    }
+    *out = j;
+    return MCSART_OK;
+}
+
+extern "C" int mcsart_job_set_projections(mcsart_job *j, const float *h_proj) {
+    int rc = check_job(j);
+    if (rc) return rc;
+    if (!h_proj) return fail(MCSART_EINVAL, "null projections");
+    MCSART_HIP_TRY(hipMemcpyAsync(j->d_proj, h_proj, sizeof(float) * j->g.ndet() * j->V,
+                                  hipMemcpyHostToDevice, j->ctx->stream));
+    MCSART_HIP_TRY(hipStreamSynchronize(j->ctx->stream));
+    return MCSART_OK;
+}
+
+extern "C" int mcsart_job_set_projections_device(mcsart_job *j, const float *d_proj) {
+    int rc = check_job(j);
+    if (rc) return rc;
+    if (!d_proj) return fail(MCSART_EINVAL, "null projections");
+    MCSART_HIP_TRY(hipMemcpyAsync(j->d_proj, d_proj, sizeof(float) * j->g.ndet() * j->V,
+                                  hipMemcpyDeviceToDevice, j->ctx->stream));
+    return MCSART_OK;
+}
+
+extern "C" int mcsart_job_set_image(mcsart_job *j, const float *h_img) {
+    int rc = check_job(j);
+    if (rc) return rc;
+    const size_t bytes = sizeof(float) * j->g.nvox();
+    if (h_img)
+        MCSART_HIP_TRY(hipMemcpyAsync(j->d_init, h_img, bytes, hipMemcpyHostToDevice, j->ctx->stream));
+    else
+        MCSART_HIP_TRY(hipMemsetAsync(j->d_init, 0, bytes, j->ctx->stream));
+    MCSART_HIP_TRY(hipMemcpyAsync(j->d_img, j->d_init, bytes, hipMemcpyDeviceToDevice, j->ctx->stream));
+    MCSART_HIP_TRY(hipStreamSynchronize(j->ctx->stream));
+    return MCSART_OK;
+}
+
+extern "C" int mcsart_job_set_fields(mcsart_job *j, const float *const h_fields[6]) {
+    int rc = check_job(j);
+    if (rc) return rc;
+    const size_t N = (size_t)j->g.nvox();
+    float *init = j->d_init + N;
+    for (int c = 0; c < 6; ++c) {
+        if (h_fields && h_fields[c])
+            MCSART_HIP_TRY(hipMemcpyAsync(init + c * N, h_fields[c], sizeof(float) * N,
+                                          hipMemcpyHostToDevice, j->ctx->stream));
+        else
+            MCSART_HIP_TRY(hipMemsetAsync(init + c * N, 0, sizeof(float) * N, j->ctx->stream));
+    }
+    MCSART_HIP_TRY(hipMemcpyAsync(j->d_fields, init, sizeof(float) * N * 6,
+                                  hipMemcpyDeviceToDevice, j->ctx->stream));
+    MCSART_HIP_TRY(hipStreamSynchronize(j->ctx->stream));
+    return MCSART_OK;
+}
+
+extern "C" int mcsart_job_reset(mcsart_job *j) {
+    int rc = check_job(j);
+    if (rc) return rc;
+    const size_t N = (size_t)j->g.nvox();
+    hipStream_t st = j->ctx->stream;
+    MCSART_HIP_TRY(hipMemcpyAsync(j->d_img, j->d_init, sizeof(float) * N, hipMemcpyDeviceToDevice, st));
+    MCSART_HIP_TRY(hipMemcpyAsync(j->d_fields, j->d_init + N, sizeof(float) * N * 6,
+                                  hipMemcpyDeviceToDevice, st));
+    rc = launch_fill(j->d_norm, (long long)j->n_iter * j->V, 0.f, st);
+    if (rc) return rc;
+    if (j->d_resid) rc = launch_fill(j->d_resid, j->g.ndet() * j->V, 0.f, st);
+    return rc;
+}
+
+extern "C" int mcsart_job_run_sweep(mcsart_job *j, int it) {
+    int rc = check_job(j);
+    if (rc) return rc;
+    if (it < 0 || it >= j->n_iter) return fail(MCSART_EINVAL, "sweep index out of range");
+    return (j->op_iter[it] == 1) ? image_sweep(j, it) : motion_sweep(j, it);
+}
+
+extern "C" int mcsart_job_run(mcsart_job *j) {
+    int rc = check_job(j);
+    if (rc) return rc;
+    for (int it = 0; it < j->n_iter; ++it) {
+        rc = mcsart_job_run_sweep(j, it);
+        if (rc) return rc;
+    }
+    return MCSART_OK;
+}
+
+extern "C" int mcsart_job_get_image(mcsart_job *j, float *h_img) {
+    int rc = check_job(j);
+    if (rc) return rc;
+    MCSART_HIP_TRY(hipStreamSynchronize(j->ctx->stream));
+    MCSART_HIP_TRY(hipMemcpy(h_img, j->d_img, sizeof(float) * j->g.nvox(), hipMemcpyDeviceToHost));
+    return MCSART_OK;
+}
+
+extern "C" int mcsart_job_get_norms(mcsart_job *j, float *h_norm) {
+    int rc = check_job(j);
+    if (rc) return rc;
+    MCSART_HIP_TRY(hipStreamSynchronize(j->ctx->stream));
+    MCSART_HIP_TRY(hipMemcpy(h_norm, j->d_norm, sizeof(float) * (size_t)j->n_iter * j->V,
+                             hipMemcpyDeviceToHost));
+    return MCSART_OK;
+}
+
+extern "C" int mcsart_job_get_residual(mcsart_job *j, float *h_proj) {
+    int rc = check_job(j);
+    if (rc) return rc;
+    if (!j->d_resid) return fail(MCSART_EINVAL, "job was created without keep_residual");
+    MCSART_HIP_TRY(hipStreamSynchronize(j->ctx->stream));
+    MCSART_HIP_TRY(hipMemcpy(h_proj, j->d_resid, sizeof(float) * j->g.ndet() * j->V,
+                             hipMemcpyDeviceToHost));
+    return MCSART_OK;
+}
+
+extern "C" int mcsart_job_get_fields(mcsart_job *j, float *const h_fields[6]) {
+    int rc = check_job(j);
+    if (rc) return rc;
+    MCSART_HIP_TRY(hipStreamSynchronize(j->ctx->stream));
+    const size_t N = (size_t)j->g.nvox();
+    for (int c = 0; c < 6; ++c)
+        if (h_fields[c])
+            MCSART_HIP_TRY(hipMemcpy(h_fields[c], j->d_fields + c * N, sizeof(float) * N,
+                                     hipMemcpyDeviceToHost));
+    return MCSART_OK;
+}
+
+extern "C" int mcsart_job_device_image(mcsart_job *j, float **d_img) {
+    int rc = check_job(j);
+    if (rc) return rc;
+    if (!d_img) return fail(MCSART_EINVAL, "null pointer");
+    *d_img = j->d_img;
+    return MCSART_OK;
+}
+
+extern "C" int mcsart_job_destroy(mcsart_job *j) {
+    if (!j) return MCSART_OK;
+    hipSetDevice(j->ctx->device);
+    hipStreamSynchronize(j->ctx->stream);
+    hipFree(j->block);
+    delete j;
+    return MCSART_OK;
+}
+
+// ---------------------------------------------------------------------------------------------
+// host_AUMISART drop-ins (include/AUMISART.h:17,19)
+// ---------------------------------------------------------------------------------------------
+namespace {
+
+int run_aumisart(float *h_outimg, float *h_outproj, float *h_outnorm, float *h_outalphax,
+                 const float *h_img, const float *h_proj, int nx, int ny, int nz, int na, int nb,
+                 int n_views, int n_iter, const int *op_iter, float da, float db, float ai,
+                 float bi, float SO, float SD, float lambda, const float *volumes,
+                 const float *flows, const float *angles, const float *const fields[6],
+                 int motion) {
+    if (!h_outimg || !h_outproj || !h_outnorm || !h_outalphax || !h_img || !h_proj || !volumes ||
+        !flows || !angles)
+        return fail(MCSART_EINVAL, "null host pointer");
+    mcsart_ctx *ctx = default_ctx();
+    if (!ctx) return fail(MCSART_EHIP, "no HIP device / context");
+    const mcsart_geom g{nx, ny, nz, na, nb, da, db, ai, bi, SO, SD};
+    mcsart_opts o{};
+    o.motion = motion;
+    o.nonneg = 0;   // the reference applies no clamp (src/AUMISART.cu:62,223)
+    o.keep_residual = 1;
+    o.lambda = lambda;
+    mcsart_job *j = nullptr;
+    int rc = mcsart_job_create(ctx, &g, n_views, angles, volumes, flows, &o, n_iter, op_iter, &j);
+    if (rc) return rc;
+    if (!(rc = mcsart_job_set_projections(j, h_proj)) && !(rc = mcsart_job_set_image(j, h_img)) &&
+        !(rc = mcsart_job_set_fields(j, fields)) && !(rc = mcsart_job_run(j)) &&
+        !(rc = mcsart_job_get_image(j, h_outimg)) && !(rc = mcsart_job_get_norms(j, h_outnorm)) &&
+        !(rc = mcsart_job_get_residual(j, h_outproj))) {
+        float *fo[6] = {h_outalphax, nullptr, nullptr, nullptr, nullptr, nullptr};
+        rc = mcsart_job_get_fields(j, fo);
+    }
+    mcsart_job_destroy(j);
+    return rc;
+}
+
+}  // namespace
+
+extern "C" int mcsart_aumisart(float *h_outimg, float *h_outproj, float *h_outnorm,
+                               float *h_outalphax, const float *h_img, const float *h_proj, int nx,
+                               int ny, int nz, int na, int nb, int outIter, int n_views,
+                               int n_iter, const int *op_iter, float da, float db, float ai,
+                               float bi, float SO, float SD, float dx, float lambda,
+                               const float *volumes, const float *flows, const float *err_weights,
+                               const float *angles) {
+    (void)outIter;
+    (void)dx;
+    (void)err_weights;   // parsed but unused by the reference
+    return run_aumisart(h_outimg, h_outproj, h_outnorm, h_outalphax, h_img, h_proj, nx, ny, nz,
+                        na, nb, n_views, n_iter, op_iter, da, db, ai, bi, SO, SD, lambda, volumes,
+                        flows, angles, nullptr, 0);
+}
+
+extern "C" int mcsart_aumisart_motion(float *h_outimg, float *h_outproj, float *h_outnorm,
+                                      float *h_outalphax, const float *h_img, const float *h_proj,
+                                      int nx, int ny, int nz, int na, int nb, int outIter,
+                                      int n_views, int n_iter, const int *op_iter, float da,
+                                      float db, float ai, float bi, float SO, float SD, float dx,
+                                      float lambda, const float *volumes, const float *flows,
+                                      const float *err_weights, const float *angles,
+                                      const float *ax, const float *ay, const float *az,
+                                      const float *bx, const float *by, const float *bz) {
+    (void)outIter;
+    (void)dx;
+    (void)err_weights;
+    if (!ax || !ay || !az || !bx || !by || !bz) return fail(MCSART_EINVAL, "null field");
+    const float *const f[6] = {ax, ay, az, bx, by, bz};
+    return run_aumisart(h_outimg, h_outproj, h_outnorm, h_outalphax, h_img, h_proj, nx, ny, nz,
+                        na, nb, n_views, n_iter, op_iter, da, db, ai, bi, SO, SD, lambda, volumes,
+                        flows, angles, f, 1);
+}

@@ -1,0 +1,179 @@
+// Footprint geometry of the projector and backprojector, shared by the device kernels.
+//
+// The reference forms footprint bounds as ratio*xc + y1 + ny/2 with xc ~ SO (src/cu_projection.cu:
+// 52-61) and a/da - ai + 0.5 with a ~ na*da/2 (src/cu_backprojection.cu:135-138), in FP32.  Both are
+// badly conditioned: the terms reach ~10^3 and cancel.  Here the same quantities are rearranged
+// into per-ray / per-column constants evaluated in FP64, so that the per-plane / per-voxel work in
+// FP32 only touches well-scaled numbers.  The mathematics is identical to the reference's.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <math.h>
+
+namespace mcsart {
+
+struct GeomArgs {
+    int nx, ny, nz, na, nb;
+    float da, db, ai, bi, SO, SD;
+};
+
+// ------------------------------------------------------------------------------------------
+// Projector ray (src/cu_projection.cu:22-77).
+//
+// A ray from source S=(-SO c, -SO s, 0) to the detector cell centre.  If |x21| > |y21| (:39) the
+// ray steps through x-planes ix and its footprint in each plane spans, along y, the min..max of
+// four lines through the source with slopes yl/xl, yl/xr, yr/xl, yr/xr (:48-64), and along z
+// the two lines with slopes zl/x21, zr/x21 (:67-77); otherwise the roles of x and y swap
+// (:93-130).  With p = ip + 0.5 - n_plane/2 the plane centre offset, the plane's distance from
+// the source is xc = p + SO*c (x-branch) or p + SO*s (y-branch), and a line of slope r gives the
+// bound r*xc + (y1 + ny/2) = r*p + K with K = r*SO*c + y1 + ny/2.  The min/max over four slopes
+// of r*xc equals the min/max over the two extreme slopes, for either sign of xc.
+// ------------------------------------------------------------------------------------------
+struct Ray {
+    int xbranch;       // 1: planes are x-planes (transverse axis y)
+    float rA, KA;      // transverse bound A: t = rA*p + KA
+    float rB, KB;      // transverse bound B
+    float rzA, KzA;    // z bound A
+    float rzB, KzB;    // z bound B
+};
+
+__host__ __device__ inline void ray_setup(const GeomArgs &g, double c, double s, int ia, int ib,
+                                          Ray &R) {
+    const double SO = g.SO, SD = g.SD, da = g.da, db = g.db;
+    const double x1 = -SO * c, y1 = -SO * s;                       // :25-26
+    const double x20 = SD - SO;                                     // :28
+    const double y20 = ((double)ia + (double)g.ai) * da;            // :29
+    const double x2 = x20 * c - y20 * s, y2 = x20 * s + y20 * c;    // :30-31
+    const double z2 = ((double)ib + (double)g.bi) * db;             // :32
+    const double x21 = x2 - x1, y21 = y2 - y1, z21 = z2;            // :34-36
+    const double hda = da / 2, hdb = db / 2;
+    const double xl = x21 - hda * s, xr = x21 + hda * s;            // :48-49
+    const double yl = y21 - hda * c, yr = y21 + hda * c;            // :50-51
+    const double zl = z21 - hdb, zr = z21 + hdb;                    // :67-68
+    double r0, r1, r2, r3, rz0, rz1, dist, base, zbase;
+    R.xbranch = fabs(x21) > fabs(y21) ? 1 : 0;                      // :39
+    if (R.xbranch) {
+        r0 = yl / xl; r1 = yl / xr; r2 = yr / xl; r3 = yr / xr;     // :54-61
+        rz0 = zl / x21; rz1 = zr / x21;                              // :71-74
+        dist = SO * c;                                               // xc = p - x1
+        base = y1 + (double)(g.ny / 2);                              // integer ny/2
+    } else {
+        r0 = xl / yl; r1 = xl / yr; r2 = xr / yl; r3 = xr / yr;     // :107-114
+        rz0 = zl / y21; rz1 = zr / y21;                              // :124-127
+        dist = SO * s;                                               // yc = p - y1
+        base = x1 + (double)(g.nx / 2);                              // integer nx/2
+    }
+    const double rmin = fmin(fmin(r0, r1), fmin(r2, r3));
+    const double rmax = fmax(fmax(r0, r1), fmax(r2, r3));
+    zbase = (double)(g.nz / 2);                                      // integer nz/2
+    R.rA = (float)rmin;
+    R.KA = (float)(rmin * dist + base);
+    R.rB = (float)rmax;
+    R.KB = (float)(rmax * dist + base);
+    R.rzA = (float)rz0;
+    R.KzA = (float)(rz0 * dist + zbase);
+    R.rzB = (float)rz1;
+    R.KzB = (float)(rz1 * dist + zbase);
+}
+
+// ------------------------------------------------------------------------------------------
+// Backprojector column (src/cu_backprojection.cu:62-138).
+//
+// A voxel at centre (xc, yc, zc) (integer n/2, :66-68) is rotated by -phi (:82-97) into
+// (xr0, yr0); six points (xr0 +-0.5, yr0 +-0.5, zc) and (xr0, yr0, zc +- 0.5) are mapped to the
+// detector with ratio = SD/(x + SO) (:105-122).  The transverse (a) bounds depend only on the
+// (ix, iy) column; the axial (b) bounds are the extremes of four lines in zc with slopes
+// Rm = SD/(xr0-0.5+SO), Rp = SD/(xr0+0.5+SO) and Rc = SD/(xr0+SO) (the latter at zc +- 0.5).
+// ------------------------------------------------------------------------------------------
+struct Column {
+    double Amin, Amax;   // a bounds in cell coordinates: a/da - ai + 0.5 (:135-136)
+    double qm, qp, qc;   // b slopes divided by db
+    double C0;           // 0.5 - bi
+};
+
+__host__ __device__ inline void column_setup(const GeomArgs &g, double c, double s, int ix, int iy,
+                                             Column &C) {
+    const double SO = g.SO, SD = g.SD;
+    const double xc = (double)ix - (double)(g.nx / 2) + 0.5;   // :66
+    const double yc = (double)iy - (double)(g.ny / 2) + 0.5;   // :67
+    const double xr0 = xc * c + yc * s;                        // :92
+    const double yr0 = -xc * s + yc * c;                       // :93
+    const double Rm = SD / (xr0 - 0.5 + SO);                   // ll, lr (:105,:111)
+    const double Rp = SD / (xr0 + 0.5 + SO);                   // rr, rl (:108,:114)
+    const double Rc = SD / (xr0 + SO);                         // t, b (:117,:120)
+    const double a_ll = Rm * (yr0 - 0.5), a_lr = Rm * (yr0 + 0.5);
+    const double a_rr = Rp * (yr0 + 0.5), a_rl = Rp * (yr0 - 0.5);
+    const double a_tb = Rc * yr0;
+    const double amax = fmax(fmax(fmax(a_ll, a_lr), fmax(a_rr, a_rl)), a_tb);   // :129
+    const double amin = fmin(fmin(fmin(a_ll, a_lr), fmin(a_rr, a_rl)), a_tb);   // :130
+    const double da = g.da, db = g.db;
+    C.Amax = amax / da - (double)g.ai + 0.5;
+    C.Amin = amin / da - (double)g.ai + 0.5;
+    C.qm = Rm / db;
+    C.qp = Rp / db;
+    C.qc = Rc / db;
+    C.C0 = 0.5 - (double)g.bi;
+}
+
+// b bounds of voxel iz of a column: b/db - bi + 0.5 over the six points (:131-132,:137-138).
+__host__ __device__ inline void column_b(const Column &C, int nz, int iz, double &Bmin,
+                                         double &Bmax) {
+    const double zc = (double)iz - (double)(nz / 2) + 0.5;   // :68
+    const double bm = C.qm * zc, bp = C.qp * zc;
+    const double bt = C.qc * (zc + 0.5), bb = C.qc * (zc - 0.5);
+    Bmax = fmax(fmax(bm, bp), fmax(bt, bb)) + C.C0;
+    Bmin = fmin(fmin(bm, bp), fmin(bt, bb)) + C.C0;
+}
+
+}  // namespace mcsart
+
+namespace mcsart {
+
+// ------------------------------------------------------------------------------------------
+// One plane of one ray: the footprint [t1, t2] x [z1, z2] in voxel-index units and the
+// normalised overlap weights of src/cu_projection.cu:81-89 (x-branch) / :134-142 (y-branch):
+// w(i) = (min(i+1, hi) - max(i, lo)) / (hi - lo) for i in [max(0, floor(lo)), min(n-1, floor(hi))].
+// When the footprint spans at most two voxels along both axes (the common case), the weights of
+// the two candidate voxels are returned directly (wt*, wz* already masked to zero for voxels
+// outside the volume) and `fast` is set; otherwise the caller loops over the ranges.
+// ------------------------------------------------------------------------------------------
+struct PlaneFP {
+    float t1, t2, z1, z2;   // bounds
+    float it, iz;           // 1/(t2-t1), 1/(z2-z1)
+    int T0, Z0;             // floor(t1), floor(z1)
+    float wt0, wt1, wz0, wz1;
+    bool fast;
+};
+
+__host__ __device__ inline bool plane_fp(const Ray &R, float p, int nt, int nz, PlaneFP &F) {
+    const float ta = fmaf(R.rA, p, R.KA), tb = fmaf(R.rB, p, R.KB);
+    const float za = fmaf(R.rzA, p, R.KzA), zb = fmaf(R.rzB, p, R.KzB);
+    F.t1 = fminf(ta, tb);
+    F.t2 = fmaxf(ta, tb);
+    F.z1 = fminf(za, zb);
+    F.z2 = fmaxf(za, zb);
+    if (!(F.t2 > 0.f && F.t1 < (float)nt && F.z2 > 0.f && F.z1 < (float)nz)) return false;
+    const float ft = floorf(F.t1), fz = floorf(F.z1);
+    // t1 < nt and z1 < nz here; clamping the lower side keeps every float->int conversion in
+    // range.  The fast path needs ft >= -1 (t2 > 0 and t2 < ft + 2), where T0 == ft exactly.
+    F.T0 = (int)fmaxf(ft, -2.f);
+    F.Z0 = (int)fmaxf(fz, -2.f);
+    F.it = 1.0f / (F.t2 - F.t1);
+    F.iz = 1.0f / (F.z2 - F.z1);
+    const float wt0 = fminf(ft + 1.f, F.t2) - F.t1;
+    const float wt1 = F.t2 - (ft + 1.f);
+    const float wz0 = fminf(fz + 1.f, F.z2) - F.z1;
+    const float wz1 = F.z2 - (fz + 1.f);
+    F.fast = (wt1 < 1.f) && (wz1 < 1.f);
+    const bool vt0 = (F.T0 >= 0) && (F.T0 < nt), vt1 = (F.T0 >= -1) && (F.T0 + 1 < nt);
+    const bool vz0 = (F.Z0 >= 0) && (F.Z0 < nz), vz1 = (F.Z0 >= -1) && (F.Z0 + 1 < nz);
+    F.wt0 = vt0 ? wt0 : 0.f;
+    F.wt1 = vt1 ? fmaxf(wt1, 0.f) : 0.f;
+    F.wz0 = vz0 ? wz0 : 0.f;
+    F.wz1 = vz1 ? fmaxf(wz1, 0.f) : 0.f;
+    return true;
+}
+
+__host__ __device__ inline int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+}  // namespace mcsart

@@ -1,0 +1,159 @@
+// Internal declarations shared by the libmcsart_hip translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "mcsart_hip.h"
+
+namespace mcsart {
+
+// ---- error handling -------------------------------------------------------------------------
+void set_error(const std::string &msg);
+int fail(int status, const std::string &msg);
+
+#define MCSART_HIP_TRY(expr)                                                              \
+    do {                                                                                  \
+        hipError_t e_ = (expr);                                                           \
+        if (e_ != hipSuccess)                                                             \
+            return ::mcsart::fail(MCSART_EHIP, std::string(#expr) + ": " +                \
+                                                   hipGetErrorString(e_));                \
+    } while (0)
+
+#define MCSART_CHECK_LAUNCH()                                                             \
+    do {                                                                                  \
+        hipError_t e_ = hipGetLastError();                                                \
+        if (e_ != hipSuccess)                                                             \
+            return ::mcsart::fail(MCSART_EHIP, std::string("kernel launch: ") +           \
+                                                   hipGetErrorString(e_));                \
+    } while (0)
+
+// ---- geometry -------------------------------------------------------------------------------
+struct Geom {
+    int nx, ny, nz, na, nb;
+    float da, db, ai, bi, SO, SD;
+    long long nvox() const { return (long long)nx * ny * nz; }
+    long long ndet() const { return (long long)na * nb; }
+};
+
+int validate_geom(const Geom &g);
+int validate_volume(int nx, int ny, int nz, bool need_square);
+
+// Per-view trigonometry, evaluated once on the host in FP64 from the float angle the reference
+// passes (src/cu_projection.cu:23-24, src/cu_backprojection.cu:62-63).
+struct ViewTrig {
+    double c, s;
+};
+ViewTrig view_trig(float angle);
+
+// ---- projector ------------------------------------------------------------------------------
+enum ProjMode {
+    PROJ_PLAIN = 0,   // out[id] = P(x)
+    PROJ_IMAGE = 1,   // rp[id] = (p[id] - P(x), P(1)), norms of p and r
+    PROJ_MOTION = 2,  // rp[id] = (p[id] - P(g), P(q)), norms of p and r
+};
+
+struct ProjArgs {
+    const float *vol0;     // x (PLAIN/IMAGE) or g (MOTION)
+    const float *vol1;     // q (MOTION)
+    const float *meas;     // measured projection of this view (IMAGE/MOTION)
+    float *out;            // PLAIN: D floats
+    float2 *rp;            // IMAGE/MOTION: D float2 (residual, normaliser projection)
+    float *resid_copy;     // optional: residual copy (reference h_outproj slice)
+    double *partials;      // per-block (sum p^2, sum r^2)
+    unsigned *counter;     // last-block-done counter (left at zero)
+    float *norm_out;       // ||r|| / ||p|| destination
+};
+int launch_projector(const Geom &g, const ViewTrig &t, ProjMode mode, const ProjArgs &a,
+                     hipStream_t st);
+int projector_partials(const Geom &g);   // number of per-block partial slots needed
+
+// ---- backprojector --------------------------------------------------------------------------
+enum BackMode {
+    BACK_PLAIN = 0,  // out = B(proj)
+    BACK_RATIO = 1,  // out = B(rp.x) / B(rp.y), NaN/Inf -> 0
+};
+int launch_backprojector(const Geom &g, const ViewTrig &t, BackMode mode, const float *proj,
+                         const float2 *rp, float *out, hipStream_t st);
+
+// ---- warps and elementwise ------------------------------------------------------------------
+struct Fields {
+    const float *ax, *ay, *az, *bx, *by, *bz;
+};
+// out = W(src; vol, flow); out must not alias src.
+int launch_warp(float *out, const float *src, int nx, int ny, int nz, float vol, float flow,
+                const Fields &f, hipStream_t st);
+// x += lambda * W(u; vol, flow), then optional clamp at zero.
+int launch_warp_add(float *x, const float *u, int nx, int ny, int nz, float vol, float flow,
+                    const Fields &f, float lambda, int nonneg, hipStream_t st);
+int launch_deform2(float *out, const float *src, int nx, int ny, int nz, float vol, float flow,
+                   const Fields &f, hipStream_t st);
+// x += lambda * u, then optional clamp at zero.
+int launch_add_clamp(float *x, const float *u, long long n, float lambda, int nonneg,
+                     hipStream_t st);
+int launch_add(float *a, const float *b, long long n, float w, hipStream_t st);
+int launch_add2(float *a, const float *b, int nx, int ny, int nz, const float *c, float w,
+                int ind, hipStream_t st);
+int launch_fill(float *a, long long n, float v, hipStream_t st);
+int launch_initial2(float *img, int nx, int ny, int nz, const float *img0, float vol, float flow,
+                    hipStream_t st);
+int launch_division(float *a, const float *b, long long n, hipStream_t st);
+int launch_diff(float *out, const float *img, int nx, int ny, int nz, int ind, int backward,
+                hipStream_t st);
+int launch_update_udvf(float *ax, float *ay, float *az, float *bx, float *by, float *bz,
+                       const float *img, const float *img0, float vd, float fd, int nx, int ny,
+                       int nz, hipStream_t st);
+// Motion sweep, fused (src/AUMISART.cu:246-277):
+//   g = x - sum_c (alpha_c^T vd + beta_c^T fd) * D_c x,  q = (D1+D2+D3)x * (-vd - fd)
+int launch_motion_prep(float *g_out, float *q_out, const float *x, const float *const fields[6],
+                       int nx, int ny, int nz, float vd, float fd, hipStream_t st);
+//   alpha_c += u^T * wv * D_c x,  beta_c += u^T * wf * D_c x
+int launch_motion_update(float *const fields[6], const float *u, const float *x, int nx, int ny,
+                         int nz, float wv, float wf, hipStream_t st);
+
+// ---- contexts -------------------------------------------------------------------------------
+enum TimingKind { TK_PROJ = 0, TK_BACK = 1, TK_WARP = 2, TK_MOTION = 3, TK_COUNT = 4 };
+
+}  // namespace mcsart
+
+struct mcsart_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    // timing (events recorded around kernels on `stream`)
+    bool timing = false;
+    struct Pending {
+        int kind;
+        hipEvent_t a, b;
+    };
+    std::vector<Pending> pending;
+    std::vector<hipEvent_t> event_pool;
+    double total_ms[mcsart::TK_COUNT] = {0, 0, 0, 0};
+    long long launches[mcsart::TK_COUNT] = {0, 0, 0, 0};
+    // scratch for the synchronous operator shims
+    void *scratch = nullptr;
+    size_t scratch_bytes = 0;
+    // norm-reduction scratch (per-block partials + a counter that every launch leaves at zero)
+    void *red = nullptr;
+    size_t red_bytes = 0;
+};
+
+namespace mcsart {
+mcsart_ctx *default_ctx();   // per-device context behind the synchronous operators
+int ctx_scratch(mcsart_ctx *ctx, size_t bytes, void **ptr);
+// Reduction scratch for the projector's SART modes: 2*nblocks doubles of partials followed by
+// the last-block counter, zero-filled when (re)allocated.
+int ctx_reduction(mcsart_ctx *ctx, int nblocks, double **partials, unsigned **counter);
+// Timing helpers: bracket a launch with events when ctx->timing is on.
+struct TimedRegion {
+    mcsart_ctx *ctx;
+    int kind;
+    hipEvent_t a = nullptr, b = nullptr;
+    TimedRegion(mcsart_ctx *c, int k);
+    ~TimedRegion();
+};
+int ctx_collect_timing(mcsart_ctx *ctx);
+}  // namespace mcsart

@@ -1,0 +1,282 @@
+"""McSART drivers: the ``entry_AUMISART`` gateway mirror and device-resident jobs.
+
+``entry_aumisart(img0, proj, geo, it)`` follows ``src/entry_AUMISART.cu``: it parses GEO_PARA /
+ITER_PARA dictionaries with the reference's field names, aliases and defaults (:18-160), picks the
+static or motion-compensated overload by the presence of ``alpha_x`` (:193-207) and returns
+``(img, nrm, alpha_x, residual_proj)`` (:172-190).  ``Job`` wraps the library's device-resident
+job API for callers that keep a scan on the GPU (benchmarks, batching, serving).
+"""
+from __future__ import annotations
+
+import ctypes
+from typing import Mapping, Optional, Sequence
+
+import numpy as np
+
+from ._lib import McsartError, check, load
+from .geometry import Geometry
+
+
+class MexBadInput(ValueError):
+    """Raised where the reference calls mexErrMsgIdAndTxt("MATLAB:badInput", ...)."""
+
+
+def _get(d: Mapping, *names):
+    for n in names:
+        if n in d and d[n] is not None:
+            return d[n]
+    return None
+
+
+def parse_geo(geo: Mapping) -> Geometry:
+    """GEO_PARA -> Geometry (src/entry_AUMISART.cu:18-108)."""
+    vals = {}
+    for k in ("nx", "ny", "nz"):
+        v = _get(geo, k)
+        if v is None:
+            raise MexBadInput(f"Can't found valid volume resolution {k}.")
+        vals[k] = int(v)
+    na = _get(geo, "na", "nv")
+    if na is None:
+        raise MexBadInput("Can't found valid number of detector in plane, which is denoted as na or nu.")
+    nb = _get(geo, "nb", "nu")
+    if nb is None:
+        raise MexBadInput("Can't found valid number of detector across plane, which is denoted as nb or nv.")
+    da = _get(geo, "da")
+    db = _get(geo, "db")
+    SO = _get(geo, "SO", "SI")
+    if SO is None:
+        raise MexBadInput("Can't found valid distance between source and isocenter, which is denoted with para.SO or para.DI.")
+    SD = _get(geo, "SD")
+    if SD is None:
+        DI = _get(geo, "DI")
+        if DI is None:
+            raise MexBadInput("Can't found valid distance between source and detector plane, which is denoted with para.SD or para.SI + para.DI.")
+        SD = np.float32(DI) + np.float32(SO)
+    return Geometry.from_user(vals["nx"], vals["ny"], vals["nz"], int(na), int(nb),
+                              da=1.0 if da is None else da, db=1.0 if db is None else db,
+                              ai=_get(geo, "ai"), bi=_get(geo, "bi"), SO=SO, SD=SD)
+
+
+def parse_iter(it: Mapping):
+    """ITER_PARA (src/entry_AUMISART.cu:111-160)."""
+    n_iter = int(_get(it, "n_iter") or 1)
+    op = _get(it, "op_iter")
+    if op is None:
+        raise MexBadInput("Can't found valid iteration distribution as iter_para.op_iter.")
+    n_views = _get(it, "n_views")
+    if n_views is None:
+        raise MexBadInput("Can't found valid number bins, which is denoted as iter_para.n_views.")
+    lam = _get(it, "lambda")
+    if lam is None:
+        raise MexBadInput("Can't found valid coefficience iter_para.lambda.")
+    if _get(it, "outIter") is None:
+        raise MexBadInput("Can't found valid coefficience iter_para.outIter.")
+    out = {}
+    for k, msg in (("volumes", "volume in iter_para.volumes"), ("flows", "flow in iter_para.flows"),
+                   ("err_weights", "flow in iter_para.err_weights"),
+                   ("angles", "angles iter_para.angles")):
+        v = _get(it, k)
+        if v is None:
+            raise MexBadInput(f"Can't found valid {msg}.")
+        out[k] = np.ascontiguousarray(v, dtype=np.float32)
+    op = np.ascontiguousarray(op, dtype=np.int32)
+    if op.size < n_iter:
+        raise MexBadInput("op_iter has fewer than n_iter entries")
+    return dict(n_iter=n_iter, op_iter=op[:n_iter], n_views=int(n_views), lam=float(lam),
+                outIter=int(_get(it, "outIter")), **out)
+
+
+def entry_aumisart(img0: np.ndarray, proj: np.ndarray, geo: Mapping, it: Mapping):
+    """``[img, nrm, alpha_x, res] = entry_AUMISART(img0, proj, geo, iter)``."""
+    g = parse_geo(geo)
+    p = parse_iter(it)
+    V = p["n_views"]
+    lib = load()
+    img0 = np.ascontiguousarray(img0, dtype=np.float32).reshape(g.vol_shape)
+    proj = np.ascontiguousarray(proj, dtype=np.float32).reshape((V,) + g.det_shape)
+    out_img = np.empty(g.vol_shape, np.float32)
+    out_norm = np.empty(p["n_iter"] * V, np.float32)
+    out_ax = np.empty(g.vol_shape, np.float32)
+    out_proj = np.empty((V,) + g.det_shape, np.float32)
+    args = [out_img.ctypes.data, out_proj.ctypes.data, out_norm.ctypes.data, out_ax.ctypes.data,
+            img0.ctypes.data, proj.ctypes.data, g.nx, g.ny, g.nz, g.na, g.nb, p["outIter"], V,
+            p["n_iter"], p["op_iter"].ctypes.data, g.da, g.db, g.ai, g.bi, g.SO, g.SD,
+            float(_get(geo, "dx") or 1.0), p["lam"], p["volumes"].ctypes.data,
+            p["flows"].ctypes.data, p["err_weights"].ctypes.data, p["angles"].ctypes.data]
+    if _get(it, "alpha_x") is None:
+        check(lib.mcsart_aumisart(*args), "mcsart_aumisart")
+    else:
+        fs = [np.ascontiguousarray(it[k], dtype=np.float32).reshape(g.vol_shape)
+              for k in ("alpha_x", "alpha_y", "alpha_z", "beta_x", "beta_y", "beta_z")]
+        check(lib.mcsart_aumisart_motion(*args, *[f.ctypes.data for f in fs]),
+              "mcsart_aumisart_motion")
+    return out_img, out_norm, out_ax, out_proj
+
+
+# ---- device-resident jobs -------------------------------------------------------------------
+class CGeom(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int) for n in ("nx", "ny", "nz", "na", "nb")] + \
+               [(n, ctypes.c_float) for n in ("da", "db", "ai", "bi", "SO", "SD")]
+
+    @staticmethod
+    def of(g: Geometry) -> "CGeom":
+        return CGeom(g.nx, g.ny, g.nz, g.na, g.nb, g.da, g.db, g.ai, g.bi, g.SO, g.SD)
+
+
+class COpts(ctypes.Structure):
+    _fields_ = [("motion", ctypes.c_int), ("nonneg", ctypes.c_int),
+                ("keep_residual", ctypes.c_int), ("lam", ctypes.c_float)]
+
+
+TIMING_KINDS = {"projector": 0, "backprojector": 1, "warp": 2, "motion": 3}
+
+
+class Context:
+    """A stream on one device (mcsart_ctx)."""
+
+    def __init__(self, device: int = 0):
+        self._lib = load()
+        p = ctypes.c_void_p()
+        check(self._lib.mcsart_ctx_create(ctypes.byref(p), device), "mcsart_ctx_create")
+        self.ptr = p.value
+
+    def synchronize(self):
+        check(self._lib.mcsart_ctx_synchronize(self.ptr), "mcsart_ctx_synchronize")
+
+    def enable_timing(self, on: bool = True):
+        check(self._lib.mcsart_ctx_enable_timing(self.ptr, int(on)), "mcsart_ctx_enable_timing")
+
+    def reset_timing(self):
+        check(self._lib.mcsart_ctx_reset_timing(self.ptr), "mcsart_ctx_reset_timing")
+
+    def kernel_time(self, kind: str):
+        ms = ctypes.c_double()
+        n = ctypes.c_longlong()
+        check(self._lib.mcsart_ctx_kernel_time(self.ptr, TIMING_KINDS[kind], ctypes.byref(ms),
+                                               ctypes.byref(n)), "mcsart_ctx_kernel_time")
+        return ms.value, n.value
+
+    def close(self):
+        if self.ptr:
+            self._lib.mcsart_ctx_destroy(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def job_bytes(g: Geometry, n_views: int, n_iter: int, keep_residual: bool = False) -> int:
+    cg = CGeom.of(g)
+    return int(load().mcsart_job_bytes(ctypes.byref(cg), n_views, n_iter, int(keep_residual)))
+
+
+class Job:
+    """One scan resident on the device (mcsart_job)."""
+
+    def __init__(self, ctx: Context, g: Geometry, angles, volumes, flows, op_iter,
+                 lam: float = 1.0, motion: bool = True, nonneg: bool = False,
+                 keep_residual: bool = False):
+        self._lib = load()
+        self.ctx = ctx
+        self.g = g
+        self.angles = np.ascontiguousarray(angles, np.float32)
+        self.volumes = np.ascontiguousarray(volumes, np.float32)
+        self.flows = np.ascontiguousarray(flows, np.float32)
+        self.op_iter = np.ascontiguousarray(op_iter, np.int32)
+        self.V = int(self.angles.size)
+        self.n_iter = int(self.op_iter.size)
+        self.keep_residual = keep_residual
+        cg = CGeom.of(g)
+        opts = COpts(int(motion), int(nonneg), int(keep_residual), float(lam))
+        p = ctypes.c_void_p()
+        check(self._lib.mcsart_job_create(ctx.ptr, ctypes.byref(cg), self.V,
+                                          self.angles.ctypes.data, self.volumes.ctypes.data,
+                                          self.flows.ctypes.data, ctypes.byref(opts),
+                                          self.n_iter, self.op_iter.ctypes.data, ctypes.byref(p)),
+              "mcsart_job_create")
+        self.ptr = p.value
+
+    def set_projections(self, proj: np.ndarray):
+        proj = np.ascontiguousarray(proj, np.float32)
+        assert proj.size == self.V * self.g.ndet
+        check(self._lib.mcsart_job_set_projections(self.ptr, proj.ctypes.data),
+              "mcsart_job_set_projections")
+
+    def set_projections_device(self, d_proj):
+        check(self._lib.mcsart_job_set_projections_device(self.ptr, int(getattr(d_proj, "ptr", d_proj))),
+              "mcsart_job_set_projections_device")
+
+    def set_image(self, img: Optional[np.ndarray]):
+        if img is None:
+            check(self._lib.mcsart_job_set_image(self.ptr, None), "mcsart_job_set_image")
+        else:
+            img = np.ascontiguousarray(img, np.float32)
+            assert img.size == self.g.nvox
+            check(self._lib.mcsart_job_set_image(self.ptr, img.ctypes.data), "mcsart_job_set_image")
+
+    def set_fields(self, fields: Optional[Sequence[np.ndarray]]):
+        arr = (ctypes.c_void_p * 6)()
+        keep = []
+        for c in range(6):
+            if fields is None or fields[c] is None:
+                arr[c] = None
+            else:
+                f = np.ascontiguousarray(fields[c], np.float32)
+                assert f.size == self.g.nvox
+                keep.append(f)
+                arr[c] = f.ctypes.data
+        check(self._lib.mcsart_job_set_fields(self.ptr, arr), "mcsart_job_set_fields")
+
+    def reset(self):
+        check(self._lib.mcsart_job_reset(self.ptr), "mcsart_job_reset")
+
+    def run(self):
+        check(self._lib.mcsart_job_run(self.ptr), "mcsart_job_run")
+
+    def run_sweep(self, it: int):
+        check(self._lib.mcsart_job_run_sweep(self.ptr, it), "mcsart_job_run_sweep")
+
+    def image(self) -> np.ndarray:
+        out = np.empty(self.g.vol_shape, np.float32)
+        check(self._lib.mcsart_job_get_image(self.ptr, out.ctypes.data), "mcsart_job_get_image")
+        return out
+
+    def norms(self) -> np.ndarray:
+        out = np.empty(self.n_iter * self.V, np.float32)
+        check(self._lib.mcsart_job_get_norms(self.ptr, out.ctypes.data), "mcsart_job_get_norms")
+        return out
+
+    def residual(self) -> np.ndarray:
+        out = np.empty((self.V,) + self.g.det_shape, np.float32)
+        check(self._lib.mcsart_job_get_residual(self.ptr, out.ctypes.data), "mcsart_job_get_residual")
+        return out
+
+    def fields(self):
+        outs = [np.empty(self.g.vol_shape, np.float32) for _ in range(6)]
+        arr = (ctypes.c_void_p * 6)(*[o.ctypes.data for o in outs])
+        check(self._lib.mcsart_job_get_fields(self.ptr, arr), "mcsart_job_get_fields")
+        return outs
+
+    def device_image_ptr(self) -> int:
+        p = ctypes.c_void_p()
+        check(self._lib.mcsart_job_device_image(self.ptr, ctypes.byref(p)), "mcsart_job_device_image")
+        return p.value
+
+    def close(self):
+        if self.ptr:
+            self._lib.mcsart_job_destroy(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+__all__ = ["entry_aumisart", "parse_geo", "parse_iter", "Context", "Job", "job_bytes",
+           "MexBadInput", "McsartError"]

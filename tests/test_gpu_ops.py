@@ -1,0 +1,192 @@
+"""GPU parity of each operator against the FP64 oracle (rel-L2 <= 1e-5 unless stated).
+
+All calls go through the C ABI of libmcsart_hip (device pointers, reference argument order).
+"""
+import numpy as np
+import pytest
+
+import oracle
+from mcsart_amd import configs
+from mcsart_amd.ops import (DeviceArray, backproject, deform, diff_forward, host_add, host_add2,
+                            host_deform, host_deform2, host_diff, host_division, host_initial,
+                            host_initial2, host_update_udvf, project)
+from mcsart_amd.synth import ellipsoid_phantom
+from _util import random_fields, rel_l2, smooth_random
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-5   # per-operator tolerance against the FP64 oracle
+
+
+def _phantom(spec):
+    g = spec.geometry
+    return ellipsoid_phantom(g.nx, g.ny, g.nz, spec.voxel, spec.seed)
+
+
+@pytest.mark.parametrize("spec_fn,views", [
+    (configs.c1_static64, "all"),
+    (configs.c2_mcsart128, [0, 1, 24, 25, 26, 50, 75, 99, 124, 175]),
+    (configs.c3_clinical, [0, 82, 200]),
+])
+def test_projection_parity(gpu_lib, spec_fn, views):
+    spec = spec_fn()
+    g = spec.geometry
+    ph = _phantom(spec)
+    vs = range(spec.n_views) if views == "all" else views
+    worst = 0.0
+    for v in vs:
+        ang = spec.angles[v]
+        got = project(ph, g, ang)
+        ref = oracle.projection(ph, g, ang)
+        e = rel_l2(got, ref)
+        worst = max(worst, e)
+        assert e <= TOL, f"{spec.name} view {v}: rel-L2 {e:.3e}"
+    print(f"{spec.name}: worst projection rel-L2 {worst:.3e}")
+
+
+def test_projection_ones_closed_form(gpu_lib):
+    """A ones-volume gives the number of planes the ray's normalised footprint covers
+    (src/cu_projection.cu:83,86); central rays of the 64^3 geometry stay inside every plane."""
+    spec = configs.c1_static64()
+    g = spec.geometry
+    ones = np.ones(g.vol_shape, np.float32)
+    for ang in (0.0, 0.3, np.pi / 2):
+        p = project(ones, g, np.float32(ang))
+        c = p[g.nb // 2 - 2:g.nb // 2 + 2, g.na // 2 - 2:g.na // 2 + 2]
+        assert np.allclose(c, g.nx, rtol=1e-5), c
+
+
+@pytest.mark.parametrize("spec_fn,views", [
+    (configs.c1_static64, [0, 7, 15, 22, 30, 45]),
+    (configs.c2_mcsart128, [0, 25, 60, 133]),
+    (configs.c3_clinical, [0, 82]),
+])
+def test_backprojection_parity(gpu_lib, spec_fn, views):
+    spec = spec_fn()
+    g = spec.geometry
+    for v in views:
+        ang = spec.angles[v]
+        proj = smooth_random(g.det_shape, spec.seed + v)
+        got = backproject(proj, g, ang)
+        ref = oracle.backprojection(proj, g, ang)
+        e = rel_l2(got, ref)
+        assert e <= TOL, f"{spec.name} view {v}: rel-L2 {e:.3e}"
+
+
+def test_backprojection_constant(gpu_lib):
+    """A constant projection c backprojects to c times the clipped footprint area, which for a
+    voxel near the isocentre is the full (unnormalised) footprint; it must equal the oracle."""
+    spec = configs.c1_static64()
+    g = spec.geometry
+    proj = np.full(g.det_shape, 2.0, np.float32)
+    got = backproject(proj, g, np.float32(0.0))
+    ref = oracle.backprojection(proj, g, np.float32(0.0))
+    assert rel_l2(got, ref) <= TOL
+    assert np.all(got >= 0)
+
+
+@pytest.mark.parametrize("shape", [(16, 24, 24), (32, 64, 64)])
+def test_deform_parity(gpu_lib, shape):
+    img = smooth_random(shape, 11)
+    fields = random_fields(shape, 23)
+    for vol, flow in ((0.7, -0.3), (-1.3, 0.45), (0.0, 0.0)):
+        got = deform(img, vol, flow, fields)
+        ref = oracle.deform(img, vol, flow, fields)
+        assert rel_l2(got, ref) <= TOL, (vol, flow)
+    # zero DVF is an exact identity
+    got = deform(img, 0.0, 0.0, fields)
+    assert np.array_equal(got, img)
+
+
+def test_deform_in_place(gpu_lib):
+    shape = (12, 20, 20)
+    nz, ny, nx = shape
+    img = smooth_random(shape, 5)
+    fields = random_fields(shape, 7)
+    d = [DeviceArray.from_host(f) for f in fields]
+    x = DeviceArray.from_host(img)
+    host_deform(x, x, nx, ny, nz, 0.9, 0.2, *d)
+    ref = oracle.deform(img, 0.9, 0.2, fields)
+    assert rel_l2(x.to_host(), ref) <= TOL
+
+
+def test_deform2_parity(gpu_lib):
+    shape = (10, 16, 16)
+    nz, ny, nx = shape
+    img = smooth_random(shape, 3)
+    fields = random_fields(shape, 4)
+    d = [DeviceArray.from_host(f) for f in fields]
+    x = DeviceArray.from_host(img)
+    out = DeviceArray.empty(shape)
+    host_deform2(out, x, nx, ny, nz, 0.5, -0.25, *d)
+    ref = oracle.deform2(img, 0.5, -0.25, fields)
+    assert rel_l2(out.to_host(), ref) <= TOL
+
+
+def test_elementwise_parity(gpu_lib):
+    shape = (9, 14, 14)
+    nz, ny, nx = shape
+    a = smooth_random(shape, 1)
+    b = smooth_random(shape, 2)
+    c = smooth_random(shape, 3)
+    # add
+    da, db = DeviceArray.from_host(a), DeviceArray.from_host(b)
+    host_add(da, db, nx, ny, nz, -0.75)
+    assert rel_l2(da.to_host(), oracle.add(a, b, -0.75)) <= 1e-6
+    # add2, all three directions (transposed read of b)
+    for ind in (1, 2, 3):
+        da, db, dc = DeviceArray.from_host(a), DeviceArray.from_host(b), DeviceArray.from_host(c)
+        host_add2(da, db, nx, ny, nz, dc, 1.7, ind)
+        assert rel_l2(da.to_host(), oracle.add2(a, b, c, 1.7, ind)) <= 1e-6, ind
+    # initial / initial2
+    d = DeviceArray.empty(shape)
+    host_initial(d, nx, ny, nz, 3.25)
+    assert np.all(d.to_host() == np.float32(3.25))
+    dc = DeviceArray.from_host(c)
+    host_initial2(d, nx, ny, nz, dc, -0.4, 0.15)
+    assert rel_l2(d.to_host(), oracle.initial2(c, -0.4, 0.15)) <= 1e-6
+    # division with NaN/Inf mapping
+    num = a.copy()
+    den = b.copy()
+    den[0, 0, :5] = 0.0
+    num[0, 0, :2] = 0.0
+    dn, dd = DeviceArray.from_host(num), DeviceArray.from_host(den)
+    host_division(dn, dd, nx, ny, nz)
+    got = dn.to_host()
+    ref = oracle.division(num, den)
+    assert np.all(got[0, 0, :5] == 0.0)
+    assert rel_l2(got, ref) <= 1e-6
+    # diffs
+    for ind in (1, 2, 3):
+        dx = DeviceArray.from_host(c)
+        out = DeviceArray.empty(shape)
+        host_diff(out, dx, nx, ny, nz, ind)
+        assert np.allclose(out.to_host(), oracle.diff_backward(c, ind), atol=1e-6), ind
+        diff_forward(out, dx, nx, ny, nz, ind)
+        assert np.allclose(out.to_host(), oracle.diff_forward(c, ind), atol=1e-6), ind
+
+
+def test_update_udvf_parity(gpu_lib):
+    shape = (8, 12, 12)
+    nz, ny, nx = shape
+    img = smooth_random(shape, 8)
+    img0 = smooth_random(shape, 9)
+    img[2, 3, :] = img[2, 3, 0]   # flat rows: zero gradient (w == 0 early return) at some voxels
+    fields = random_fields(shape, 10)
+    d = [DeviceArray.from_host(f) for f in fields]
+    host_update_udvf(*d, DeviceArray.from_host(img), DeviceArray.from_host(img0), 0.3, -0.8,
+                     nx, ny, nz, 0)
+    ref = oracle.update_udvf(fields, img, img0, 0.3, -0.8)
+    for k in range(6):
+        assert rel_l2(d[k].to_host(), ref[k]) <= 1e-5, k
+
+
+def test_invalid_arguments(gpu_lib):
+    from mcsart_amd._lib import McsartError
+    x = DeviceArray.zeros((4, 6, 8))
+    y = DeviceArray.zeros((4, 6, 8))
+    with pytest.raises(McsartError):
+        host_add2(x, y, 8, 6, 4, y, 1.0, 1)          # nx != ny
+    with pytest.raises(McsartError):
+        host_diff(x, y, 8, 6, 4, 7)                  # bad ind
+    with pytest.raises(McsartError):
+        host_add(x, y, 0, 6, 4, 1.0)                 # empty volume
