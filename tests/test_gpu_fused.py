@@ -63,7 +63,13 @@ def test_sart_residual_motion_mode(gpu_lib, scene):
 
 @pytest.mark.parametrize("view", [0, 25, 140])
 def test_sart_backprojection_ratio(gpu_lib, scene, view):
-    """u = B(r) / B(P1) with NaN/Inf -> 0 outside the footprint."""
+    """u = B(r) / B(P1) with NaN/Inf -> 0 outside the footprint.
+
+    The rule 0/0 -> 0 makes u discontinuous where a voxel's footprint only grazes the detector
+    edge: B(P1) ~ 1e-7 gives u = r/p1 of the grazed cell, B(P1) == 0 gives 0.  At the 45 deg view
+    (25 of 200) such grazing voxels exist and the reference's own FP32 arithmetic already differs
+    from FP64 by ~1e-2 there.  Bars: 1e-5 where B(P1) > 1e-3 max, and over all voxels no worse
+    than the reference-as-written FP32 (oracle f32 instance)."""
     spec, g, x, other = scene
     ang = spec.angles[view]
     r = smooth_random(g.det_shape, view) - 0.3
@@ -72,9 +78,15 @@ def test_sart_backprojection_ratio(gpu_lib, scene, view):
     d_u = DeviceArray.empty(g.vol_shape)
     sart_backprojection(d_u, DeviceArray.from_host(rp), *_geo_args(g, ang))
     u = d_u.to_host()
-    ref = oracle.division(oracle.backprojection(r, g, ang), oracle.backprojection(p1, g, ang))
-    assert rel_l2(u, ref) <= TOL
-    assert np.array_equal(u == 0, ref == 0)
+    bp = oracle.backprojection(p1, g, ang)
+    ref = oracle.division(oracle.backprojection(r, g, ang), bp)
+    ref32 = oracle.division(oracle.backprojection(r, g, ang, dtype=np.float32),
+                            oracle.backprojection(p1, g, ang, dtype=np.float32), dtype=np.float32)
+    ok = bp > 1e-3 * bp.max()
+    assert rel_l2(u[ok], ref[ok]) <= TOL
+    assert rel_l2(u, ref) <= max(TOL, 2 * rel_l2(ref32, ref))
+    flips = (u == 0) != (ref == 0)
+    assert np.all(bp[flips] < 1e-5 * bp.max())
 
 
 def test_motion_prep(gpu_lib):

@@ -1,0 +1,114 @@
+"""Host-side checks that need no GPU: the C-ABI library loads and exports every declared symbol,
+the entry_AUMISART parameter parsing, geometry conversion and the seeded synthetic data."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from mcsart_amd import _lib, configs
+from mcsart_amd.driver import MexBadInput, parse_geo, parse_iter
+from mcsart_amd.geometry import Geometry, uniform_angles
+from mcsart_amd.synth import (SplitMix64, breathing_trace, ellipsoid_phantom, motion_fields,
+                              poisson_noise, seed_for, Voxel)
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared():
+    text = open(os.path.join(ROOT, "include", "mcsart_hip.h")).read()
+    return sorted(set(re.findall(r"\b(mcsart_\w+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = _lib.load()
+    declared = _declared()
+    assert len(declared) >= 40
+    missing = [s for s in declared if not hasattr(lib, s)]
+    assert not missing, missing
+    # the ctypes signature table covers exactly the header
+    assert sorted(_lib.SIGNATURES) == declared
+
+
+def test_status_strings_without_device():
+    lib = _lib.load()
+    assert lib.mcsart_status_string(0) == b"ok"
+    assert lib.mcsart_status_string(-1) == b"invalid argument"
+    assert lib.mcsart_version() >= 10000
+
+
+def test_job_bytes_accounting():
+    from mcsart_amd.driver import job_bytes
+    g = configs.c2_mcsart128().geometry
+    b = job_bytes(g, 200, 20)
+    N, D = g.nvox, g.ndet
+    # projections + image + 6 fields + 7 pristine + 2 work volumes + (r, n) pairs + norms
+    floor = 4 * (200 * D + N + 6 * N + 7 * N + 2 * N + 2 * D + 20 * 200)
+    assert floor <= b <= floor + (1 << 20)
+
+
+def test_geometry_from_mm_matches_reference_units():
+    g = configs.c3_clinical().geometry
+    assert g.SO == np.float32(1000 / 1.5) and g.SD == np.float32(1000.0)
+    assert g.da == np.float32(0.78 / 1.5) and g.db == np.float32(0.78 / 2.0)
+    assert g.ai == -(512 / 2 - 0.5) and g.bi == -(384 / 2 - 0.5)
+
+
+def test_parse_geo_aliases_and_defaults():
+    # src/entry_AUMISART.cu:34-108: nv/nu aliases, da/db default 1, ai/bi shifted, SI and DI
+    g = parse_geo(dict(nx=8, ny=8, nz=4, nv=10, nu=6, SI=50.0, DI=25.0, ai=1.0))
+    assert (g.na, g.nb, g.da, g.db) == (10, 6, 1.0, 1.0)
+    assert g.SO == 50.0 and g.SD == 75.0
+    assert g.ai == np.float32(1.0) - np.float32(4.5) and g.bi == -2.5
+    with pytest.raises(MexBadInput):
+        parse_geo(dict(nx=8, ny=8, nz=4, na=10, nb=6))        # no SO
+    with pytest.raises(MexBadInput):
+        parse_geo(dict(ny=8, nz=4, na=10, nb=6, SO=1, SD=2))  # no nx
+
+
+def test_parse_iter_required_fields():
+    base = dict(n_iter=2, op_iter=[1, 0], n_views=3, outIter=1, volumes=[0, 0, 0],
+                flows=[0, 0, 0], err_weights=[0, 0, 0], angles=[0, 1, 2])
+    base["lambda"] = 0.5
+    p = parse_iter(base)
+    assert p["op_iter"].dtype == np.int32 and p["lam"] == 0.5
+    for k in ("op_iter", "n_views", "lambda", "outIter", "volumes", "angles"):
+        bad = dict(base)
+        del bad[k]
+        with pytest.raises(MexBadInput):
+            parse_iter(bad)
+
+
+def test_splitmix_reproducible_and_uniform():
+    a = SplitMix64(123).uniform(10000)
+    b = SplitMix64(123).uniform(10000)
+    assert np.array_equal(a, b)
+    assert 0.49 < a.mean() < 0.51 and a.min() >= 0 and a.max() < 1
+    z = SplitMix64(5).normal(20000)
+    assert abs(z.mean()) < 0.03 and abs(z.std() - 1) < 0.03
+    # known first value of the stream (pins the mixing constants)
+    assert int(SplitMix64(0).next_u64(1)[0]) == 0xE220A8397B1DCDAF
+
+
+def test_synthetic_scan_properties():
+    spec = configs.c2_mcsart128()
+    vox = spec.voxel
+    ph = ellipsoid_phantom(32, 32, 32, Voxel(8.0, 8.0), seed_for(2))
+    assert ph.dtype == np.float32 and ph.min() >= 0 and 0.01 <= ph.max() <= 0.3
+    a, b = breathing_trace(200, 7)
+    assert a.shape == (200,) and np.abs(a).max() < 1.5
+    assert abs(float(np.corrcoef(b[1:-1], np.gradient(np.sin(2 * np.pi * 0.03 * np.arange(200) / 4.0), 0.03)[1:-1])[0, 1])) > 0.3
+    fx, fy, fz = motion_fields(32, 32, 32, Voxel(8.0, 8.0), 9)
+    assert np.isclose(fz.max(), 12.0 / 8.0) and np.isclose(fx.max(), 4.0 / 8.0)
+    assert np.all(fy == 0)
+    p = poisson_noise(np.full((4, 5), 2.0, np.float32), 3, photons=1e5, dx_mm=1.0)
+    assert np.abs(p - 2.0).max() < 0.1
+
+
+def test_batch_mix_is_seeded():
+    j1 = configs.c4_batch()
+    j2 = configs.c4_batch()
+    assert [(j.nx, j.nz, j.na, j.n_views) for j in j1] == [(j.nx, j.nz, j.na, j.n_views) for j in j2]
+    assert len(j1) == 16 and all(200 <= j.n_views <= 900 for j in j1)
+    assert len({(j.nx, j.nz) for j in j1}) >= 2
