@@ -145,13 +145,14 @@ def main():
     job.set_image(None)
     job.set_fields(data.fields)
 
-    for w in range(args.warmup):
+    # Timing events are part of the captured sweep graphs, so they are switched on before the
+    # warm-up (which captures the graphs) and stay on in the timed region.
+    ctx.enable_timing(True)
+    for w in range(max(args.warmup, 1)):
         job.reset()
         job.run()
         ctx.synchronize()
-        log(f"[bench] warmup {w + 1}/{args.warmup} done")
-
-    ctx.enable_timing(True)
+        log(f"[bench] warmup {w + 1}/{max(args.warmup, 1)} done")
     ctx.reset_timing()
     ctx.synchronize()
     lat = []

@@ -126,11 +126,13 @@ int mcsart_diff_forward(float *img1, const float *img, int nx, int ny, int nz, i
  * of reference operator calls; the drivers are built from them.
  *
  * mcsart_sart_residual: fuses src/AUMISART.cu:194-209 (image sweep) and :246-263 (motion sweep).
- *   d_rp (2*na*nb floats) receives interleaved pairs (r, n) with r = meas - P(img) and
- *   n = P(1) when d_img2 is NULL (image sweep), else n = P(img2) (motion sweep, where img is
- *   x - img0 so that r = meas - P(x) + P(img0) and img2 is q).  *d_norm (one device float)
- *   receives ||r|| / ||meas||, the reference's per-view residual ratio (:209, :263). */
-int mcsart_sart_residual(float *d_rp, float *d_norm, const float *d_img, const float *d_img2,
+ *   d_rp (2*na*nb floats) receives interleaved pairs (r, n).
+ *   motion_pairs == 0 (image sweep): d_vol is a volume x (N floats); r = meas - P(x), n = P(1).
+ *   motion_pairs != 0 (motion sweep): d_vol holds N interleaved pairs (g, q) as written by
+ *   mcsart_motion_prep; r = meas - P(g) = meas - P(x) + P(img0) and n = P(q).
+ *   *d_norm (one device float) receives ||r|| / ||meas||, the reference's per-view residual
+ *   ratio (:209, :263). */
+int mcsart_sart_residual(float *d_rp, float *d_norm, const float *d_vol, int motion_pairs,
                          const float *d_meas, float angle, float SO, float SD, float da, int na,
                          float ai, float db, int nb, float bi, int nx, int ny, int nz);
 
@@ -142,8 +144,9 @@ int mcsart_sart_backprojection(float *d_u, const float *d_rp, float angle, float
 
 /* mcsart_motion_prep: fuses src/AUMISART.cu:252-258 and :267.  With
  * img0 = sum over (alpha_y,D1),(alpha_x,D2),(alpha_z,D3) of add2(alpha_c, vd) plus the same with
- * beta and fd:  d_g = img - img0  and  d_q = (D1+D2+D3)(img) * (-vd - fd).  nx == ny. */
-int mcsart_motion_prep(float *d_g, float *d_q, const float *d_img, const float *alpha_x,
+ * beta and fd, d_pairs (2*nx*ny*nz floats) receives per voxel the interleaved pair
+ * (g, q) = (img - img0, (D1+D2+D3)(img) * (-vd - fd)).  nx == ny. */
+int mcsart_motion_prep(float *d_pairs, const float *d_img, const float *alpha_x,
                        const float *alpha_y, const float *alpha_z, const float *beta_x,
                        const float *beta_y, const float *beta_z, float vd, float fd, int nx,
                        int ny, int nz);
@@ -218,8 +221,11 @@ int mcsart_job_set_fields(mcsart_job *job, const float *const h_fields[6]); /* a
 int mcsart_job_reset(mcsart_job *job);
 /* Enqueue the whole schedule (n_iter sweeps) on the context stream; returns immediately. */
 int mcsart_job_run(mcsart_job *job);
-/* Enqueue a single sweep it (0 <= it < n_iter). */
+/* Enqueue a single sweep it (0 <= it < n_iter).  By default each sweep is captured into a HIP
+ * graph on its first run and replayed afterwards; enable = 0 launches kernel by kernel instead
+ * (environment: MCSART_GRAPHS=0 sets the default for new jobs). */
 int mcsart_job_run_sweep(mcsart_job *job, int it);
+int mcsart_job_set_graphs(mcsart_job *job, int enable);
 int mcsart_job_get_image(mcsart_job *job, float *h_img);
 int mcsart_job_get_norms(mcsart_job *job, float *h_norm);                /* n_iter*n_views */
 int mcsart_job_get_residual(mcsart_job *job, float *h_proj);             /* n_views*na*nb */

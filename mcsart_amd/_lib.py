@@ -42,9 +42,9 @@ SIGNATURES = {
     "mcsart_division": ([_P, _P, _I, _I, _I], _I),
     "mcsart_diff": ([_P, _P, _I, _I, _I, _I], _I),
     "mcsart_diff_forward": ([_P, _P, _I, _I, _I, _I], _I),
-    "mcsart_sart_residual": ([_P] * 5 + [_F, _F, _F, _F, _I, _F, _F, _I, _F, _I, _I, _I], _I),
+    "mcsart_sart_residual": ([_P, _P, _P, _I, _P] + [_F, _F, _F, _F, _I, _F, _F, _I, _F, _I, _I, _I], _I),
     "mcsart_sart_backprojection": ([_P, _P, _F, _F, _F, _F, _I, _F, _F, _I, _F, _I, _I, _I], _I),
-    "mcsart_motion_prep": ([_P] * 9 + [_F, _F, _I, _I, _I], _I),
+    "mcsart_motion_prep": ([_P] * 8 + [_F, _F, _I, _I, _I], _I),
     "mcsart_motion_update": ([_P] * 8 + [_F, _F, _I, _I, _I], _I),
     "mcsart_aumisart": ([_P] * 6 + [_I] * 8 + [_P] + [_F] * 8 + [_P] * 4, _I),
     "mcsart_aumisart_motion": ([_P] * 6 + [_I] * 8 + [_P] + [_F] * 8 + [_P] * 4 + [_P] * 6, _I),
@@ -57,6 +57,7 @@ SIGNATURES = {
     "mcsart_job_reset": ([_P], _I),
     "mcsart_job_run": ([_P], _I),
     "mcsart_job_run_sweep": ([_P, _I], _I),
+    "mcsart_job_set_graphs": ([_P, _I], _I),
     "mcsart_job_get_image": ([_P, _P], _I),
     "mcsart_job_get_norms": ([_P, _P], _I),
     "mcsart_job_get_residual": ([_P, _P], _I),

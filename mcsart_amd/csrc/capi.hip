@@ -82,8 +82,8 @@ int ctx_scratch(mcsart_ctx *ctx, size_t bytes, void **ptr) {
     return MCSART_OK;
 }
 
-int ctx_reduction(mcsart_ctx *ctx, int nblocks, double **partials, unsigned **counter) {
-    const size_t need = sizeof(double) * 2 * (size_t)nblocks + 256;
+int ctx_reduction(mcsart_ctx *ctx, int nblocks, double **partials) {
+    const size_t need = sizeof(double) * 2 * (size_t)nblocks;
     if (ctx->red_bytes < need) {
         if (ctx->red) {
             MCSART_HIP_TRY(hipStreamSynchronize(ctx->stream));
@@ -95,11 +95,9 @@ int ctx_reduction(mcsart_ctx *ctx, int nblocks, double **partials, unsigned **co
             (void)hipGetLastError();
             return fail(MCSART_ENOMEM, "reduction scratch allocation failed");
         }
-        MCSART_HIP_TRY(hipMemsetAsync(ctx->red, 0, need, ctx->stream));
         ctx->red_bytes = need;
     }
     *partials = (double *)ctx->red;
-    *counter = (unsigned *)((char *)ctx->red + sizeof(double) * 2 * (size_t)nblocks);
     return MCSART_OK;
 }
 
@@ -115,15 +113,24 @@ TimedRegion::TimedRegion(mcsart_ctx *c, int k) : ctx(c), kind(k) {
         }
         return e;
     };
-    a = take();
-    b = take();
+    if (ctx->capture) {   // events owned by the graph being captured
+        if (hipEventCreate(&a) != hipSuccess) a = nullptr;
+        if (hipEventCreate(&b) != hipSuccess) b = nullptr;
+    } else {
+        a = take();
+        b = take();
+    }
     if (a && b) hipEventRecord(a, ctx->stream);
 }
 
 TimedRegion::~TimedRegion() {
     if (!ctx || !ctx->timing || !a || !b) return;
     hipEventRecord(b, ctx->stream);
-    ctx->pending.push_back({kind, a, b});
+    if (ctx->capture) {
+        ctx->capture->push_back({kind, a, b, false});
+        return;
+    }
+    ctx->pending.push_back({kind, a, b, true});
     if (ctx->pending.size() >= 4096) ctx_collect_timing(ctx);
 }
 
@@ -134,8 +141,10 @@ int ctx_collect_timing(mcsart_ctx *ctx) {
         MCSART_HIP_TRY(hipEventElapsedTime(&ms, p.a, p.b));
         ctx->total_ms[p.kind] += ms;
         ctx->launches[p.kind] += 1;
-        ctx->event_pool.push_back(p.a);
-        ctx->event_pool.push_back(p.b);
+        if (p.pooled) {
+            ctx->event_pool.push_back(p.a);
+            ctx->event_pool.push_back(p.b);
+        }
     }
     ctx->pending.clear();
     return MCSART_OK;
@@ -253,6 +262,7 @@ extern "C" int mcsart_ctx_destroy(mcsart_ctx *ctx) {
     hipSetDevice(ctx->device);
     hipStreamSynchronize(ctx->stream);
     for (auto &p : ctx->pending) {
+        if (!p.pooled) continue;   // owned by a job's graph
         hipEventDestroy(p.a);
         hipEventDestroy(p.b);
     }
@@ -322,11 +332,21 @@ extern "C" int mcsart_projection(float *d_proj, const float *d_img, float angle,
     if (rc) return rc;
     if (!d_proj || !d_img) return fail(MCSART_EINVAL, "null device pointer");
     GET_CTX();
+    const ViewTrig t = view_trig(angle);
+    const ViewInfo vi = view_info_of(g, t);
     ProjArgs a{};
     a.vol0 = d_img;
     a.out = d_proj;
+    if (vi.has_x) {   // x-branch rays read a transposed copy (contiguous footprints)
+        void *tmp = nullptr;
+        rc = ctx_scratch(ctx, sizeof(float) * (size_t)g.nvox(), &tmp);
+        if (rc) return rc;
+        rc = launch_transpose((float *)tmp, d_img, nx, ny, nz, ctx->stream);
+        if (rc) return rc;
+        a.vol0T = (const float *)tmp;
+    }
     TimedRegion tr(ctx, TK_PROJ);
-    return finish(ctx, launch_projector(g, view_trig(angle), PROJ_PLAIN, a, ctx->stream));
+    return finish(ctx, launch_projector(g, t, vi, PROJ_PLAIN, a, ctx->stream));
 }
 
 extern "C" int mcsart_backprojection(float *d_img, const float *d_proj, float angle, float SO,
@@ -363,7 +383,7 @@ extern "C" int mcsart_deform(float *d_img1, const float *d_img, int nx, int ny, 
         src = (const float *)tmp;
     }
     TimedRegion tr(ctx, TK_WARP);
-    return finish(ctx, launch_warp(d_img1, src, nx, ny, nz, volume, flow, f, ctx->stream));
+    return finish(ctx, launch_warp(d_img1, nullptr, src, nx, ny, nz, volume, flow, f, ctx->stream));
 }
 
 extern "C" int mcsart_deform2(float *d_img1, const float *d_img, int nx, int ny, int nz,
@@ -473,26 +493,43 @@ extern "C" int mcsart_diff_forward(float *img1, const float *img, int nx, int ny
 // ---------------------------------------------------------------------------------------------
 // fused SART steps
 // ---------------------------------------------------------------------------------------------
-extern "C" int mcsart_sart_residual(float *d_rp, float *d_norm, const float *d_img,
-                                    const float *d_img2, const float *d_meas, float angle,
-                                    float SO, float SD, float da, int na, float ai, float db,
-                                    int nb, float bi, int nx, int ny, int nz) {
+extern "C" int mcsart_sart_residual(float *d_rp, float *d_norm, const float *d_vol,
+                                    int motion_pairs, const float *d_meas, float angle, float SO,
+                                    float SD, float da, int na, float ai, float db, int nb,
+                                    float bi, int nx, int ny, int nz) {
     const Geom g{nx, ny, nz, na, nb, da, db, ai, bi, SO, SD};
     int rc = validate_geom(g);
     if (rc) return rc;
-    if (!d_rp || !d_norm || !d_img || !d_meas) return fail(MCSART_EINVAL, "null device pointer");
+    if (!d_rp || !d_norm || !d_vol || !d_meas) return fail(MCSART_EINVAL, "null device pointer");
     GET_CTX();
+    const ViewTrig t = view_trig(angle);
+    const ViewInfo vi = view_info_of(g, t);
     ProjArgs a{};
-    a.vol0 = d_img;
-    a.vol1 = d_img2;
+    if (motion_pairs) {
+        a.pair = (const float2 *)d_vol;
+    } else {
+        a.vol0 = d_vol;
+        if (vi.has_x) {
+            void *tmp = nullptr;
+            rc = ctx_scratch(ctx, sizeof(float) * (size_t)g.nvox(), &tmp);
+            if (rc) return rc;
+            rc = launch_transpose((float *)tmp, d_vol, nx, ny, nz, ctx->stream);
+            if (rc) return rc;
+            a.vol0T = (const float *)tmp;
+        }
+    }
     a.meas = d_meas;
     a.rp = (float2 *)d_rp;
-    a.norm_out = d_norm;
-    rc = ctx_reduction(ctx, projector_partials(g), &a.partials, &a.counter);
+    rc = ctx_reduction(ctx, projector_partials(g), &a.partials);
     if (rc) return rc;
-    TimedRegion tr(ctx, TK_PROJ);
-    return finish(ctx, launch_projector(g, view_trig(angle), d_img2 ? PROJ_MOTION : PROJ_IMAGE, a,
-                                        ctx->stream));
+    int nparts = 0;
+    {
+        TimedRegion tr(ctx, TK_PROJ);
+        rc = launch_projector(g, t, vi, motion_pairs ? PROJ_MOTION : PROJ_IMAGE, a, ctx->stream,
+                              &nparts);
+        if (rc) return rc;
+    }
+    return finish(ctx, launch_norm_finish(NormFinish{a.partials, nparts, d_norm}, ctx->stream));
 }
 
 extern "C" int mcsart_sart_backprojection(float *d_u, const float *d_rp, float angle, float SO,
@@ -508,21 +545,20 @@ extern "C" int mcsart_sart_backprojection(float *d_u, const float *d_rp, float a
                                             (const float2 *)d_rp, d_u, ctx->stream));
 }
 
-extern "C" int mcsart_motion_prep(float *d_g, float *d_q, const float *d_img,
-                                  const float *alpha_x, const float *alpha_y,
-                                  const float *alpha_z, const float *beta_x, const float *beta_y,
-                                  const float *beta_z, float vd, float fd, int nx, int ny, int nz) {
+extern "C" int mcsart_motion_prep(float *d_pairs, const float *d_img, const float *alpha_x,
+                                  const float *alpha_y, const float *alpha_z,
+                                  const float *beta_x, const float *beta_y, const float *beta_z,
+                                  float vd, float fd, int nx, int ny, int nz) {
     int rc = validate_volume(nx, ny, nz, true);
     if (rc) return rc;
-    if (!d_g || !d_q || !d_img || !alpha_x || !alpha_y || !alpha_z || !beta_x || !beta_y ||
-        !beta_z)
+    if (!d_pairs || !d_img || !alpha_x || !alpha_y || !alpha_z || !beta_x || !beta_y || !beta_z)
         return fail(MCSART_EINVAL, "null device pointer");
-    if (d_g == d_img || d_q == d_img || d_g == d_q)
-        return fail(MCSART_EINVAL, "motion_prep outputs must not alias each other or the image");
+    if (d_pairs == d_img) return fail(MCSART_EINVAL, "motion_prep output must not alias the image");
     GET_CTX();
     const float *f[6] = {alpha_x, alpha_y, alpha_z, beta_x, beta_y, beta_z};
     TimedRegion tr(ctx, TK_MOTION);
-    return finish(ctx, launch_motion_prep(d_g, d_q, d_img, f, nx, ny, nz, vd, fd, ctx->stream));
+    return finish(ctx, launch_motion_prep((float2 *)d_pairs, nullptr, d_img, f, nx, ny, nz, vd, fd,
+                                          ctx->stream));
 }
 
 extern "C" int mcsart_motion_update(float *alpha_x, float *alpha_y, float *alpha_z,

@@ -23,19 +23,32 @@ struct mcsart_job {
     mcsart_opts opts{};
     std::vector<float> angles, volumes, flows;
     std::vector<ViewTrig> trig;
+    std::vector<ViewInfo> vinfo;   // per view: branches present, footprint spans
     float *d_proj = nullptr;     // V*D measured projections
     float *d_img = nullptr;      // N
     float *d_fields = nullptr;   // 6N: ax ay az bx by bz
     float *d_init = nullptr;     // 7N: initial image and fields (mcsart_job_reset)
-    float *d_w1 = nullptr;       // N: warped image (image sweep) / g (motion sweep)
-    float *d_w2 = nullptr;       // N: q then u
+    float *d_w1 = nullptr;       // 2N: warped image (image sweep) / (g, q) pairs (motion sweep)
+    float *d_w1T = nullptr;      // 2N: the same in the xy-transposed layout / transposed image
+    float *d_w2 = nullptr;       // N: u
     float2 *d_rp = nullptr;      // D
     float *d_norm = nullptr;     // n_iter*V
     float *d_resid = nullptr;    // V*D (keep_residual)
     double *d_partials = nullptr;
-    unsigned *d_counter = nullptr;
     void *block = nullptr;       // single allocation backing all of the above
     size_t bytes = 0;
+    // One captured graph per sweep of the schedule: the kernels of a sweep and all their
+    // arguments are fixed for the life of the job, so a sweep is captured on first use and
+    // replayed afterwards (one launch instead of ~4 per view).
+    struct SweepGraph {
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+        bool timed = false;    // captured with timing events
+        bool queued = false;   // its events are in the context's pending list
+        std::vector<mcsart_ctx::Pending> ev;
+    };
+    std::vector<SweepGraph> graphs;
+    bool use_graphs = true;
 };
 
 namespace {
@@ -43,7 +56,7 @@ namespace {
 size_t align_up(size_t v) { return (v + 255) & ~(size_t)255; }
 
 struct Layout {
-    size_t proj, img, fields, init, w1, w2, rp, norm, resid, partials, counter, total;
+    size_t proj, img, fields, init, w1, w1T, w2, rp, norm, resid, partials, total;
 };
 
 Layout layout(const Geom &g, int V, int n_iter, int keep_residual) {
@@ -59,13 +72,13 @@ Layout layout(const Geom &g, int V, int n_iter, int keep_residual) {
     L.img = take(sizeof(float) * N);
     L.fields = take(sizeof(float) * N * 6);
     L.init = take(sizeof(float) * N * 7);
-    L.w1 = take(sizeof(float) * N);
+    L.w1 = take(sizeof(float) * N * 2);
+    L.w1T = take(sizeof(float) * N * 2);
     L.w2 = take(sizeof(float) * N);
     L.rp = take(sizeof(float2) * D);
     L.norm = take(sizeof(float) * (size_t)n_iter * V);
     L.resid = take(keep_residual ? sizeof(float) * D * V : 0);
     L.partials = take(sizeof(double) * 2 * (size_t)projector_partials(g));
-    L.counter = take(sizeof(unsigned));
     L.total = off;
     return L;
 }
@@ -88,32 +101,42 @@ int image_sweep(mcsart_job *j, int it) {
     hipStream_t st = j->ctx->stream;
     const Fields f = fields_of(j);
     const bool motion = j->opts.motion != 0;
-    int rc;
+    int rc, nparts = 0;
+    // Static sweeps project x itself: keep its transposed copy (in d_w1T) current for views with
+    // x-branch rays; the update kernel of view i writes it when view i+1 needs it.
+    if (!motion && j->vinfo[0].has_x) {
+        TimedRegion tr(j->ctx, TK_WARP);
+        rc = launch_transpose(j->d_w1T, j->d_img, g.nx, g.ny, g.nz, st);
+        if (rc) return rc;
+    }
     for (int iv = 0; iv < j->V; ++iv) {
-        const float *src = j->d_img;
-        if (motion) {   // x_i = W(x; v_i - v_0, f_i - f_0)  (:193)
+        const ViewInfo &vi = j->vinfo[iv];
+        ProjArgs a{};
+        if (motion) {   // x_i = W(x; v_i - v_0, f_i - f_0)  (:193), in the layouts P reads
             TimedRegion tr(j->ctx, TK_WARP);
-            rc = launch_warp(j->d_w1, j->d_img, g.nx, g.ny, g.nz, j->volumes[iv] - j->volumes[0],
+            rc = launch_warp(vi.has_y ? j->d_w1 : nullptr, vi.has_x ? j->d_w1T : nullptr,
+                             j->d_img, g.nx, g.ny, g.nz, j->volumes[iv] - j->volumes[0],
                              j->flows[iv] - j->flows[0], f, st);
             if (rc) return rc;
-            src = j->d_w1;
+            a.vol0 = j->d_w1;
+        } else {
+            a.vol0 = j->d_img;
         }
+        a.vol0T = vi.has_x ? j->d_w1T : nullptr;
         {   // r = p_i - P x_i, P(1), rho_i  (:194-211)
-            ProjArgs a{};
-            a.vol0 = src;
             a.meas = j->d_proj + D * iv;
             a.rp = j->d_rp;
             a.resid_copy = j->d_resid ? j->d_resid + D * iv : nullptr;
             a.partials = j->d_partials;
-            a.counter = j->d_counter;
-            a.norm_out = j->d_norm + (long long)it * j->V + iv;
             TimedRegion tr(j->ctx, TK_PROJ);
-            rc = launch_projector(g, j->trig[iv], PROJ_IMAGE, a, st);
+            rc = launch_projector(g, j->trig[iv], vi, PROJ_IMAGE, a, st, &nparts);
             if (rc) return rc;
         }
-        {   // u = B(r) / B(P1)  (:212-218)
+        {   // u = B(r) / B(P1)  (:212-218); also rho_i from the projector's partials
+            const NormFinish nf{j->d_partials, nparts, j->d_norm + (long long)it * j->V + iv};
             TimedRegion tr(j->ctx, TK_BACK);
-            rc = launch_backprojector(g, j->trig[iv], BACK_RATIO, nullptr, j->d_rp, j->d_w2, st);
+            rc = launch_backprojector(g, j->trig[iv], BACK_RATIO, nullptr, j->d_rp, j->d_w2, st,
+                                      &nf);
             if (rc) return rc;
         }
         {   // x += lambda * W(u; v_0 - v_i, f_0 - f_i)  (:222-223)
@@ -123,7 +146,9 @@ int image_sweep(mcsart_job *j, int it) {
                                      j->volumes[0] - j->volumes[iv], j->flows[0] - j->flows[iv],
                                      f, j->opts.lambda, j->opts.nonneg, st);
             else
-                rc = launch_add_clamp(j->d_img, j->d_w2, N, j->opts.lambda, j->opts.nonneg, st);
+                rc = launch_add_tile(j->d_img,
+                                     (iv + 1 < j->V && j->vinfo[iv + 1].has_x) ? j->d_w1T : nullptr,
+                                     j->d_w2, g.nx, g.ny, g.nz, j->opts.lambda, j->opts.nonneg, st);
             if (rc) return rc;
         }
     }
@@ -139,31 +164,34 @@ int motion_sweep(mcsart_job *j, int it) {
     for (int c = 0; c < 6; ++c) fl[c] = j->d_fields + c * N;
     const float *cfl[6] = {fl[0], fl[1], fl[2], fl[3], fl[4], fl[5]};
     const int stride = j->opts.motion ? 1 : 10;
-    int rc;
+    int rc, nparts = 0;
     for (int iv = 1; iv < j->V; iv += stride) {
         const float vd = j->volumes[iv] - j->volumes[iv - 1];   // :249
         const float fd = j->flows[iv] - j->flows[iv - 1];       // :250
+        const ViewInfo &vi = j->vinfo[iv];
+        float2 *pair = (float2 *)j->d_w1, *pairT = (float2 *)j->d_w1T;
         {
             TimedRegion tr(j->ctx, TK_MOTION);
-            rc = launch_motion_prep(j->d_w1, j->d_w2, j->d_img, cfl, g.nx, g.ny, g.nz, vd, fd, st);
+            rc = launch_motion_prep(vi.has_y ? pair : nullptr, vi.has_x ? pairT : nullptr,
+                                    j->d_img, cfl, g.nx, g.ny, g.nz, vd, fd, st);
             if (rc) return rc;
         }
         {   // r = p_i - P(x - img0), P(q), rho_i  (:246-263)
             ProjArgs a{};
-            a.vol0 = j->d_w1;
-            a.vol1 = j->d_w2;
+            a.pair = pair;
+            a.pairT = vi.has_x ? pairT : nullptr;
             a.meas = j->d_proj + D * iv;
             a.rp = j->d_rp;
             a.partials = j->d_partials;
-            a.counter = j->d_counter;
-            a.norm_out = j->d_norm + (long long)it * j->V + iv;
             TimedRegion tr(j->ctx, TK_PROJ);
-            rc = launch_projector(g, j->trig[iv], PROJ_MOTION, a, st);
+            rc = launch_projector(g, j->trig[iv], vi, PROJ_MOTION, a, st, &nparts);
             if (rc) return rc;
         }
-        {   // u = B(r) / B(P q)  (:265-270)
+        {   // u = B(r) / B(P q)  (:265-270); also rho_i
+            const NormFinish nf{j->d_partials, nparts, j->d_norm + (long long)it * j->V + iv};
             TimedRegion tr(j->ctx, TK_BACK);
-            rc = launch_backprojector(g, j->trig[iv], BACK_RATIO, nullptr, j->d_rp, j->d_w2, st);
+            rc = launch_backprojector(g, j->trig[iv], BACK_RATIO, nullptr, j->d_rp, j->d_w2, st,
+                                      &nf);
             if (rc) return rc;
         }
         {   // fields += u^T * (-vd | -fd) * D_c x  (:272-277)
@@ -216,7 +244,12 @@ extern "C" int mcsart_job_create(mcsart_ctx *ctx, const mcsart_geom *gp, int n_v
     j->angles.assign(angles, angles + n_views);
     j->volumes.assign(volumes, volumes + n_views);
     j->flows.assign(flows, flows + n_views);
-    for (int i = 0; i < n_views; ++i) j->trig.push_back(view_trig(angles[i]));
+    for (int i = 0; i < n_views; ++i) {
+        j->trig.push_back(view_trig(angles[i]));
+        j->vinfo.push_back(view_info_of(g, j->trig.back()));
+    }
+    j->graphs.resize(n_iter);
+    if (const char *env = getenv("MCSART_GRAPHS")) j->use_graphs = atoi(env) != 0;
     if (hipMalloc(&j->block, L.total) != hipSuccess) {
         (void)hipGetLastError();
         delete j;
@@ -229,13 +262,13 @@ extern "C" int mcsart_job_create(mcsart_ctx *ctx, const mcsart_geom *gp, int n_v
     j->d_fields = (float *)(b + L.fields);
     j->d_init = (float *)(b + L.init);
     j->d_w1 = (float *)(b + L.w1);
+    j->d_w1T = (float *)(b + L.w1T);
     j->d_w2 = (float *)(b + L.w2);
     j->d_rp = (float2 *)(b + L.rp);
     j->d_norm = (float *)(b + L.norm);
     j->d_resid = opts->keep_residual ? (float *)(b + L.resid) : nullptr;
     j->d_partials = (double *)(b + L.partials);
-    j->d_counter = (unsigned *)(b + L.counter);
-    // everything starts at zero: image, fields, norms, residuals, reduction counter
+    // everything starts at zero: image, fields, norms, residuals
     if (hipMemsetAsync(j->block, 0, L.total, ctx->stream) != hipSuccess ||
         hipStreamSynchronize(ctx->stream) != hipSuccess) {
         hipFree(j->block);
@@ -310,11 +343,75 @@ extern "C" int mcsart_job_reset(mcsart_job *j) {
     return rc;
 }
 
+namespace {
+
+void drop_graph(mcsart_job::SweepGraph &sg) {
+    if (sg.exec) hipGraphExecDestroy(sg.exec);
+    if (sg.graph) hipGraphDestroy(sg.graph);
+    for (auto &p : sg.ev) {
+        hipEventDestroy(p.a);
+        hipEventDestroy(p.b);
+    }
+    sg = mcsart_job::SweepGraph{};
+}
+
+int run_sweep_eager(mcsart_job *j, int it) {
+    return (j->op_iter[it] == 1) ? image_sweep(j, it) : motion_sweep(j, it);
+}
+
+int capture_sweep(mcsart_job *j, int it) {
+    mcsart_ctx *ctx = j->ctx;
+    auto &sg = j->graphs[it];
+    drop_graph(sg);
+    MCSART_HIP_TRY(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeRelaxed));
+    ctx->capture = &sg.ev;
+    const int rc = run_sweep_eager(j, it);
+    ctx->capture = nullptr;
+    hipGraph_t graph = nullptr;
+    const hipError_t e = hipStreamEndCapture(ctx->stream, &graph);
+    sg.graph = graph;
+    if (rc || e != hipSuccess) {
+        drop_graph(sg);
+        return rc ? rc : fail(MCSART_EHIP, std::string("stream capture: ") + hipGetErrorString(e));
+    }
+    if (hipGraphInstantiate(&sg.exec, graph, nullptr, nullptr, 0) != hipSuccess) {
+        drop_graph(sg);
+        return fail(MCSART_EHIP, "hipGraphInstantiate failed");
+    }
+    sg.timed = ctx->timing;
+    return MCSART_OK;
+}
+
+}  // namespace
+
+extern "C" int mcsart_job_set_graphs(mcsart_job *j, int enable) {
+    int rc = check_job(j);
+    if (rc) return rc;
+    j->use_graphs = enable != 0;
+    return MCSART_OK;
+}
+
 extern "C" int mcsart_job_run_sweep(mcsart_job *j, int it) {
     int rc = check_job(j);
     if (rc) return rc;
     if (it < 0 || it >= j->n_iter) return fail(MCSART_EINVAL, "sweep index out of range");
-    return (j->op_iter[it] == 1) ? image_sweep(j, it) : motion_sweep(j, it);
+    if (!j->use_graphs) return run_sweep_eager(j, it);
+    mcsart_ctx *ctx = j->ctx;
+    auto &sg = j->graphs[it];
+    if (!sg.exec || sg.timed != ctx->timing) {
+        rc = capture_sweep(j, it);
+        if (rc) return rc;
+    }
+    if (ctx->timing && sg.queued) {   // read the previous replay's times before re-recording
+        rc = ctx_collect_timing(ctx);
+        if (rc) return rc;
+    }
+    MCSART_HIP_TRY(hipGraphLaunch(sg.exec, ctx->stream));
+    if (ctx->timing) {
+        for (const auto &p : sg.ev) ctx->pending.push_back(p);
+        sg.queued = true;
+    }
+    return MCSART_OK;
 }
 
 extern "C" int mcsart_job_run(mcsart_job *j) {
@@ -378,6 +475,11 @@ extern "C" int mcsart_job_destroy(mcsart_job *j) {
     if (!j) return MCSART_OK;
     hipSetDevice(j->ctx->device);
     hipStreamSynchronize(j->ctx->stream);
+    // timing entries of this job's graphs must not outlive their events
+    bool owned = false;
+    for (auto &sg : j->graphs) owned = owned || sg.queued;
+    if (owned) ctx_collect_timing(j->ctx);
+    for (auto &sg : j->graphs) drop_graph(sg);
     hipFree(j->block);
     delete j;
     return MCSART_OK;

@@ -176,4 +176,56 @@ __host__ __device__ inline bool plane_fp(const Ray &R, float p, int nt, int nz, 
 
 __host__ __device__ inline int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
+// ------------------------------------------------------------------------------------------
+// Per-view facts the host needs before launching the projector:
+//   has_x / has_y : whether any ray takes the x-branch / y-branch (the branch depends on the
+//                   detector column ia only: x21 and y21 do not involve ib);
+//   wt / wz       : the most voxels any footprint can cover along the transverse axis / z, i.e.
+//                   floor(max width) + 2.  The transverse width of ray ia in the plane at
+//                   distance xc from the source is (rmax - rmin)*|xc|, the z width db*|xc|/|x21|
+//                   (x-branch; mirrored for the y-branch); |xc| is largest at an end plane.
+// ------------------------------------------------------------------------------------------
+struct ViewInfo {
+    int has_x, has_y;
+    int wt, wz;
+};
+
+inline ViewInfo view_info(const GeomArgs &g, double c, double s) {
+    ViewInfo v{0, 0, 0, 0};
+    double wt_max = 0.0, wz_max = 0.0;
+    const double SO = g.SO, SD = g.SD, da = g.da, db = g.db;
+    for (int ia = 0; ia < g.na; ++ia) {
+        const double y20 = ((double)ia + (double)g.ai) * da;
+        const double x21 = (SD - SO) * c - y20 * s + SO * c;
+        const double y21 = (SD - SO) * s + y20 * c + SO * s;
+        const double h = da / 2;
+        const double xl = x21 - h * s, xr = x21 + h * s, yl = y21 - h * c, yr = y21 + h * c;
+        double r0, r1, r2, r3, dist, zslope;
+        int np;
+        const bool xb = fabs(x21) > fabs(y21);
+        if (xb) {
+            r0 = yl / xl; r1 = yl / xr; r2 = yr / xl; r3 = yr / xr;
+            dist = SO * c;
+            zslope = db / fabs(x21);
+            np = g.nx;
+            v.has_x = 1;
+        } else {
+            r0 = xl / yl; r1 = xl / yr; r2 = xr / yl; r3 = xr / yr;
+            dist = SO * s;
+            zslope = db / fabs(y21);
+            np = g.ny;
+            v.has_y = 1;
+        }
+        const double dr = fmax(fmax(r0, r1), fmax(r2, r3)) - fmin(fmin(r0, r1), fmin(r2, r3));
+        const double p0 = 0.5 - 0.5 * np, p1 = p0 + (np - 1);
+        const double xc = fmax(fabs(p0 + dist), fabs(p1 + dist));
+        wt_max = fmax(wt_max, dr * xc);
+        wz_max = fmax(wz_max, zslope * xc);
+    }
+    // a small margin absorbs the FP32 rounding of the per-plane bounds
+    v.wt = (int)floor(wt_max * (1.0 + 1e-5) + 1e-5) + 2;
+    v.wz = (int)floor(wz_max * (1.0 + 1e-5) + 1e-5) + 2;
+    return v;
+}
+
 }  // namespace mcsart

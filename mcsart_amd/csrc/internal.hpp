@@ -8,6 +8,7 @@
 #include <string>
 #include <vector>
 
+#include "footprint.hpp"
 #include "mcsart_hip.h"
 
 namespace mcsart {
@@ -57,36 +58,76 @@ enum ProjMode {
     PROJ_MOTION = 2,  // rp[id] = (p[id] - P(g), P(q)), norms of p and r
 };
 
+// Volumes the projector reads.  y-branch rays read the natural layout img[ix + iy*nx + iz*nx*ny];
+// x-branch rays read the xy-transposed layout imgT[iy + ix*ny + iz*nx*ny] when it is given (it
+// makes their gathers contiguous) and the natural layout otherwise.  PLAIN/IMAGE project vol0;
+// MOTION projects the interleaved pairs (g, q).
 struct ProjArgs {
-    const float *vol0;     // x (PLAIN/IMAGE) or g (MOTION)
-    const float *vol1;     // q (MOTION)
+    const float *vol0;     // natural layout (PLAIN/IMAGE)
+    const float *vol0T;    // transposed layout or null
+    const float2 *pair;    // (g, q) natural layout (MOTION)
+    const float2 *pairT;   // (g, q) transposed layout or null (MOTION)
     const float *meas;     // measured projection of this view (IMAGE/MOTION)
     float *out;            // PLAIN: D floats
     float2 *rp;            // IMAGE/MOTION: D float2 (residual, normaliser projection)
     float *resid_copy;     // optional: residual copy (reference h_outproj slice)
-    double *partials;      // per-block (sum p^2, sum r^2)
-    unsigned *counter;     // last-block-done counter (left at zero)
-    float *norm_out;       // ||r|| / ||p|| destination
+    double *partials;      // per-block (sum p^2, sum r^2), plain stores
 };
-int launch_projector(const Geom &g, const ViewTrig &t, ProjMode mode, const ProjArgs &a,
-                     hipStream_t st);
+// vi: the view's branches and footprint spans (footprint.hpp view_info).  SART modes leave one
+// (sum p^2, sum r^2) pair per workgroup in a.partials and report their count in *nparts; a later
+// kernel on the same stream turns them into ||r||/||p|| (NormFinish).  No fence or atomic is
+// needed: the kernel boundary orders the stores.
+int launch_projector(const Geom &g, const ViewTrig &t, const ViewInfo &vi, ProjMode mode,
+                     const ProjArgs &a, hipStream_t st, int *nparts = nullptr);
+ViewInfo view_info_of(const Geom &g, const ViewTrig &t);
 int projector_partials(const Geom &g);   // number of per-block partial slots needed
+
+// Per-view residual ratio ||r|| / ||p|| (src/AUMISART.cu:204-209) from projector partials,
+// summed in a fixed order by one wavefront.
+struct NormFinish {
+    const double *partials;
+    int nparts;
+    float *out;
+};
+__device__ inline void finish_norm_wave(const NormFinish &nf) {
+    const int lane = threadIdx.x & 63;
+    double sp = 0.0, sr = 0.0;
+    for (int k = lane; k < nf.nparts; k += 64) {
+        sp += nf.partials[2 * k];
+        sr += nf.partials[2 * k + 1];
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        sp += __shfl_xor(sp, off, 64);
+        sr += __shfl_xor(sr, off, 64);
+    }
+    if (lane == 0) *nf.out = (float)sqrt(sr) / (float)sqrt(sp);
+}
+int launch_norm_finish(const NormFinish &nf, hipStream_t st);
 
 // ---- backprojector --------------------------------------------------------------------------
 enum BackMode {
     BACK_PLAIN = 0,  // out = B(proj)
     BACK_RATIO = 1,  // out = B(rp.x) / B(rp.y), NaN/Inf -> 0
 };
+// nf (optional): the first wavefront of the first workgroup also finishes the preceding
+// projector's norm reduction.
 int launch_backprojector(const Geom &g, const ViewTrig &t, BackMode mode, const float *proj,
-                         const float2 *rp, float *out, hipStream_t st);
+                         const float2 *rp, float *out, hipStream_t st,
+                         const NormFinish *nf = nullptr);
 
 // ---- warps and elementwise ------------------------------------------------------------------
 struct Fields {
     const float *ax, *ay, *az, *bx, *by, *bz;
 };
-// out = W(src; vol, flow); out must not alias src.
-int launch_warp(float *out, const float *src, int nx, int ny, int nz, float vol, float flow,
-                const Fields &f, hipStream_t st);
+// out = W(src; vol, flow) in the natural layout and/or (outT) the xy-transposed layout; either
+// output may be null; neither may alias src.
+int launch_warp(float *out, float *outT, const float *src, int nx, int ny, int nz, float vol,
+                float flow, const Fields &f, hipStream_t st);
+// dst = xy-transpose of src (dst[iy + ix*ny + iz*nx*ny] = src[ix + iy*nx + iz*nx*ny])
+int launch_transpose(float *dst, const float *src, int nx, int ny, int nz, hipStream_t st);
+// x += lambda * u, optional clamp, and (xT non-null) the transposed copy of the result
+int launch_add_tile(float *x, float *xT, const float *u, int nx, int ny, int nz, float lambda,
+                    int nonneg, hipStream_t st);
 // x += lambda * W(u; vol, flow), then optional clamp at zero.
 int launch_warp_add(float *x, const float *u, int nx, int ny, int nz, float vol, float flow,
                     const Fields &f, float lambda, int nonneg, hipStream_t st);
@@ -109,7 +150,8 @@ int launch_update_udvf(float *ax, float *ay, float *az, float *bx, float *by, fl
                        int nz, hipStream_t st);
 // Motion sweep, fused (src/AUMISART.cu:246-277):
 //   g = x - sum_c (alpha_c^T vd + beta_c^T fd) * D_c x,  q = (D1+D2+D3)x * (-vd - fd)
-int launch_motion_prep(float *g_out, float *q_out, const float *x, const float *const fields[6],
+// Pairs (g, q) go to `pair` (natural layout) and/or `pairT` (transposed); either may be null.
+int launch_motion_prep(float2 *pair, float2 *pairT, const float *x, const float *const fields[6],
                        int nx, int ny, int nz, float vd, float fd, hipStream_t st);
 //   alpha_c += u^T * wv * D_c x,  beta_c += u^T * wf * D_c x
 int launch_motion_update(float *const fields[6], const float *u, const float *x, int nx, int ny,
@@ -128,15 +170,18 @@ struct mcsart_ctx {
     struct Pending {
         int kind;
         hipEvent_t a, b;
+        bool pooled;   // false: the events belong to a captured graph (re-recorded per replay)
     };
     std::vector<Pending> pending;
     std::vector<hipEvent_t> event_pool;
+    // while a sweep is being captured into a graph, timed regions record into this list
+    std::vector<Pending> *capture = nullptr;
     double total_ms[mcsart::TK_COUNT] = {0, 0, 0, 0};
     long long launches[mcsart::TK_COUNT] = {0, 0, 0, 0};
     // scratch for the synchronous operator shims
     void *scratch = nullptr;
     size_t scratch_bytes = 0;
-    // norm-reduction scratch (per-block partials + a counter that every launch leaves at zero)
+    // norm-reduction scratch (per-block partials)
     void *red = nullptr;
     size_t red_bytes = 0;
 };
@@ -144,9 +189,8 @@ struct mcsart_ctx {
 namespace mcsart {
 mcsart_ctx *default_ctx();   // per-device context behind the synchronous operators
 int ctx_scratch(mcsart_ctx *ctx, size_t bytes, void **ptr);
-// Reduction scratch for the projector's SART modes: 2*nblocks doubles of partials followed by
-// the last-block counter, zero-filled when (re)allocated.
-int ctx_reduction(mcsart_ctx *ctx, int nblocks, double **partials, unsigned **counter);
+// Reduction scratch for the projector's SART modes: 2*nblocks doubles of partials.
+int ctx_reduction(mcsart_ctx *ctx, int nblocks, double **partials);
 // Timing helpers: bracket a launch with events when ctx->timing is on.
 struct TimedRegion {
     mcsart_ctx *ctx;

@@ -23,7 +23,11 @@ template <int MODE>
 __global__ void __launch_bounds__(BX *BY) k_backproject(GeomArgs g, double c, double s,
                                                        const float *__restrict__ proj,
                                                        const float2 *__restrict__ rp,
-                                                       float *__restrict__ out, int zslab) {
+                                                       float *__restrict__ out, int zslab,
+                                                       NormFinish nf) {
+    // the preceding projector's per-view norm (its partials are complete: kernel boundary)
+    if (nf.out && blockIdx.x == 0 && blockIdx.y == 0 && blockIdx.z == 0 && threadIdx.y == 0)
+        finish_norm_wave(nf);
     const int ix = blockIdx.x * BX + threadIdx.x;
     const int iy = blockIdx.y * BY + threadIdx.y;
     if (ix >= g.nx || iy >= g.ny) return;
@@ -85,17 +89,18 @@ int pick_zslab(const Geom &g) {
 }  // namespace
 
 int launch_backprojector(const Geom &g, const ViewTrig &t, BackMode mode, const float *proj,
-                         const float2 *rp, float *out, hipStream_t st) {
+                         const float2 *rp, float *out, hipStream_t st, const NormFinish *nfp) {
     const GeomArgs ga{g.nx, g.ny, g.nz, g.na, g.nb, g.da, g.db, g.ai, g.bi, g.SO, g.SD};
     const int zs = pick_zslab(g);
     const dim3 block(BX, BY);
     const dim3 grid((g.nx + BX - 1) / BX, (g.ny + BY - 1) / BY, (g.nz + zs - 1) / zs);
+    const NormFinish nf = nfp ? *nfp : NormFinish{nullptr, 0, nullptr};
     if (mode == BACK_PLAIN)
         hipLaunchKernelGGL(k_backproject<BACK_PLAIN>, grid, block, 0, st, ga, t.c, t.s, proj,
-                           rp, out, zs);
+                           rp, out, zs, nf);
     else
         hipLaunchKernelGGL(k_backproject<BACK_RATIO>, grid, block, 0, st, ga, t.c, t.s, proj,
-                           rp, out, zs);
+                           rp, out, zs, nf);
     MCSART_CHECK_LAUNCH();
     return MCSART_OK;
 }

@@ -204,9 +204,11 @@ __device__ inline void load_transposed(float (*t)[TT + 1], const float *__restri
 }
 
 // g = x - img0, img0 = sum over (alpha_y,D1),(alpha_x,D2),(alpha_z,D3) of alpha^T*vd*D x plus the
-// same with beta and fd (src/AUMISART.cu:252-258); q = (D1+D2+D3)x * (-vd - fd) (:267).
-__global__ void __launch_bounds__(TT *TR) k_motion_prep(float *__restrict__ g_out,
-                                                       float *__restrict__ q_out,
+// same with beta and fd (src/AUMISART.cu:252-258); q = (D1+D2+D3)x * (-vd - fd) (:267).  The pair
+// (g, q) is written interleaved in the natural layout (pair) and/or, through the LDS tile, in the
+// xy-transposed layout (pairT), whichever the next projection reads.
+__global__ void __launch_bounds__(TT *TR) k_motion_prep(float2 *__restrict__ pair,
+                                                       float2 *__restrict__ pairT,
                                                        const float *__restrict__ x, Fields f,
                                                        int nx, int ny, int nz, float vd, float fd) {
     __shared__ float t[6][TT][TT + 1];
@@ -220,11 +222,15 @@ __global__ void __launch_bounds__(TT *TR) k_motion_prep(float *__restrict__ g_ou
     __syncthreads();
     const int tx = threadIdx.x;
     const int i0 = X0 + tx;
-    if (i0 >= nx) return;
-    for (int r = threadIdx.y; r < TT; r += TR) {
+    const long long plane = (long long)nx * ny;
+    float gv[TT / TR], qv[TT / TR];
+#pragma unroll
+    for (int k = 0; k < TT / TR; ++k) {
+        const int r = threadIdx.y + k * TR;
         const int i1 = Y0 + r;
-        if (i1 >= ny) break;
-        const long long id = i0 + (long long)i1 * nx + (long long)i2 * nx * ny;
+        gv[k] = qv[k] = 0.f;
+        if (i0 >= nx || i1 >= ny) continue;
+        const long long id = i0 + (long long)i1 * nx + (long long)i2 * plane;
         const float d1 = dfwd(x, id, i0, i1, i2, nx, ny, nz, 1);
         const float d2 = dfwd(x, id, i0, i1, i2, nx, ny, nz, 2);
         const float d3 = dfwd(x, id, i0, i1, i2, nx, ny, nz, 3);
@@ -235,9 +241,106 @@ __global__ void __launch_bounds__(TT *TR) k_motion_prep(float *__restrict__ g_ou
         img0 += t[4][tx][r] * fd * d1;   // beta_y
         img0 += t[3][tx][r] * fd * d2;   // beta_x
         img0 += t[5][tx][r] * fd * d3;   // beta_z
-        g_out[id] = x[id] - img0;
-        q_out[id] = (d1 + d2 + d3) * (-vd + -fd);
+        gv[k] = x[id] - img0;
+        qv[k] = (d1 + d2 + d3) * (-vd + -fd);
+        if (pair) pair[id] = make_float2(gv[k], qv[k]);
     }
+    if (pairT) {   // uniform over the workgroup
+        __syncthreads();   // every lane has finished reading the field tiles
+#pragma unroll
+        for (int k = 0; k < TT / TR; ++k) {
+            const int r = threadIdx.y + k * TR;
+            t[0][r][tx] = gv[k];
+            t[1][r][tx] = qv[k];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < TT / TR; ++k) {
+            const int r = threadIdx.y + k * TR;
+            const int j0 = Y0 + tx, j1 = X0 + r;   // transposed element (iy = j0, ix = j1)
+            if (j0 < ny && j1 < nx)
+                pairT[j0 + (long long)j1 * ny + (long long)i2 * plane] = make_float2(t[0][tx][r], t[1][tx][r]);
+        }
+    }
+}
+
+// Store a 32x32 tile held in registers (lane (tx, r) owns voxel (X0 + tx, Y0 + r)) in the
+// xy-transposed layout through LDS; lane (tx, r) then writes element (iy = Y0 + tx, ix = X0 + r).
+__device__ inline void store_transposed(float (*t)[TT + 1], float *__restrict__ outT,
+                                        const float *v, int nx, int ny, int X0, int Y0, int i2) {
+    const int tx = threadIdx.x;
+#pragma unroll
+    for (int k = 0; k < TT / TR; ++k) t[threadIdx.y + k * TR][tx] = v[k];
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < TT / TR; ++k) {
+        const int r = threadIdx.y + k * TR;
+        const int j0 = Y0 + tx, j1 = X0 + r;
+        if (j0 < ny && j1 < nx)
+            outT[j0 + (long long)j1 * ny + (long long)i2 * nx * ny] = t[tx][r];
+    }
+}
+
+// Forward warp over 32x32 tiles: natural output (out) and/or transposed output (outT).
+__global__ void __launch_bounds__(TT *TR) k_warp_tile(float *__restrict__ out,
+                                                     float *__restrict__ outT,
+                                                     const float *__restrict__ src, int nx, int ny,
+                                                     int nz, float vol, float flow, Fields f) {
+    __shared__ float t[TT][TT + 1];
+    const int X0 = blockIdx.x * TT, Y0 = blockIdx.y * TT, i2 = blockIdx.z;
+    const int i0 = X0 + threadIdx.x;
+    float v[TT / TR];
+#pragma unroll
+    for (int k = 0; k < TT / TR; ++k) {
+        const int i1 = Y0 + threadIdx.y + k * TR;
+        v[k] = 0.f;
+        if (i0 >= nx || i1 >= ny) continue;
+        const long long id = i0 + (long long)i1 * nx + (long long)i2 * nx * ny;
+        const float mx = f.ax[id] * vol + f.bx[id] * flow;   // src/cu_deform.cu:74-76
+        const float my = f.ay[id] * vol + f.by[id] * flow;
+        const float mz = f.az[id] * vol + f.bz[id] * flow;
+        v[k] = trilinear(src, nx, ny, nz, i0, my, i1, mx, i2, mz);   // :85-89
+        if (out) out[id] = v[k];
+    }
+    if (outT) store_transposed(t, outT, v, nx, ny, X0, Y0, i2);
+}
+
+// x += lambda * u (optional clamp at zero) over 32x32 tiles, with the transposed copy of the
+// result when xT is given.
+__global__ void __launch_bounds__(TT *TR) k_add_tile(float *__restrict__ x, float *__restrict__ xT,
+                                                    const float *__restrict__ u, int nx, int ny,
+                                                    int nz, float lambda, int nonneg) {
+    __shared__ float t[TT][TT + 1];
+    const int X0 = blockIdx.x * TT, Y0 = blockIdx.y * TT, i2 = blockIdx.z;
+    const int i0 = X0 + threadIdx.x;
+    float v[TT / TR];
+#pragma unroll
+    for (int k = 0; k < TT / TR; ++k) {
+        const int i1 = Y0 + threadIdx.y + k * TR;
+        v[k] = 0.f;
+        if (i0 >= nx || i1 >= ny) continue;
+        const long long id = i0 + (long long)i1 * nx + (long long)i2 * nx * ny;
+        float y = x[id] + u[id] * lambda;   // src/cu_add.cu:17
+        if (nonneg) y = fmaxf(y, 0.f);
+        x[id] = y;
+        v[k] = y;
+    }
+    if (xT) store_transposed(t, xT, v, nx, ny, X0, Y0, i2);
+}
+
+__global__ void __launch_bounds__(TT *TR) k_transpose(float *__restrict__ dst,
+                                                     const float *__restrict__ src, int nx, int ny,
+                                                     int nz) {
+    __shared__ float t[TT][TT + 1];
+    const int X0 = blockIdx.x * TT, Y0 = blockIdx.y * TT, i2 = blockIdx.z;
+    const int i0 = X0 + threadIdx.x;
+    float v[TT / TR];
+#pragma unroll
+    for (int k = 0; k < TT / TR; ++k) {
+        const int i1 = Y0 + threadIdx.y + k * TR;
+        v[k] = (i0 < nx && i1 < ny) ? src[i0 + (long long)i1 * nx + (long long)i2 * nx * ny] : 0.f;
+    }
+    store_transposed(t, dst, v, nx, ny, X0, Y0, i2);
 }
 
 // alpha_c += u^T * wv * D_c x, beta_c += u^T * wf * D_c x with the pairs (alpha_y, D1),
@@ -278,10 +381,29 @@ inline dim3 lin_grid(long long n) { return dim3((unsigned)((n + 255) / 256)); }
 
 }  // namespace
 
-int launch_warp(float *out, const float *src, int nx, int ny, int nz, float vol, float flow,
-                const Fields &f, hipStream_t st) {
-    hipLaunchKernelGGL(k_warp<false>, vgrid(nx, ny, nz), dim3(EX, EY), 0, st, out, src, nx, ny,
-                       nz, vol, flow, f, 0.f, 0);
+inline dim3 tile_grid(int nx, int ny, int nz) {
+    return dim3((nx + TT - 1) / TT, (ny + TT - 1) / TT, nz);
+}
+
+int launch_warp(float *out, float *outT, const float *src, int nx, int ny, int nz, float vol,
+                float flow, const Fields &f, hipStream_t st) {
+    hipLaunchKernelGGL(k_warp_tile, tile_grid(nx, ny, nz), dim3(TT, TR), 0, st, out, outT, src,
+                       nx, ny, nz, vol, flow, f);
+    MCSART_CHECK_LAUNCH();
+    return MCSART_OK;
+}
+
+int launch_transpose(float *dst, const float *src, int nx, int ny, int nz, hipStream_t st) {
+    hipLaunchKernelGGL(k_transpose, tile_grid(nx, ny, nz), dim3(TT, TR), 0, st, dst, src, nx, ny,
+                       nz);
+    MCSART_CHECK_LAUNCH();
+    return MCSART_OK;
+}
+
+int launch_add_tile(float *x, float *xT, const float *u, int nx, int ny, int nz, float lambda,
+                    int nonneg, hipStream_t st) {
+    hipLaunchKernelGGL(k_add_tile, tile_grid(nx, ny, nz), dim3(TT, TR), 0, st, x, xT, u, nx, ny,
+                       nz, lambda, nonneg);
     MCSART_CHECK_LAUNCH();
     return MCSART_OK;
 }
@@ -358,12 +480,11 @@ int launch_update_udvf(float *ax, float *ay, float *az, float *bx, float *by, fl
     return MCSART_OK;
 }
 
-int launch_motion_prep(float *g_out, float *q_out, const float *x, const float *const fields[6],
+int launch_motion_prep(float2 *pair, float2 *pairT, const float *x, const float *const fields[6],
                        int nx, int ny, int nz, float vd, float fd, hipStream_t st) {
     const Fields f{fields[0], fields[1], fields[2], fields[3], fields[4], fields[5]};
-    const dim3 grid((nx + TT - 1) / TT, (ny + TT - 1) / TT, nz);
-    hipLaunchKernelGGL(k_motion_prep, grid, dim3(TT, TR), 0, st, g_out, q_out, x, f, nx, ny, nz,
-                       vd, fd);
+    hipLaunchKernelGGL(k_motion_prep, tile_grid(nx, ny, nz), dim3(TT, TR), 0, st, pair, pairT, x,
+                       f, nx, ny, nz, vd, fd);
     MCSART_CHECK_LAUNCH();
     return MCSART_OK;
 }

@@ -234,6 +234,9 @@ class Job:
     def reset(self):
         check(self._lib.mcsart_job_reset(self.ptr), "mcsart_job_reset")
 
+    def set_graphs(self, enable: bool):
+        check(self._lib.mcsart_job_set_graphs(self.ptr, int(enable)), "mcsart_job_set_graphs")
+
     def run(self):
         check(self._lib.mcsart_job_run(self.ptr), "mcsart_job_run")
 

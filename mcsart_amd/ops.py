@@ -164,13 +164,13 @@ def diff_forward(img1, img, nx, ny, nz, ind):
     check(load().mcsart_diff_forward(_p(img1), _p(img), nx, ny, nz, ind), "mcsart_diff_forward")
 
 
-def sart_residual(d_rp, d_norm, d_img, d_img2, d_meas, angle, SO, SD, da, na, ai, db, nb, bi,
-                  nx, ny, nz):
-    """Fused residual projector (src/AUMISART.cu:194-209 / :246-263): d_rp gets (r, n) pairs."""
-    check(load().mcsart_sart_residual(_p(d_rp), _p(d_norm), _p(d_img),
-                                      None if d_img2 is None else _p(d_img2), _p(d_meas), angle,
-                                      SO, SD, da, na, ai, db, nb, bi, nx, ny, nz),
-          "mcsart_sart_residual")
+def sart_residual(d_rp, d_norm, d_vol, motion_pairs, d_meas, angle, SO, SD, da, na, ai, db, nb,
+                  bi, nx, ny, nz):
+    """Fused residual projector (src/AUMISART.cu:194-209 / :246-263): d_rp gets (r, n) pairs.
+    motion_pairs False: d_vol is x, n = P(1); True: d_vol holds (g, q) pairs, n = P(q)."""
+    check(load().mcsart_sart_residual(_p(d_rp), _p(d_norm), _p(d_vol), int(bool(motion_pairs)),
+                                      _p(d_meas), angle, SO, SD, da, na, ai, db, nb, bi, nx, ny,
+                                      nz), "mcsart_sart_residual")
 
 
 def sart_backprojection(d_u, d_rp, angle, SO, SD, da, na, ai, db, nb, bi, nx, ny, nz):
@@ -179,9 +179,9 @@ def sart_backprojection(d_u, d_rp, angle, SO, SD, da, na, ai, db, nb, bi, nx, ny
                                             bi, nx, ny, nz), "mcsart_sart_backprojection")
 
 
-def motion_prep(d_g, d_q, d_img, ax, ay, az, bx, by, bz, vd, fd, nx, ny, nz):
-    """g = x - img0, q = (D1+D2+D3)x (-vd-fd)  (src/AUMISART.cu:252-258, :267)."""
-    check(load().mcsart_motion_prep(_p(d_g), _p(d_q), _p(d_img), _p(ax), _p(ay), _p(az), _p(bx),
+def motion_prep(d_pairs, d_img, ax, ay, az, bx, by, bz, vd, fd, nx, ny, nz):
+    """(g, q) pairs: g = x - img0, q = (D1+D2+D3)x (-vd-fd)  (src/AUMISART.cu:252-258, :267)."""
+    check(load().mcsart_motion_prep(_p(d_pairs), _p(d_img), _p(ax), _p(ay), _p(az), _p(bx),
                                     _p(by), _p(bz), vd, fd, nx, ny, nz), "mcsart_motion_prep")
 
 

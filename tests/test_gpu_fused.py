@@ -36,7 +36,7 @@ def test_sart_residual_image_mode(gpu_lib, scene, view):
     meas = oracle.projection(other, g, ang).astype(np.float32)
     d_rp = DeviceArray.empty((g.nb, g.na, 2))
     d_n = DeviceArray.empty((1,))
-    sart_residual(d_rp, d_n, DeviceArray.from_host(x), None, DeviceArray.from_host(meas),
+    sart_residual(d_rp, d_n, DeviceArray.from_host(x), False, DeviceArray.from_host(meas),
                   *_geo_args(g, ang))
     rp = d_rp.to_host()
     r_ref = meas - oracle.projection(x, g, ang)
@@ -52,10 +52,11 @@ def test_sart_residual_motion_mode(gpu_lib, scene):
     ang = spec.angles[33]
     meas = oracle.projection(other, g, ang).astype(np.float32)
     q = smooth_random(g.vol_shape, 5) - 0.5
+    pairs = np.stack([x, q], axis=-1)
     d_rp = DeviceArray.empty((g.nb, g.na, 2))
     d_n = DeviceArray.empty((1,))
-    sart_residual(d_rp, d_n, DeviceArray.from_host(x), DeviceArray.from_host(q),
-                  DeviceArray.from_host(meas), *_geo_args(g, ang))
+    sart_residual(d_rp, d_n, DeviceArray.from_host(pairs), True, DeviceArray.from_host(meas),
+                  *_geo_args(g, ang))
     rp = d_rp.to_host()
     assert rel_l2(rp[..., 0], meas - oracle.projection(x, g, ang)) <= TOL
     assert rel_l2(rp[..., 1], oracle.projection(q, g, ang)) <= TOL
@@ -95,15 +96,16 @@ def test_motion_prep(gpu_lib):
     x = smooth_random(shape, 31)
     F = random_fields(shape, 32)
     vd, fd = np.float32(0.043), np.float32(-1.37)
-    d_g, d_q = DeviceArray.empty(shape), DeviceArray.empty(shape)
+    d_pairs = DeviceArray.empty(shape + (2,))
     dF = [DeviceArray.from_host(f) for f in F]
-    motion_prep(d_g, d_q, DeviceArray.from_host(x), *dF, vd, fd, nx, ny, nz)
+    motion_prep(d_pairs, DeviceArray.from_host(x), *dF, vd, fd, nx, ny, nz)
+    pairs = d_pairs.to_host()
     ax, ay, az, bx, by, bz = F
     img0 = np.zeros(shape)
     for fld, w, ind in ((ay, vd, 1), (ax, vd, 2), (az, vd, 3), (by, fd, 1), (bx, fd, 2), (bz, fd, 3)):
         img0 = oracle.add2(img0, fld, x, w, ind)   # src/AUMISART.cu:253-258
-    assert rel_l2(d_g.to_host(), x - img0) <= TOL
-    assert rel_l2(d_q.to_host(), oracle.initial2(x, -vd, -fd)) <= TOL
+    assert rel_l2(pairs[..., 0], x - img0) <= TOL
+    assert rel_l2(pairs[..., 1], oracle.initial2(x, -vd, -fd)) <= TOL
 
 
 def test_motion_update(gpu_lib):
@@ -135,6 +137,6 @@ def test_reduction_counter_reused(gpu_lib, scene):
     for view in (3, 3, 3, 3, 3):
         ang = spec.angles[view]
         meas = DeviceArray.from_host(oracle.projection(other, g, ang).astype(np.float32))
-        sart_residual(d_rp, d_n, d_x, None, meas, *_geo_args(g, ang))
+        sart_residual(d_rp, d_n, d_x, False, meas, *_geo_args(g, ang))
         vals.append(d_n.to_host()[0])
     assert len(set(vals)) == 1 and np.isfinite(vals[0])
