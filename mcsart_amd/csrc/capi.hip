@@ -113,24 +113,15 @@ TimedRegion::TimedRegion(mcsart_ctx *c, int k) : ctx(c), kind(k) {
         }
         return e;
     };
-    if (ctx->capture) {   // events owned by the graph being captured
-        if (hipEventCreate(&a) != hipSuccess) a = nullptr;
-        if (hipEventCreate(&b) != hipSuccess) b = nullptr;
-    } else {
-        a = take();
-        b = take();
-    }
+    a = take();
+    b = take();
     if (a && b) hipEventRecord(a, ctx->stream);
 }
 
 TimedRegion::~TimedRegion() {
     if (!ctx || !ctx->timing || !a || !b) return;
     hipEventRecord(b, ctx->stream);
-    if (ctx->capture) {
-        ctx->capture->push_back({kind, a, b, false});
-        return;
-    }
-    ctx->pending.push_back({kind, a, b, true});
+    ctx->pending.push_back({kind, a, b});
     if (ctx->pending.size() >= 4096) ctx_collect_timing(ctx);
 }
 
@@ -141,10 +132,8 @@ int ctx_collect_timing(mcsart_ctx *ctx) {
         MCSART_HIP_TRY(hipEventElapsedTime(&ms, p.a, p.b));
         ctx->total_ms[p.kind] += ms;
         ctx->launches[p.kind] += 1;
-        if (p.pooled) {
-            ctx->event_pool.push_back(p.a);
-            ctx->event_pool.push_back(p.b);
-        }
+        ctx->event_pool.push_back(p.a);
+        ctx->event_pool.push_back(p.b);
     }
     ctx->pending.clear();
     return MCSART_OK;
@@ -262,7 +251,6 @@ extern "C" int mcsart_ctx_destroy(mcsart_ctx *ctx) {
     hipSetDevice(ctx->device);
     hipStreamSynchronize(ctx->stream);
     for (auto &p : ctx->pending) {
-        if (!p.pooled) continue;   // owned by a job's graph
         hipEventDestroy(p.a);
         hipEventDestroy(p.b);
     }

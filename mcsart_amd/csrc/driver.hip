@@ -39,13 +39,11 @@ struct mcsart_job {
     size_t bytes = 0;
     // One captured graph per sweep of the schedule: the kernels of a sweep and all their
     // arguments are fixed for the life of the job, so a sweep is captured on first use and
-    // replayed afterwards (one launch instead of ~4 per view).
+    // replayed afterwards (one launch instead of ~4 per view).  Sweeps run eagerly while the
+    // context times kernels (HIP events recorded inside a graph carry no timestamps).
     struct SweepGraph {
         hipGraph_t graph = nullptr;
         hipGraphExec_t exec = nullptr;
-        bool timed = false;    // captured with timing events
-        bool queued = false;   // its events are in the context's pending list
-        std::vector<mcsart_ctx::Pending> ev;
     };
     std::vector<SweepGraph> graphs;
     bool use_graphs = true;
@@ -348,10 +346,6 @@ namespace {
 void drop_graph(mcsart_job::SweepGraph &sg) {
     if (sg.exec) hipGraphExecDestroy(sg.exec);
     if (sg.graph) hipGraphDestroy(sg.graph);
-    for (auto &p : sg.ev) {
-        hipEventDestroy(p.a);
-        hipEventDestroy(p.b);
-    }
     sg = mcsart_job::SweepGraph{};
 }
 
@@ -364,9 +358,7 @@ int capture_sweep(mcsart_job *j, int it) {
     auto &sg = j->graphs[it];
     drop_graph(sg);
     MCSART_HIP_TRY(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeRelaxed));
-    ctx->capture = &sg.ev;
     const int rc = run_sweep_eager(j, it);
-    ctx->capture = nullptr;
     hipGraph_t graph = nullptr;
     const hipError_t e = hipStreamEndCapture(ctx->stream, &graph);
     sg.graph = graph;
@@ -378,7 +370,6 @@ int capture_sweep(mcsart_job *j, int it) {
         drop_graph(sg);
         return fail(MCSART_EHIP, "hipGraphInstantiate failed");
     }
-    sg.timed = ctx->timing;
     return MCSART_OK;
 }
 
@@ -395,22 +386,14 @@ extern "C" int mcsart_job_run_sweep(mcsart_job *j, int it) {
     int rc = check_job(j);
     if (rc) return rc;
     if (it < 0 || it >= j->n_iter) return fail(MCSART_EINVAL, "sweep index out of range");
-    if (!j->use_graphs) return run_sweep_eager(j, it);
     mcsart_ctx *ctx = j->ctx;
+    if (!j->use_graphs || ctx->timing) return run_sweep_eager(j, it);
     auto &sg = j->graphs[it];
-    if (!sg.exec || sg.timed != ctx->timing) {
+    if (!sg.exec) {
         rc = capture_sweep(j, it);
         if (rc) return rc;
     }
-    if (ctx->timing && sg.queued) {   // read the previous replay's times before re-recording
-        rc = ctx_collect_timing(ctx);
-        if (rc) return rc;
-    }
     MCSART_HIP_TRY(hipGraphLaunch(sg.exec, ctx->stream));
-    if (ctx->timing) {
-        for (const auto &p : sg.ev) ctx->pending.push_back(p);
-        sg.queued = true;
-    }
     return MCSART_OK;
 }
 
@@ -475,10 +458,6 @@ extern "C" int mcsart_job_destroy(mcsart_job *j) {
     if (!j) return MCSART_OK;
     hipSetDevice(j->ctx->device);
     hipStreamSynchronize(j->ctx->stream);
-    // timing entries of this job's graphs must not outlive their events
-    bool owned = false;
-    for (auto &sg : j->graphs) owned = owned || sg.queued;
-    if (owned) ctx_collect_timing(j->ctx);
     for (auto &sg : j->graphs) drop_graph(sg);
     hipFree(j->block);
     delete j;

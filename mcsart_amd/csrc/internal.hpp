@@ -170,12 +170,9 @@ struct mcsart_ctx {
     struct Pending {
         int kind;
         hipEvent_t a, b;
-        bool pooled;   // false: the events belong to a captured graph (re-recorded per replay)
     };
     std::vector<Pending> pending;
     std::vector<hipEvent_t> event_pool;
-    // while a sweep is being captured into a graph, timed regions record into this list
-    std::vector<Pending> *capture = nullptr;
     double total_ms[mcsart::TK_COUNT] = {0, 0, 0, 0};
     long long launches[mcsart::TK_COUNT] = {0, 0, 0, 0};
     // scratch for the synchronous operator shims

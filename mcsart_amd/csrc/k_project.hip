@@ -22,9 +22,10 @@
 // (g, q) are interleaved as float2, so each gather is one 8-byte load for both projections.
 //
 // SART modes fuse the residual r = p - P(x), the normaliser projection P(1) (image sweep) or the
-// second projection P(q) (motion sweep) and the per-view norms ||p||, ||r|| (per-block FP64
-// partials, reduced in a fixed order by the last workgroup), replacing src/AUMISART.cu:46-52,
-// :194-209 and :246-263.
+// second projection P(q) (motion sweep) and the per-view norms ||p||, ||r|| (per-workgroup FP64
+// partials written with plain stores; the next kernel on the stream sums them in a fixed order),
+// replacing src/AUMISART.cu:46-52, :194-209 and :246-263.  No fence or atomic is used: on a
+// multi-XCD part a device-scope release in every workgroup forces an L2 write-back each time.
 #include "footprint.hpp"
 #include "internal.hpp"
 
@@ -34,9 +35,105 @@ namespace {
 constexpr int RAYS = 64;
 constexpr int MAXSPLIT = 8;
 
-template <typename T>
-__device__ inline T ldg(const T *p, unsigned i) {
-    return p[i];
+// Footprint of one plane over a fixed WT x WZ window of voxels: the overlap weights of
+// src/cu_projection.cu:81-89 (masked to zero outside the volume), the normalisation
+// 1/((t2-t1)(z2-z1)) (:83,:86) and the gather indices (clamped into the volume).
+template <int WT, int WZ>
+struct PlaneW {
+    bool in;
+    float sc;
+    float wt[WT], wz[WZ];
+    unsigned idx[WT][WZ];
+};
+
+template <int WT, int WZ>
+__device__ inline void plane_weights(const Ray &R, int ip, float p0, int nt, int nz, float ntf,
+                                     float nzf, unsigned sp, unsigned st, unsigned sz,
+                                     PlaneW<WT, WZ> &P) {
+    const float p = (float)ip + p0;
+    const float ta = fmaf(R.rA, p, R.KA), tb = fmaf(R.rB, p, R.KB);
+    const float za = fmaf(R.rzA, p, R.KzA), zb = fmaf(R.rzB, p, R.KzB);
+    const float t1 = fminf(ta, tb), t2 = fmaxf(ta, tb);
+    const float z1 = fminf(za, zb), z2 = fmaxf(za, zb);
+    P.in = (t2 > 0.f) & (t1 < ntf) & (z2 > 0.f) & (z1 < nzf);
+    P.sc = P.in ? __builtin_amdgcn_rcpf((t2 - t1) * (z2 - z1)) : 0.f;
+    const float ft = fminf(fmaxf(floorf(t1), -4.f), ntf);
+    const float fz = fminf(fmaxf(floorf(z1), -4.f), nzf);
+    const int T0 = (int)ft, Z0 = (int)fz;
+    unsigned it[WT], iz[WZ];
+#pragma unroll
+    for (int k = 0; k < WT; ++k) {
+        // overlap of voxel T0+k with [t1, t2]: min(i+1, t2) - max(i, t1), >= 0
+        const float lo = (k == 0) ? t1 : ft + (float)k;
+        const float w = fmaxf(fminf(ft + (float)(k + 1), t2) - lo, 0.f);
+        const int i = T0 + k;
+        P.wt[k] = (i >= 0 && i < nt) ? w : 0.f;
+        it[k] = (unsigned)clampi(i, 0, nt - 1) * st;
+    }
+#pragma unroll
+    for (int k = 0; k < WZ; ++k) {
+        const float lo = (k == 0) ? z1 : fz + (float)k;
+        const float w = fmaxf(fminf(fz + (float)(k + 1), z2) - lo, 0.f);
+        const int i = Z0 + k;
+        P.wz[k] = (i >= 0 && i < nz) ? w : 0.f;
+        iz[k] = (unsigned)clampi(i, 0, nz - 1) * sz;
+    }
+    const unsigned pb = (unsigned)ip * sp;
+#pragma unroll
+    for (int k = 0; k < WT; ++k)
+#pragma unroll
+        for (int j = 0; j < WZ; ++j) P.idx[k][j] = pb + it[k] + iz[j];
+}
+
+template <typename T, int WT, int WZ>
+__device__ inline void gather(const T *__restrict__ v, const PlaneW<WT, WZ> &P, T (&out)[WT][WZ]) {
+#pragma unroll
+    for (int k = 0; k < WT; ++k)
+#pragma unroll
+        for (int j = 0; j < WZ; ++j) out[k][j] = v[P.idx[k][j]];
+}
+
+// select, not multiply: a plane outside the volume adds exactly nothing even when the clamped
+// gathers hit non-finite values
+template <bool ONES, int WT, int WZ>
+__device__ inline void accumulate1(const PlaneW<WT, WZ> &P, const float (&val)[WT][WZ],
+                                   float &acc0, float &acc1) {
+    float s0 = 0.f;
+#pragma unroll
+    for (int k = 0; k < WT; ++k) {
+        float r0 = 0.f;
+#pragma unroll
+        for (int j = 0; j < WZ; ++j) r0 = fmaf(P.wz[j], val[k][j], r0);
+        s0 = fmaf(P.wt[k], r0, s0);
+    }
+    acc0 += P.in ? P.sc * s0 : 0.f;
+    if (ONES) {   // P(1): the same weights on a volume of ones
+        float swt = 0.f, swz = 0.f;
+#pragma unroll
+        for (int k = 0; k < WT; ++k) swt += P.wt[k];
+#pragma unroll
+        for (int k = 0; k < WZ; ++k) swz += P.wz[k];
+        acc1 = fmaf(P.sc, swt * swz, acc1);
+    }
+}
+
+template <int WT, int WZ>
+__device__ inline void accumulate2(const PlaneW<WT, WZ> &P, const float2 (&val)[WT][WZ],
+                                   float &acc0, float &acc1) {
+    float s0 = 0.f, s1 = 0.f;
+#pragma unroll
+    for (int k = 0; k < WT; ++k) {
+        float r0 = 0.f, r1 = 0.f;
+#pragma unroll
+        for (int j = 0; j < WZ; ++j) {
+            r0 = fmaf(P.wz[j], val[k][j].x, r0);
+            r1 = fmaf(P.wz[j], val[k][j].y, r1);
+        }
+        s0 = fmaf(P.wt[k], r0, s0);
+        s1 = fmaf(P.wt[k], r1, s1);
+    }
+    acc0 += P.in ? P.sc * s0 : 0.f;
+    acc1 += P.in ? P.sc * s1 : 0.f;
 }
 
 template <int MODE, int WT, int WZ>
@@ -70,77 +167,38 @@ __global__ void __launch_bounds__(RAYS *MAXSPLIT) k_project(GeomArgs g, double c
         const int chunk = (np + nsplit - 1) / nsplit;
         const int ip_lo = ty * chunk;
         const int ip_hi = min(np, ip_lo + chunk);
-#pragma unroll 2
-        for (int ip = ip_lo; ip < ip_hi; ++ip) {
-            const float p = (float)ip + p0;
-            const float ta = fmaf(R.rA, p, R.KA), tb = fmaf(R.rB, p, R.KB);
-            const float za = fmaf(R.rzA, p, R.KzA), zb = fmaf(R.rzB, p, R.KzB);
-            const float t1 = fminf(ta, tb), t2 = fmaxf(ta, tb);
-            const float z1 = fminf(za, zb), z2 = fmaxf(za, zb);
-            const bool in = (t2 > 0.f) & (t1 < ntf) & (z2 > 0.f) & (z1 < nzf);
-            // normalisation 1/((t2-t1)(z2-z1)) of src/cu_projection.cu:83,86
-            const float sc = in ? __builtin_amdgcn_rcpf((t2 - t1) * (z2 - z1)) : 0.f;
-            const float ft = fminf(fmaxf(floorf(t1), -4.f), ntf);
-            const float fz = fminf(fmaxf(floorf(z1), -4.f), nzf);
-            const int T0 = (int)ft, Z0 = (int)fz;
-            float wt[WT], wz[WZ];
-            unsigned it[WT], iz[WZ];
-#pragma unroll
-            for (int k = 0; k < WT; ++k) {
-                // overlap of voxel T0+k with [t1, t2]: min(i+1, t2) - max(i, t1), >= 0
-                const float lo = (k == 0) ? t1 : ft + (float)k;
-                float w = fmaxf(fminf(ft + (float)(k + 1), t2) - lo, 0.f);
-                const int i = T0 + k;
-                w = (i >= 0 && i < nt) ? w : 0.f;
-                wt[k] = w;
-                it[k] = (unsigned)clampi(i, 0, nt - 1) * st;
-            }
-#pragma unroll
-            for (int k = 0; k < WZ; ++k) {
-                const float lo = (k == 0) ? z1 : fz + (float)k;
-                float w = fmaxf(fminf(fz + (float)(k + 1), z2) - lo, 0.f);
-                const int i = Z0 + k;
-                w = (i >= 0 && i < nz) ? w : 0.f;
-                wz[k] = w;
-                iz[k] = (unsigned)clampi(i, 0, nz - 1) * sz;
-            }
-            const unsigned pb = (unsigned)ip * sp;
+        // Two planes per iteration: both footprints first, then all their gathers (into distinct
+        // registers, so they are in flight together), then the weighted sums.
+        int ip = ip_lo;
+        for (; ip + 1 < ip_hi; ip += 2) {
+            PlaneW<WT, WZ> A, B;
+            plane_weights(R, ip, p0, nt, nz, ntf, nzf, sp, st, sz, A);
+            plane_weights(R, ip + 1, p0, nt, nz, ntf, nzf, sp, st, sz, B);
             if (MODE == PROJ_MOTION) {
-                float s0 = 0.f, s1 = 0.f;
-#pragma unroll
-                for (int k = 0; k < WT; ++k) {
-                    float r0 = 0.f, r1 = 0.f;
-#pragma unroll
-                    for (int j = 0; j < WZ; ++j) {
-                        const float2 q = ldg(v2, pb + it[k] + iz[j]);
-                        r0 = fmaf(wz[j], q.x, r0);
-                        r1 = fmaf(wz[j], q.y, r1);
-                    }
-                    s0 = fmaf(wt[k], r0, s0);
-                    s1 = fmaf(wt[k], r1, s1);
-                }
-                // select, not multiply: a plane outside the volume adds exactly nothing even
-                // when the clamped gathers hit non-finite values
-                acc0 += in ? sc * s0 : 0.f;
-                acc1 += in ? sc * s1 : 0.f;
+                float2 va[WT][WZ], vb[WT][WZ];
+                gather(v2, A, va);
+                gather(v2, B, vb);
+                accumulate2(A, va, acc0, acc1);
+                accumulate2(B, vb, acc0, acc1);
             } else {
-                float s0 = 0.f;
-#pragma unroll
-                for (int k = 0; k < WT; ++k) {
-                    float r0 = 0.f;
-#pragma unroll
-                    for (int j = 0; j < WZ; ++j) r0 = fmaf(wz[j], ldg(v, pb + it[k] + iz[j]), r0);
-                    s0 = fmaf(wt[k], r0, s0);
-                }
-                acc0 += in ? sc * s0 : 0.f;
-                if (MODE == PROJ_IMAGE) {   // P(1): the same weights on a volume of ones
-                    float swt = 0.f, swz = 0.f;
-#pragma unroll
-                    for (int k = 0; k < WT; ++k) swt += wt[k];
-#pragma unroll
-                    for (int k = 0; k < WZ; ++k) swz += wz[k];
-                    acc1 = fmaf(sc, swt * swz, acc1);
-                }
+                float va[WT][WZ], vb[WT][WZ];
+                gather(v, A, va);
+                gather(v, B, vb);
+                accumulate1<MODE == PROJ_IMAGE>(A, va, acc0, acc1);
+                accumulate1<MODE == PROJ_IMAGE>(B, vb, acc0, acc1);
+            }
+        }
+        if (ip < ip_hi) {
+            PlaneW<WT, WZ> A;
+            plane_weights(R, ip, p0, nt, nz, ntf, nzf, sp, st, sz, A);
+            if (MODE == PROJ_MOTION) {
+                float2 va[WT][WZ];
+                gather(v2, A, va);
+                accumulate2(A, va, acc0, acc1);
+            } else {
+                float va[WT][WZ];
+                gather(v, A, va);
+                accumulate1<MODE == PROJ_IMAGE>(A, va, acc0, acc1);
             }
         }
     }

@@ -1,0 +1,85 @@
+"""Run a bounded piece of a McSART reconstruction for profilers (rocprofv3 kernel traces and
+hardware-counter passes).
+
+    python -m mcsart_amd.profile_sweep --workload mcsart128 --sweeps 1,0 --copy-mb 64
+
+Builds the named workload's scan (synthetic inputs, measured projections made on the GPU), then
+runs the given sweeps (1 = image sweep, 0 = motion sweep) kernel by kernel (no graphs, so every
+kernel is its own dispatch), and finally one device-to-device copy of --copy-mb MiB: the
+profiler's FETCH_SIZE / WRITE_SIZE for that copy calibrate the counters against a known byte
+count.
+"""
+import argparse
+import sys
+import time
+
+import numpy as np
+
+from . import configs
+from .configs import make_scan_inputs
+from .driver import Context, Job
+from .ops import DeviceArray, host_deform, host_projection
+from ._lib import check, load
+
+WORKLOADS = {"mcsart128": configs.c2_mcsart128, "clinical256": configs.c3_clinical,
+             "static64": configs.c1_static64}
+
+
+def measured_projections(spec, data):
+    g = spec.geometry
+    d_ph = DeviceArray.from_host(data.phantom)
+    d_w = DeviceArray.empty(g.vol_shape)
+    d_f = [DeviceArray.from_host(f) for f in data.fields] if data.fields else None
+    d_p = DeviceArray.empty(g.det_shape)
+    out = np.empty((spec.n_views,) + g.det_shape, np.float32)
+    for i, a in enumerate(spec.angles):
+        src = d_ph
+        if d_f is not None:
+            host_deform(d_w, d_ph, g.nx, g.ny, g.nz, float(data.alpha[i]), float(data.beta[i]), *d_f)
+            src = d_w
+        host_projection(d_p, src, float(a), g.SO, g.SD, g.da, g.na, g.ai, g.db, g.nb, g.bi,
+                        g.nx, g.ny, g.nz)
+        out[i] = d_p.to_host()
+    return out
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--workload", default="mcsart128", choices=sorted(WORKLOADS))
+    ap.add_argument("--sweeps", default="1,0")
+    ap.add_argument("--views", type=int, default=0, help="use only the first V views (0: all)")
+    ap.add_argument("--copy-mb", type=int, default=64)
+    args = ap.parse_args(argv)
+    spec = WORKLOADS[args.workload]()
+    if args.views:
+        spec = configs.scaled(spec, n_views=args.views)
+    sweeps = [int(s) for s in args.sweeps.split(",") if s != ""]
+    g = spec.geometry
+    data = make_scan_inputs(spec)
+    proj = measured_projections(spec, data)
+    ctx = Context(0)
+    job = Job(ctx, g, spec.angles, data.alpha, data.beta, sweeps, lam=spec.lam,
+              motion=spec.motion, nonneg=spec.nonneg)
+    job.set_graphs(False)
+    job.set_projections(proj)
+    job.set_image(None)
+    job.set_fields(data.fields)
+    t0 = time.perf_counter()
+    job.run()
+    ctx.synchronize()
+    print(f"[profile_sweep] {spec.name}: sweeps {sweeps} over {spec.n_views} views in "
+          f"{(time.perf_counter() - t0) * 1e3:.1f} ms", flush=True)
+    if args.copy_mb:
+        n = args.copy_mb << 20
+        a, b = DeviceArray.empty((n // 4,)), DeviceArray.empty((n // 4,))
+        check(load().mcsart_memset(a.ptr, 0, n), "mcsart_memset")
+        check(load().mcsart_memcpy_dtod(b.ptr, a.ptr, n), "mcsart_memcpy_dtod")
+        check(load().mcsart_device_synchronize(), "sync")
+        print(f"[profile_sweep] calibration copy: {n} bytes", flush=True)
+    job.close()
+    ctx.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())
