@@ -234,6 +234,20 @@ int mcsart_job_get_fields(mcsart_job *job, float *const h_fields[6]);
 int mcsart_job_device_image(mcsart_job *job, float **d_img);
 int mcsart_job_destroy(mcsart_job *job);
 
+/* Launch shapes of the two footprint kernels (the reference has fixed 16x16 blocks,
+ * src/cu_projection.cu:140, src/cu_backprojection.cu:186).  Zero fields mean automatic. */
+typedef struct {
+    int proj_split;   /* projector workgroup slices along each ray: 0, 1, 2, 4 or 8 */
+    int back_zslab;   /* backprojector voxels per lane along z: 0 or 1..64 */
+    int back_stage;   /* 1: backprojector workgroups stage their detector rectangle in LDS */
+} mcsart_tuning;
+int mcsart_job_set_tuning(mcsart_job *job, const mcsart_tuning *t);
+int mcsart_job_get_tuning(mcsart_job *job, mcsart_tuning *t);
+/* Time the candidate launch shapes on a few of the job's views (projector in residual mode on the
+ * current image, backprojector in ratio mode into scratch; the job's image, fields and norms are
+ * untouched), keep the fastest and report it in *chosen (may be NULL).  Synchronous. */
+int mcsart_job_autotune(mcsart_job *job, mcsart_tuning *chosen);
+
 /* Per-kernel timing of the last run(s) on this context, from HIP events recorded on the stream
  * the kernels run on.  kind: 0 projector, 1 backprojector, 2 warp, 3 motion elementwise. */
 int mcsart_ctx_enable_timing(mcsart_ctx *ctx, int enable);

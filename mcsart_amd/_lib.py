@@ -64,6 +64,9 @@ SIGNATURES = {
     "mcsart_job_get_fields": ([_P, _P], _I),
     "mcsart_job_device_image": ([_P, ctypes.POINTER(_P)], _I),
     "mcsart_job_destroy": ([_P], _I),
+    "mcsart_job_set_tuning": ([_P, _P], _I),
+    "mcsart_job_get_tuning": ([_P, _P], _I),
+    "mcsart_job_autotune": ([_P, _P], _I),
     "mcsart_ctx_enable_timing": ([_P, _I], _I),
     "mcsart_ctx_kernel_time": ([_P, _I, ctypes.POINTER(ctypes.c_double),
                                 ctypes.POINTER(ctypes.c_longlong)], _I),

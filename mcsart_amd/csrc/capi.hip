@@ -345,9 +345,11 @@ extern "C" int mcsart_backprojection(float *d_img, const float *d_proj, float an
     if (rc) return rc;
     if (!d_proj || !d_img) return fail(MCSART_EINVAL, "null device pointer");
     GET_CTX();
+    const ViewTrig t = view_trig(angle);
+    const ViewInfo vi = view_info_of(g, t);
     TimedRegion tr(ctx, TK_BACK);
-    return finish(ctx, launch_backprojector(g, view_trig(angle), BACK_PLAIN, d_proj, nullptr,
-                                            d_img, ctx->stream));
+    return finish(ctx, launch_backprojector(g, t, vi, BACK_PLAIN, d_proj, nullptr, d_img,
+                                            ctx->stream));
 }
 
 extern "C" int mcsart_deform(float *d_img1, const float *d_img, int nx, int ny, int nz,
@@ -528,9 +530,11 @@ extern "C" int mcsart_sart_backprojection(float *d_u, const float *d_rp, float a
     if (rc) return rc;
     if (!d_u || !d_rp) return fail(MCSART_EINVAL, "null device pointer");
     GET_CTX();
+    const ViewTrig t = view_trig(angle);
+    const ViewInfo vi = view_info_of(g, t);
     TimedRegion tr(ctx, TK_BACK);
-    return finish(ctx, launch_backprojector(g, view_trig(angle), BACK_RATIO, nullptr,
-                                            (const float2 *)d_rp, d_u, ctx->stream));
+    return finish(ctx, launch_backprojector(g, t, vi, BACK_RATIO, nullptr, (const float2 *)d_rp,
+                                            d_u, ctx->stream));
 }
 
 extern "C" int mcsart_motion_prep(float *d_pairs, const float *d_img, const float *alpha_x,

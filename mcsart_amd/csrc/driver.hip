@@ -47,6 +47,8 @@ struct mcsart_job {
     };
     std::vector<SweepGraph> graphs;
     bool use_graphs = true;
+    int proj_split = 0;         // projector launch shape (0 = automatic)
+    BackTune back_tune{0, 0};   // backprojector launch shape (0 = automatic)
 };
 
 namespace {
@@ -127,14 +129,14 @@ int image_sweep(mcsart_job *j, int it) {
             a.resid_copy = j->d_resid ? j->d_resid + D * iv : nullptr;
             a.partials = j->d_partials;
             TimedRegion tr(j->ctx, TK_PROJ);
-            rc = launch_projector(g, j->trig[iv], vi, PROJ_IMAGE, a, st, &nparts);
+            rc = launch_projector(g, j->trig[iv], vi, PROJ_IMAGE, a, st, &nparts, j->proj_split);
             if (rc) return rc;
         }
         {   // u = B(r) / B(P1)  (:212-218); also rho_i from the projector's partials
             const NormFinish nf{j->d_partials, nparts, j->d_norm + (long long)it * j->V + iv};
             TimedRegion tr(j->ctx, TK_BACK);
-            rc = launch_backprojector(g, j->trig[iv], BACK_RATIO, nullptr, j->d_rp, j->d_w2, st,
-                                      &nf);
+            rc = launch_backprojector(g, j->trig[iv], vi, BACK_RATIO, nullptr, j->d_rp, j->d_w2,
+                                      st, &nf, &j->back_tune);
             if (rc) return rc;
         }
         {   // x += lambda * W(u; v_0 - v_i, f_0 - f_i)  (:222-223)
@@ -182,14 +184,14 @@ int motion_sweep(mcsart_job *j, int it) {
             a.rp = j->d_rp;
             a.partials = j->d_partials;
             TimedRegion tr(j->ctx, TK_PROJ);
-            rc = launch_projector(g, j->trig[iv], vi, PROJ_MOTION, a, st, &nparts);
+            rc = launch_projector(g, j->trig[iv], vi, PROJ_MOTION, a, st, &nparts, j->proj_split);
             if (rc) return rc;
         }
         {   // u = B(r) / B(P q)  (:265-270); also rho_i
             const NormFinish nf{j->d_partials, nparts, j->d_norm + (long long)it * j->V + iv};
             TimedRegion tr(j->ctx, TK_BACK);
-            rc = launch_backprojector(g, j->trig[iv], BACK_RATIO, nullptr, j->d_rp, j->d_w2, st,
-                                      &nf);
+            rc = launch_backprojector(g, j->trig[iv], vi, BACK_RATIO, nullptr, j->d_rp, j->d_w2,
+                                      st, &nf, &j->back_tune);
             if (rc) return rc;
         }
         {   // fields += u^T * (-vd | -fd) * D_c x  (:272-277)
@@ -451,6 +453,143 @@ extern "C" int mcsart_job_device_image(mcsart_job *j, float **d_img) {
     if (rc) return rc;
     if (!d_img) return fail(MCSART_EINVAL, "null pointer");
     *d_img = j->d_img;
+    return MCSART_OK;
+}
+
+namespace {
+
+int check_tuning(const mcsart_tuning &t) {
+    if (t.proj_split < 0 || t.proj_split > 8 || (t.proj_split & (t.proj_split - 1)))
+        return fail(MCSART_EINVAL, "proj_split must be 0, 1, 2, 4 or 8");
+    if (t.back_zslab < 0 || t.back_zslab > 64)
+        return fail(MCSART_EINVAL, "back_zslab must be in 0..64");
+    if (t.back_stage != 0 && t.back_stage != 1)
+        return fail(MCSART_EINVAL, "back_stage must be 0 or 1");
+    return MCSART_OK;
+}
+
+void apply_tuning(mcsart_job *j, const mcsart_tuning &t) {
+    if (j->proj_split != t.proj_split || j->back_tune.zslab != t.back_zslab ||
+        j->back_tune.stage != t.back_stage)
+        for (auto &sg : j->graphs) drop_graph(sg);   // captured launches carry the old shapes
+    j->proj_split = t.proj_split;
+    j->back_tune = BackTune{t.back_zslab, t.back_stage};
+}
+
+// Elapsed milliseconds of `fn` enqueued `reps` times on the job's stream.
+template <class F>
+int time_on_stream(mcsart_job *j, int reps, F &&fn, float *ms) {
+    hipStream_t st = j->ctx->stream;
+    hipEvent_t a = nullptr, b = nullptr;
+    MCSART_HIP_TRY(hipEventCreate(&a));
+    if (hipEventCreate(&b) != hipSuccess) {
+        hipEventDestroy(a);
+        return fail(MCSART_EHIP, "hipEventCreate failed");
+    }
+    int rc = MCSART_OK;
+    if (hipEventRecord(a, st) != hipSuccess) rc = fail(MCSART_EHIP, "hipEventRecord failed");
+    for (int r = 0; r < reps && !rc; ++r) rc = fn();
+    if (!rc && (hipEventRecord(b, st) != hipSuccess || hipEventSynchronize(b) != hipSuccess ||
+                hipEventElapsedTime(ms, a, b) != hipSuccess))
+        rc = fail(MCSART_EHIP, "event timing failed");
+    hipEventDestroy(a);
+    hipEventDestroy(b);
+    return rc;
+}
+
+}  // namespace
+
+extern "C" int mcsart_job_set_tuning(mcsart_job *j, const mcsart_tuning *t) {
+    int rc = check_job(j);
+    if (rc) return rc;
+    if (!t) return fail(MCSART_EINVAL, "null tuning");
+    rc = check_tuning(*t);
+    if (rc) return rc;
+    apply_tuning(j, *t);
+    return MCSART_OK;
+}
+
+extern "C" int mcsart_job_get_tuning(mcsart_job *j, mcsart_tuning *t) {
+    int rc = check_job(j);
+    if (rc) return rc;
+    if (!t) return fail(MCSART_EINVAL, "null tuning");
+    *t = mcsart_tuning{j->proj_split, j->back_tune.zslab, j->back_tune.stage};
+    return MCSART_OK;
+}
+
+extern "C" int mcsart_job_autotune(mcsart_job *j, mcsart_tuning *chosen) {
+    int rc = check_job(j);
+    if (rc) return rc;
+    mcsart_ctx *ctx = j->ctx;
+    if (ctx->timing) return fail(MCSART_EINVAL, "autotune while kernel timing is enabled");
+    MCSART_HIP_TRY(hipSetDevice(ctx->device));
+    const Geom &g = j->g;
+    const long long D = g.ndet();
+    hipStream_t st = ctx->stream;
+    // up to four views spread over the scan: footprint shapes change with the angle
+    std::vector<int> views;
+    const int nv = std::min(j->V, 4);
+    for (int k = 0; k < nv; ++k) views.push_back((int)((long long)k * j->V / nv));
+    // transposed image for x-branch rays, into the job's scratch
+    MCSART_HIP_TRY(hipStreamSynchronize(st));
+    rc = launch_transpose(j->d_w1T, j->d_img, g.nx, g.ny, g.nz, st);
+    if (rc) return rc;
+    auto project = [&](int split) {
+        for (int iv : views) {
+            ProjArgs a{};
+            a.vol0 = j->d_img;
+            a.vol0T = j->vinfo[iv].has_x ? j->d_w1T : nullptr;
+            a.meas = j->d_proj + D * iv;
+            a.rp = j->d_rp;
+            a.partials = j->d_partials;
+            int np = 0;
+            const int r = launch_projector(g, j->trig[iv], j->vinfo[iv], PROJ_IMAGE, a, st, &np,
+                                           split);
+            if (r) return r;
+        }
+        return (int)MCSART_OK;
+    };
+    auto backproject = [&](const BackTune &bt) {
+        for (int iv : views) {
+            const int r = launch_backprojector(g, j->trig[iv], j->vinfo[iv], BACK_RATIO, nullptr,
+                                               j->d_rp, j->d_w2, st, nullptr, &bt);
+            if (r) return r;
+        }
+        return (int)MCSART_OK;
+    };
+    const int reps = 3;
+    mcsart_tuning best{0, 0, 0};
+    float best_ms = 3.4e38f;
+    const int np = std::max(g.nx, g.ny);
+    for (int split = 1; split <= 8; split *= 2) {
+        if (split > 1 && np / split < 8) break;
+        rc = project(split);   // warm-up (code object load, caches)
+        if (rc) return rc;
+        float ms = 0.f;
+        rc = time_on_stream(j, reps, [&] { return project(split); }, &ms);
+        if (rc) return rc;
+        if (ms < best_ms) {
+            best_ms = ms;
+            best.proj_split = split;
+        }
+    }
+    best_ms = 3.4e38f;
+    for (int stage = 0; stage <= 1; ++stage)
+        for (int zs = 1; zs <= 16 && zs <= g.nz; zs *= 2) {
+            const BackTune bt{zs, stage};
+            rc = backproject(bt);
+            if (rc) return rc;
+            float ms = 0.f;
+            rc = time_on_stream(j, reps, [&] { return backproject(bt); }, &ms);
+            if (rc) return rc;
+            if (ms < best_ms) {
+                best_ms = ms;
+                best.back_zslab = zs;
+                best.back_stage = stage;
+            }
+        }
+    apply_tuning(j, best);
+    if (chosen) *chosen = best;
     return MCSART_OK;
 }
 

@@ -187,11 +187,12 @@ __host__ __device__ inline int clampi(int v, int lo, int hi) { return v < lo ? l
 // ------------------------------------------------------------------------------------------
 struct ViewInfo {
     int has_x, has_y;
-    int wt, wz;
+    int wt, wz;   // projector windows
+    int wa;       // backprojector transverse window (cells)
 };
 
 inline ViewInfo view_info(const GeomArgs &g, double c, double s) {
-    ViewInfo v{0, 0, 0, 0};
+    ViewInfo v{0, 0, 0, 0, 0};
     double wt_max = 0.0, wz_max = 0.0;
     const double SO = g.SO, SD = g.SD, da = g.da, db = g.db;
     for (int ia = 0; ia < g.na; ++ia) {
@@ -225,6 +226,25 @@ inline ViewInfo view_info(const GeomArgs &g, double c, double s) {
     // a small margin absorbs the FP32 rounding of the per-plane bounds
     v.wt = (int)floor(wt_max * (1.0 + 1e-5) + 1e-5) + 2;
     v.wz = (int)floor(wz_max * (1.0 + 1e-5) + 1e-5) + 2;
+    // backprojector: the widest transverse footprint of a voxel column, in cells.  The width is
+    // largest where magnification and |y'| are largest, i.e. on the boundary of the volume's
+    // footprint in the plane, so the boundary columns are scanned (plus a one-cell margin); a
+    // column that still exceeds it is handled by a generic loop in the kernel.
+    double wa_max = 0.0;
+    auto scan = [&](int ix, int iy) {
+        Column C;
+        column_setup(g, c, s, ix, iy, C);
+        wa_max = fmax(wa_max, C.Amax - C.Amin);
+    };
+    for (int ix = 0; ix < g.nx; ++ix) {
+        scan(ix, 0);
+        scan(ix, g.ny - 1);
+    }
+    for (int iy = 0; iy < g.ny; ++iy) {
+        scan(0, iy);
+        scan(g.nx - 1, iy);
+    }
+    v.wa = (int)floor(wa_max + 1.0) + 2;
     return v;
 }
 

@@ -77,8 +77,9 @@ struct ProjArgs {
 // (sum p^2, sum r^2) pair per workgroup in a.partials and report their count in *nparts; a later
 // kernel on the same stream turns them into ||r||/||p|| (NormFinish).  No fence or atomic is
 // needed: the kernel boundary orders the stores.
+// split: workgroup slices along the ray (power of two up to 8; 0 = automatic).
 int launch_projector(const Geom &g, const ViewTrig &t, const ViewInfo &vi, ProjMode mode,
-                     const ProjArgs &a, hipStream_t st, int *nparts = nullptr);
+                     const ProjArgs &a, hipStream_t st, int *nparts = nullptr, int split = 0);
 ViewInfo view_info_of(const Geom &g, const ViewTrig &t);
 int projector_partials(const Geom &g);   // number of per-block partial slots needed
 
@@ -89,8 +90,8 @@ struct NormFinish {
     int nparts;
     float *out;
 };
-__device__ inline void finish_norm_wave(const NormFinish &nf) {
-    const int lane = threadIdx.x & 63;
+// Must be called by all 64 lanes of one wavefront; `lane` is the caller's lane index.
+__device__ inline void finish_norm_wave(const NormFinish &nf, int lane) {
     double sp = 0.0, sr = 0.0;
     for (int k = lane; k < nf.nparts; k += 64) {
         sp += nf.partials[2 * k];
@@ -109,11 +110,17 @@ enum BackMode {
     BACK_PLAIN = 0,  // out = B(proj)
     BACK_RATIO = 1,  // out = B(rp.x) / B(rp.y), NaN/Inf -> 0
 };
-// nf (optional): the first wavefront of the first workgroup also finishes the preceding
-// projector's norm reduction.
-int launch_backprojector(const Geom &g, const ViewTrig &t, BackMode mode, const float *proj,
-                         const float2 *rp, float *out, hipStream_t st,
-                         const NormFinish *nf = nullptr);
+// Launch shape of the backprojector: voxels per workgroup along z (0 = automatic) and whether
+// the workgroup stages its detector rectangle in LDS.
+struct BackTune {
+    int zslab;
+    int stage;
+};
+// vi: the view's transverse window (view_info).  nf (optional): the first wavefront of the first
+// workgroup also finishes the preceding projector's norm reduction.
+int launch_backprojector(const Geom &g, const ViewTrig &t, const ViewInfo &vi, BackMode mode,
+                         const float *proj, const float2 *rp, float *out, hipStream_t st,
+                         const NormFinish *nf = nullptr, const BackTune *tune = nullptr);
 
 // ---- warps and elementwise ------------------------------------------------------------------
 struct Fields {

@@ -3,68 +3,212 @@
 // Reference: kernel_backprojection, src/cu_backprojection.cu:49-170 (one thread per voxel, every
 // call allocates a CUDA array, copies the projection into it and builds a texture, :10-45).
 //
-// Here one lane owns an (ix, iy) column and walks a z-slab of voxels: the transverse footprint
-// (a bounds, weights) of a column does not depend on z, so it is formed once per column in FP64
-// (footprint.hpp) and kept in registers; each voxel only forms its axial bounds.  A wavefront is
-// 64 neighbouring ix of one iy, so its detector reads fall on neighbouring cells of the same
-// detector rows, and its volume writes are contiguous.  The projection is read directly from
-// global memory (it is at most a few MB and stays in L2).  The SART mode reads (r, P1) interleaved
-// as float2 and writes B(r)/B(P1) with NaN/Inf -> 0, fusing src/AUMISART.cu:212-218 (two
-// backprojections and a division) into one pass.
+// Work layout: a workgroup owns 32 x 8 (ix, iy) columns and a slab of ZS voxels along z; one lane
+// owns a column.  The transverse footprint (a bounds, weights) of a column does not depend on z,
+// so it is formed once per column in FP64 (footprint.hpp); each voxel only forms its axial bounds.
+//
+// LDS staging of the detector: the workgroup first reduces, in LDS, the rectangle of detector
+// cells its voxels touch (per column the a range is exact; along z the b range is monotone, so
+// the slab's first and last voxels bound it), then copies that rectangle of the projection (or of
+// the interleaved (r, n) pairs) into LDS with coalesced row reads, and every voxel gathers its
+// footprint cells from LDS instead of issuing scattered global loads.  A cell outside the staged
+// rectangle (or a rectangle larger than the LDS budget) is read from global memory, so the
+// staging changes only speed, never results.
+//
+// The SART mode reads (r, P1) interleaved as float2 and writes B(r)/B(P1) with NaN/Inf -> 0,
+// fusing src/AUMISART.cu:212-218 (two backprojections and a division) into one pass.
 #include "footprint.hpp"
 #include "internal.hpp"
 
 namespace mcsart {
 namespace {
 
-constexpr int BX = 64, BY = 4;
+constexpr int BX = 32, BY = 8;
+constexpr int LDS_CELLS = 4096;   // staged cells per workgroup (32 KB of float2)
 
-template <int MODE>
+// Axial bounds of a column's voxels relative to an integer anchor: the four lines
+// q*zc + c of src/cu_backprojection.cu:107-132 (slopes Rm/db, Rp/db, Rc/db twice) evaluated in
+// FP64 at the slab's first voxel, then stepped in FP32 with values of a few cells at most.
+struct BLines {
+    int anchor;
+    float l0, l1, l2, l3;   // line values at the slab's first voxel, minus anchor
+    float q0, q1, q2;       // slopes per voxel (l2 and l3 share q2)
+};
+
+__device__ inline BLines blines_at(const Column &C, int nz, int iz0) {
+    const double zc = (double)iz0 - (double)(nz / 2) + 0.5;   // :68
+    const double L0 = C.qm * zc + C.C0, L1 = C.qp * zc + C.C0;
+    const double L2 = C.qc * (zc + 0.5) + C.C0, L3 = C.qc * (zc - 0.5) + C.C0;
+    const double lo = fmin(fmin(L0, L1), L3);
+    BLines b;
+    b.anchor = (int)fmin(fmax(floor(lo), -1048576.0), 1048576.0);
+    const double a = (double)b.anchor;
+    b.l0 = (float)(L0 - a);
+    b.l1 = (float)(L1 - a);
+    b.l2 = (float)(L2 - a);
+    b.l3 = (float)(L3 - a);
+    b.q0 = (float)C.qm;
+    b.q1 = (float)C.qp;
+    b.q2 = (float)C.qc;
+    return b;
+}
+
+template <int MODE, int WA, bool STAGE>
 __global__ void __launch_bounds__(BX *BY) k_backproject(GeomArgs g, double c, double s,
                                                        const float *__restrict__ proj,
                                                        const float2 *__restrict__ rp,
                                                        float *__restrict__ out, int zslab,
                                                        NormFinish nf) {
-    // the preceding projector's per-view norm (its partials are complete: kernel boundary)
-    if (nf.out && blockIdx.x == 0 && blockIdx.y == 0 && blockIdx.z == 0 && threadIdx.y == 0)
-        finish_norm_wave(nf);
+    const int tid = threadIdx.y * BX + threadIdx.x;
+    // the preceding projector's per-view norm (its partials are complete: kernel boundary); the
+    // whole first wavefront (threads 0..63) of the first workgroup does it
+    if (nf.out && blockIdx.x == 0 && blockIdx.y == 0 && blockIdx.z == 0 && tid < 64)
+        finish_norm_wave(nf, tid);
+
     const int ix = blockIdx.x * BX + threadIdx.x;
     const int iy = blockIdx.y * BY + threadIdx.y;
-    if (ix >= g.nx || iy >= g.ny) return;
+    const bool live = ix < g.nx && iy < g.ny;
     const int iz0 = blockIdx.z * zslab;
     const int iz1 = min(g.nz, iz0 + zslab);
 
+    // ---- column: transverse footprint (a) once, in FP64, clamped so conversions stay in range
     Column C;
-    column_setup(g, c, s, ix, iy, C);
-    // a range (src/cu_backprojection.cu:140,150): ia from max(0, floor(Amin)) while
-    // ia < MIN(na, Amax); weights in coordinates local to the first cell.
-    // clamped in FP64 before the conversion so that it is always in range (NaN -> 0)
-    const int ia_lo = (int)fmin(fmax(floor(C.Amin), 0.0), (double)g.na);
-    const float aminL = (float)(C.Amin - ia_lo);
-    const float amaxL = (float)(C.Amax - ia_lo);
-    const float alim = ((double)g.na < C.Amax) ? (float)(g.na - ia_lo) : amaxL;
-    const long long plane = (long long)g.nx * g.ny;
+    int ia_lo = 0, acount = 0;
+    float aminL = 0.f, amaxL = 0.f;
+    BLines bl{};
+    if (live) {
+        column_setup(g, c, s, ix, iy, C);
+        ia_lo = (int)fmin(fmax(floor(C.Amin), 0.0), (double)g.na);   // :140,:150
+        aminL = (float)(C.Amin - ia_lo);
+        amaxL = (float)(C.Amax - ia_lo);
+        const float alim = ((double)g.na < C.Amax) ? (float)(g.na - ia_lo) : amaxL;   // :150
+        acount = alim > 0.f ? (int)ceilf(alim) : 0;   // cells ia_lo .. ia_lo + acount - 1
+        bl = blines_at(C, g.nz, iz0);
+    }
+    // weights of the first WA cells (zero beyond acount); :156
+    float wa[WA];
+#pragma unroll
+    for (int k = 0; k < WA; ++k)
+        wa[k] = (k < acount) ? fminf(k + 1.f, amaxL) - fmaxf((float)k, aminL) : 0.f;
+    const bool wide = acount > WA;   // rare: handled by the generic loop below
 
+    // ---- optional LDS staging of the workgroup's detector rectangle
+    __shared__ float2 stage[STAGE ? LDS_CELLS : 1];
+    __shared__ int rA0, rA1, rB0, rB1;
+    int A0 = 0, B0 = 0, RA = 0, RB = 0;
+    bool staged = false;
+    if (STAGE) {
+        if (tid == 0) {
+            rA0 = g.na;
+            rA1 = 0;
+            rB0 = g.nb;
+            rB1 = 0;
+        }
+        __syncthreads();
+        int a0 = g.na, a1 = 0, b0 = g.nb, b1 = 0;
+        if (live && acount > 0 && iz0 < iz1) {
+            const float d = (float)(iz1 - 1 - iz0);
+            const float lo = fminf(fminf(bl.l0, bl.l1), bl.l3);
+            const float hi = fmaxf(fmaxf(fmaf(bl.q0, d, bl.l0), fmaf(bl.q1, d, bl.l1)),
+                                   fmaf(bl.q2, d, bl.l2));
+            a0 = ia_lo;
+            a1 = min(g.na, ia_lo + acount);
+            b0 = clampi(bl.anchor + (int)floorf(lo), 0, g.nb);
+            b1 = clampi(bl.anchor + (int)ceilf(fminf(hi, 4.f * g.nb)), 0, g.nb);
+        }
+        for (int off = 32; off > 0; off >>= 1) {
+            a0 = min(a0, __shfl_xor(a0, off, 64));
+            a1 = max(a1, __shfl_xor(a1, off, 64));
+            b0 = min(b0, __shfl_xor(b0, off, 64));
+            b1 = max(b1, __shfl_xor(b1, off, 64));
+        }
+        if ((tid & 63) == 0) {
+            atomicMin(&rA0, a0);
+            atomicMax(&rA1, a1);
+            atomicMin(&rB0, b0);
+            atomicMax(&rB1, b1);
+        }
+        __syncthreads();
+        A0 = rA0;
+        B0 = rB0;
+        RA = max(rA1 - A0, 0);
+        RB = max(rB1 - B0, 0);
+        staged = RA > 0 && RB > 0 && RA * RB <= LDS_CELLS;
+        if (staged) {   // coalesced copy of the rectangle, 8 loads in flight per lane
+            const int total = RA * RB;
+            const float inv = 1.0f / (float)RA;
+            for (int base = tid; base < total; base += BX * BY * 8) {
+                float2 v[8];
+                int e[8];
+#pragma unroll
+                for (int k = 0; k < 8; ++k) {
+                    e[k] = base + k * BX * BY;
+                    int row = (int)((float)e[k] * inv);   // e < 4096: off by at most one
+                    row -= (row * RA > e[k]) ? 1 : 0;
+                    row += ((row + 1) * RA <= e[k]) ? 1 : 0;
+                    const int col = e[k] - row * RA;
+                    const long long src = (long long)(B0 + min(row, RB - 1)) * g.na + A0 + col;
+                    if (e[k] < total) {
+                        if (MODE == BACK_PLAIN) v[k] = make_float2(proj[src], 0.f);
+                        else v[k] = rp[src];
+                    }
+                }
+#pragma unroll
+                for (int k = 0; k < 8; ++k)
+                    if (e[k] < total) stage[e[k]] = v[k];
+            }
+        }
+        __syncthreads();
+    }
+    if (!live) return;
+
+    auto cell = [&](int ib, int ia) -> float2 {
+        if (STAGE) {
+            const unsigned sr = (unsigned)(ib - B0), sc = (unsigned)(ia - A0);
+            if (staged && sr < (unsigned)RB && sc < (unsigned)RA) return stage[sr * RA + sc];
+        }
+        const long long k = (long long)ib * g.na + ia;
+        if (MODE == BACK_PLAIN) return make_float2(proj[k], 0.f);
+        return rp[k];
+    };
+
+    const long long plane = (long long)g.nx * g.ny;
+    const int last = max(ia_lo + acount - 1, ia_lo);   // clamp for masked window slots
     for (int iz = iz0; iz < iz1; ++iz) {
-        double Bmin, Bmax;
-        column_b(C, g.nz, iz, Bmin, Bmax);
-        const int ib_lo = (int)fmin(fmax(floor(Bmin), 0.0), (double)g.nb);
-        const float bminL = (float)(Bmin - ib_lo);
-        const float bmaxL = (float)(Bmax - ib_lo);
-        const float blim = ((double)g.nb < Bmax) ? (float)(g.nb - ib_lo) : bmaxL;
+        // axial bounds (:131-132,:137-138) in anchor-relative FP32
+        const float d = (float)(iz - iz0);
+        const float L0 = fmaf(bl.q0, d, bl.l0), L1 = fmaf(bl.q1, d, bl.l1);
+        const float L2 = fmaf(bl.q2, d, bl.l2), L3 = fmaf(bl.q2, d, bl.l3);
+        const float lBmin = fminf(fminf(L0, L1), L3), lBmax = fmaxf(fmaxf(L0, L1), L2);
+        const float flo = floorf(lBmin);
+        const int ib_lo = max(0, bl.anchor + (int)flo);   // :142,:158
+        const float off = (float)(bl.anchor - ib_lo);     // small integer
+        const float bminL = off + lBmin, bmaxL = off + lBmax;
+        // ib < MIN(nb, b_max) (:158); an empty a range contributes nothing
+        float blim = ((float)(g.nb - bl.anchor) < lBmax) ? (float)(g.nb - ib_lo) : bmaxL;
+        if (acount == 0) blim = 0.f;
         float acc = 0.f, acc1 = 0.f;
         for (int lb = 0; (float)lb < blim; ++lb) {
             const float wb = fminf(lb + 1.f, bmaxL) - fmaxf((float)lb, bminL);   // :164
-            const int row = (ib_lo + lb) * g.na + ia_lo;
-            for (int la = 0; (float)la < alim; ++la) {
-                const float wa = fminf(la + 1.f, amaxL) - fmaxf((float)la, aminL);   // :156
-                const float w = wa * wb;
-                if (MODE == BACK_PLAIN) {
-                    acc += w * proj[row + la];
-                } else {
-                    const float2 v = rp[row + la];
-                    acc += w * v.x;
-                    acc1 += w * v.y;
+            const int ib = ib_lo + lb;
+            if (!wide) {
+                float2 v[WA];
+#pragma unroll
+                for (int k = 0; k < WA; ++k) v[k] = cell(ib, min(ia_lo + k, last));
+                float s0 = 0.f, s1 = 0.f;
+#pragma unroll
+                for (int k = 0; k < WA; ++k) {
+                    s0 = fmaf(wa[k], v[k].x, s0);
+                    s1 = fmaf(wa[k], v[k].y, s1);
+                }
+                acc = fmaf(wb, s0, acc);
+                acc1 = fmaf(wb, s1, acc1);
+            } else {
+                for (int la = 0; la < acount; ++la) {
+                    const float w = fminf(la + 1.f, amaxL) - fmaxf((float)la, aminL);
+                    const float2 v = cell(ib, ia_lo + la);
+                    acc = fmaf(wb, w * v.x, acc);
+                    acc1 = fmaf(wb, w * v.y, acc1);
                 }
             }
         }
@@ -81,26 +225,47 @@ __global__ void __launch_bounds__(BX *BY) k_backproject(GeomArgs g, double c, do
 
 int pick_zslab(const Geom &g) {
     const long long cols = (long long)((g.nx + BX - 1) / BX) * ((g.ny + BY - 1) / BY);
-    int zs = 32;
-    while (zs > 1 && cols * ((g.nz + zs - 1) / zs) < 4096) zs /= 2;
+    int zs = 4;
+    while (zs > 1 && cols * ((g.nz + zs - 1) / zs) < 2048) zs /= 2;
     return zs;
+}
+
+template <int MODE, int WA>
+void launch_wa(dim3 grid, dim3 block, hipStream_t st, const GeomArgs &ga, const ViewTrig &t,
+               const float *proj, const float2 *rp, float *out, int zs, const NormFinish &nf,
+               bool stage) {
+    if (stage)
+        hipLaunchKernelGGL((k_backproject<MODE, WA, true>), grid, block, 0, st, ga, t.c, t.s,
+                           proj, rp, out, zs, nf);
+    else
+        hipLaunchKernelGGL((k_backproject<MODE, WA, false>), grid, block, 0, st, ga, t.c, t.s,
+                           proj, rp, out, zs, nf);
+}
+
+template <int MODE>
+void launch_mode(dim3 grid, dim3 block, hipStream_t st, const GeomArgs &ga, const ViewTrig &t,
+                 int wa, const float *proj, const float2 *rp, float *out, int zs,
+                 const NormFinish &nf, bool stage) {
+    if (wa <= 4) launch_wa<MODE, 4>(grid, block, st, ga, t, proj, rp, out, zs, nf, stage);
+    else if (wa <= 6) launch_wa<MODE, 6>(grid, block, st, ga, t, proj, rp, out, zs, nf, stage);
+    else launch_wa<MODE, 8>(grid, block, st, ga, t, proj, rp, out, zs, nf, stage);
 }
 
 }  // namespace
 
-int launch_backprojector(const Geom &g, const ViewTrig &t, BackMode mode, const float *proj,
-                         const float2 *rp, float *out, hipStream_t st, const NormFinish *nfp) {
+int launch_backprojector(const Geom &g, const ViewTrig &t, const ViewInfo &vi, BackMode mode,
+                         const float *proj, const float2 *rp, float *out, hipStream_t st,
+                         const NormFinish *nfp, const BackTune *tune) {
     const GeomArgs ga{g.nx, g.ny, g.nz, g.na, g.nb, g.da, g.db, g.ai, g.bi, g.SO, g.SD};
-    const int zs = pick_zslab(g);
+    const int zs = (tune && tune->zslab > 0) ? tune->zslab : pick_zslab(g);
+    const bool stage = tune ? tune->stage != 0 : false;
     const dim3 block(BX, BY);
     const dim3 grid((g.nx + BX - 1) / BX, (g.ny + BY - 1) / BY, (g.nz + zs - 1) / zs);
     const NormFinish nf = nfp ? *nfp : NormFinish{nullptr, 0, nullptr};
     if (mode == BACK_PLAIN)
-        hipLaunchKernelGGL(k_backproject<BACK_PLAIN>, grid, block, 0, st, ga, t.c, t.s, proj,
-                           rp, out, zs, nf);
+        launch_mode<BACK_PLAIN>(grid, block, st, ga, t, vi.wa, proj, rp, out, zs, nf, stage);
     else
-        hipLaunchKernelGGL(k_backproject<BACK_RATIO>, grid, block, 0, st, ga, t.c, t.s, proj,
-                           rp, out, zs, nf);
+        launch_mode<BACK_RATIO>(grid, block, st, ga, t, vi.wa, proj, rp, out, zs, nf, stage);
     MCSART_CHECK_LAUNCH();
     return MCSART_OK;
 }

@@ -265,9 +265,9 @@ int pick_split(const Geom &g) {
 
 template <int MODE>
 int launch_mode(const Geom &g, const ViewTrig &t, const ViewInfo &vi, const ProjArgs &a,
-                hipStream_t st) {
+                hipStream_t st, int split) {
     const GeomArgs ga{g.nx, g.ny, g.nz, g.na, g.nb, g.da, g.db, g.ai, g.bi, g.SO, g.SD};
-    const int s = pick_split(g);
+    const int s = split > 0 ? split : pick_split(g);
     const dim3 block(RAYS, s);
     const dim3 grid((g.na + RAYS - 1) / RAYS, g.nb);
     const int code = (vi.wt <= 2 ? 0 : vi.wt == 3 ? 1 : 2) * 3 + (vi.wz <= 2 ? 0 : vi.wz == 3 ? 1 : 2);
@@ -378,7 +378,9 @@ __global__ void __launch_bounds__(256) k_sart_finish(ProjArgs a, int n) {
     a.partials[2 * blockIdx.x + 1] = sr[0] + sr[1] + sr[2] + sr[3];
 }
 
-__global__ void __launch_bounds__(64) k_norm_finish(NormFinish nf) { finish_norm_wave(nf); }
+__global__ void __launch_bounds__(64) k_norm_finish(NormFinish nf) {
+    finish_norm_wave(nf, threadIdx.x);
+}
 
 }  // namespace
 
@@ -400,13 +402,16 @@ int projector_partials(const Geom &g) {
 }
 
 int launch_projector(const Geom &g, const ViewTrig &t, const ViewInfo &vi, ProjMode mode,
-                     const ProjArgs &a, hipStream_t st, int *nparts) {
+                     const ProjArgs &a, hipStream_t st, int *nparts, int split) {
     const bool wide = !(vi.wt <= 3 && vi.wz <= 3);
+    if (split < 0 || split > MAXSPLIT || (split & (split - 1)))
+        return fail(MCSART_EINVAL, "projector split must be 0 (automatic) or a power of two <= " +
+                                       std::to_string(MAXSPLIT));
     int rc;
     switch (mode) {
-        case PROJ_PLAIN: rc = launch_mode<PROJ_PLAIN>(g, t, vi, a, st); break;
-        case PROJ_IMAGE: rc = launch_mode<PROJ_IMAGE>(g, t, vi, a, st); break;
-        default: rc = launch_mode<PROJ_MOTION>(g, t, vi, a, st); break;
+        case PROJ_PLAIN: rc = launch_mode<PROJ_PLAIN>(g, t, vi, a, st, split); break;
+        case PROJ_IMAGE: rc = launch_mode<PROJ_IMAGE>(g, t, vi, a, st, split); break;
+        default: rc = launch_mode<PROJ_MOTION>(g, t, vi, a, st, split); break;
     }
     if (nparts) *nparts = ((g.na + RAYS - 1) / RAYS) * g.nb;
     if (rc || !wide || mode == PROJ_PLAIN) return rc;

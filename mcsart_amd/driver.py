@@ -129,6 +129,15 @@ class COpts(ctypes.Structure):
                 ("keep_residual", ctypes.c_int), ("lam", ctypes.c_float)]
 
 
+class CTuning(ctypes.Structure):
+    """Launch shapes of the footprint kernels (0 = automatic)."""
+    _fields_ = [("proj_split", ctypes.c_int), ("back_zslab", ctypes.c_int),
+                ("back_stage", ctypes.c_int)]
+
+    def as_dict(self) -> dict:
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
 TIMING_KINDS = {"projector": 0, "backprojector": 1, "warp": 2, "motion": 3}
 
 
@@ -236,6 +245,21 @@ class Job:
 
     def set_graphs(self, enable: bool):
         check(self._lib.mcsart_job_set_graphs(self.ptr, int(enable)), "mcsart_job_set_graphs")
+
+    def set_tuning(self, proj_split: int = 0, back_zslab: int = 0, back_stage: int = 0):
+        t = CTuning(proj_split, back_zslab, back_stage)
+        check(self._lib.mcsart_job_set_tuning(self.ptr, ctypes.byref(t)), "mcsart_job_set_tuning")
+
+    def tuning(self) -> dict:
+        t = CTuning()
+        check(self._lib.mcsart_job_get_tuning(self.ptr, ctypes.byref(t)), "mcsart_job_get_tuning")
+        return t.as_dict()
+
+    def autotune(self) -> dict:
+        """Time candidate launch shapes on a few views, keep the fastest, return it."""
+        t = CTuning()
+        check(self._lib.mcsart_job_autotune(self.ptr, ctypes.byref(t)), "mcsart_job_autotune")
+        return t.as_dict()
 
     def run(self):
         check(self._lib.mcsart_job_run(self.ptr), "mcsart_job_run")

@@ -7,6 +7,7 @@ import pytest
 
 import oracle
 from mcsart_amd import configs
+from mcsart_amd.geometry import Geometry
 from mcsart_amd.ops import (DeviceArray, backproject, deform, diff_forward, host_add, host_add2,
                             host_deform, host_deform2, host_diff, host_division, host_initial,
                             host_initial2, host_update_udvf, project)
@@ -70,6 +71,32 @@ def test_backprojection_parity(gpu_lib, spec_fn, views):
         ref = oracle.backprojection(proj, g, ang)
         e = rel_l2(got, ref)
         assert e <= TOL, f"{spec.name} view {v}: rel-L2 {e:.3e}"
+
+
+# Irregular geometries (voxel units): a detector much finer than the voxel shadow (footprints
+# wider than the fixed windows: the projector's and backprojector's generic loops), a truncated
+# detector offset from the axis (rays and footprints clipped at the detector edges), and sizes
+# that are multiples of nothing.
+IRREGULAR = {
+    "fine_detector": Geometry.from_user(24, 24, 16, 150, 70, da=0.2, db=0.25, SO=60.0, SD=100.0),
+    "truncated_offset": Geometry.from_user(40, 40, 24, 30, 14, da=1.5, db=1.5, ai=9.0, bi=-4.5,
+                                           SO=70.0, SD=120.0),
+    "odd_sizes": Geometry.from_user(37, 37, 19, 53, 31, da=1.3, db=1.1, ai=0.35, bi=-0.2,
+                                    SO=55.0, SD=97.0),
+}
+
+
+@pytest.mark.parametrize("name", sorted(IRREGULAR))
+def test_irregular_geometry_parity(gpu_lib, name):
+    g = IRREGULAR[name]
+    vol = smooth_random(g.vol_shape, 5)
+    for k, ang in enumerate((0.0, 0.4, np.pi / 4, 1.3, np.pi / 2, 2.6, 4.0)):
+        a = np.float32(ang)
+        e = rel_l2(project(vol, g, a), oracle.projection(vol, g, a))
+        assert e <= TOL, f"{name} projection angle {ang}: rel-L2 {e:.3e}"
+        proj = smooth_random(g.det_shape, 40 + k)
+        e = rel_l2(backproject(proj, g, a), oracle.backprojection(proj, g, a))
+        assert e <= TOL, f"{name} backprojection angle {ang}: rel-L2 {e:.3e}"
 
 
 def test_backprojection_constant(gpu_lib):
