@@ -31,32 +31,6 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def gpu_measurements(spec, data, photons_seed):
-    """Measured projections p_i = P(W(phantom; alpha_i v + beta_i f)) with Poisson noise, made on
-    the GPU with the library's own operators."""
-    from mcsart_amd.ops import DeviceArray, host_deform, host_projection
-    from mcsart_amd.synth import poisson_noise
-    g = spec.geometry
-    d_ph = DeviceArray.from_host(data.phantom)
-    d_w = DeviceArray.empty(g.vol_shape)
-    d_f = [DeviceArray.from_host(f) for f in data.fields] if data.fields else None
-    d_p = DeviceArray.empty(g.det_shape)
-    out = np.empty((spec.n_views,) + g.det_shape, np.float32)
-    for i, a in enumerate(spec.angles):
-        src = d_ph
-        if d_f is not None:
-            host_deform(d_w, d_ph, g.nx, g.ny, g.nz, float(data.alpha[i]), float(data.beta[i]), *d_f)
-            src = d_w
-        host_projection(d_p, src, float(a), g.SO, g.SD, g.da, g.na, g.ai, g.db, g.nb, g.bi,
-                        g.nx, g.ny, g.nz)
-        out[i] = d_p.to_host()
-    for x in [d_ph, d_w, d_p] + (d_f or []):
-        x.close()
-    if spec.noise_photons:
-        out = poisson_noise(out, photons_seed, photons=spec.noise_photons, dx_mm=spec.vx_mm)
-    return out
-
-
 def algorithmic_bytes(spec, kind):
     """HBM bytes one launch must move at minimum (4-byte floats).
     projector, image sweep : read x (N) and p (D), write (r, P1) (2D)      -> 4N + 12D
@@ -129,6 +103,7 @@ def main():
     from mcsart_amd import configs
     from mcsart_amd.configs import make_scan_inputs
     from mcsart_amd.driver import Context, Job
+    from mcsart_amd.simulate import measure
 
     spec = configs.c2_mcsart128()
     g = spec.geometry
@@ -136,7 +111,7 @@ def main():
         f"op_iter={spec.op_iter}")
     data = make_scan_inputs(spec)
     ctx = Context(0)
-    proj = gpu_measurements(spec, data, spec.seed + 404)
+    proj = measure(spec, data, spec.seed + 404)
     log("[bench] measured projections ready")
 
     job = Job(ctx, g, spec.angles, data.alpha, data.beta, spec.op_iter, lam=spec.lam,
@@ -144,6 +119,10 @@ def main():
     job.set_projections(proj)
     job.set_image(None)
     job.set_fields(data.fields)
+
+    # Launch shapes of the two footprint kernels, timed on a few of this scan's views.
+    tuning = job.autotune()
+    log(f"[bench] autotuned launch shapes: {tuning}")
 
     # Pass 1 (the measurement): each sweep replays its captured HIP graph; the warm-up captures.
     for w in range(max(args.warmup, 1)):
@@ -230,6 +209,7 @@ def main():
         "kernels": kernel_ms,
         "eager_ms_per_scan_with_events": round(eager_ms, 2),
         "launch": "one HIP graph per sweep",
+        "tuning": tuning,
         "roofline": roofline,
         "cpu_baseline": cpu,
         "output_finite_fraction": finite,

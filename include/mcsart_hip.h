@@ -158,6 +158,59 @@ int mcsart_motion_update(float *alpha_x, float *alpha_y, float *alpha_z, float *
                          float wv, float wf, int nx, int ny, int nz);
 
 /* ---------------------------------------------------------------------------------------------
+ * 2b. Asynchronous operators: the same operators enqueued on a context's stream (the context owns
+ *     the stream, the scratch buffers and the kernel timers; its device becomes current).  They
+ *     return once the work is enqueued; mcsart_ctx_synchronize() waits for it.  Scratch buffers are
+ *     reused in stream order, so calls on one context never race one another; calls on different
+ *     contexts share nothing.  No counterpart in the reference (its calls all end with
+ *     cudaDeviceSynchronize()).
+ * ------------------------------------------------------------------------------------------- */
+int mcsart_projection_async(mcsart_ctx *ctx, float *d_proj, const float *d_img, float angle,
+                            float SO, float SD, float da, int na, float ai, float db, int nb,
+                            float bi, int nx, int ny, int nz);
+int mcsart_backprojection_async(mcsart_ctx *ctx, float *d_img, const float *d_proj, float angle,
+                                float SO, float SD, float da, int na, float ai, float db, int nb,
+                                float bi, int nx, int ny, int nz);
+int mcsart_deform_async(mcsart_ctx *ctx, float *d_img1, const float *d_img, int nx, int ny, int nz,
+                        float volume, float flow, const float *alpha_x, const float *alpha_y,
+                        const float *alpha_z, const float *beta_x, const float *beta_y,
+                        const float *beta_z);
+int mcsart_deform2_async(mcsart_ctx *ctx, float *d_img1, const float *d_img, int nx, int ny, int nz,
+                         float volume, float flow, const float *alpha_x, const float *alpha_y,
+                         const float *alpha_z, const float *beta_x, const float *beta_y,
+                         const float *beta_z);
+int mcsart_update_udvf_async(mcsart_ctx *ctx, float *alpha_x, float *alpha_y, float *alpha_z,
+                             float *beta_x, float *beta_y, float *beta_z, const float *img,
+                             const float *img0, float volume_diff, float flow_diff, int nx, int ny,
+                             int nz, int iter);
+int mcsart_add_async(mcsart_ctx *ctx, float *img1, const float *img, int nx, int ny, int nz,
+                     float weight);
+int mcsart_add2_async(mcsart_ctx *ctx, float *img1, const float *img, int nx, int ny, int nz,
+                      const float *img0, float weight, int ind);
+int mcsart_initial_async(mcsart_ctx *ctx, float *img, int nx, int ny, int nz, float value);
+int mcsart_initial2_async(mcsart_ctx *ctx, float *img, int nx, int ny, int nz, const float *img0,
+                          float volume, float flow);
+int mcsart_division_async(mcsart_ctx *ctx, float *img1, const float *img, int nx, int ny, int nz);
+int mcsart_diff_async(mcsart_ctx *ctx, float *img1, const float *img, int nx, int ny, int nz,
+                      int ind);
+int mcsart_diff_forward_async(mcsart_ctx *ctx, float *img1, const float *img, int nx, int ny,
+                              int nz, int ind);
+int mcsart_sart_residual_async(mcsart_ctx *ctx, float *d_rp, float *d_norm, const float *d_vol,
+                               int motion_pairs, const float *d_meas, float angle, float SO,
+                               float SD, float da, int na, float ai, float db, int nb, float bi,
+                               int nx, int ny, int nz);
+int mcsart_sart_backprojection_async(mcsart_ctx *ctx, float *d_u, const float *d_rp, float angle,
+                                     float SO, float SD, float da, int na, float ai, float db,
+                                     int nb, float bi, int nx, int ny, int nz);
+int mcsart_motion_prep_async(mcsart_ctx *ctx, float *d_pairs, const float *d_img,
+                             const float *alpha_x, const float *alpha_y, const float *alpha_z,
+                             const float *beta_x, const float *beta_y, const float *beta_z,
+                             float vd, float fd, int nx, int ny, int nz);
+int mcsart_motion_update_async(mcsart_ctx *ctx, float *alpha_x, float *alpha_y, float *alpha_z,
+                               float *beta_x, float *beta_y, float *beta_z, const float *d_u,
+                               const float *d_img, float wv, float wf, int nx, int ny, int nz);
+
+/* ---------------------------------------------------------------------------------------------
  * 3. Drivers: host_AUMISART (include/AUMISART.h:17,19), host pointers, same argument order.
  * ------------------------------------------------------------------------------------------- */
 
@@ -247,6 +300,66 @@ int mcsart_job_get_tuning(mcsart_job *job, mcsart_tuning *t);
  * current image, backprojector in ratio mode into scratch; the job's image, fields and norms are
  * untouched), keep the fastest and report it in *chosen (may be NULL).  Synchronous. */
 int mcsart_job_autotune(mcsart_job *job, mcsart_tuning *chosen);
+
+/* ---------------------------------------------------------------------------------------------
+ * 5. Reconstruction server: continuous batching of jobs on one device (no counterpart in the
+ *    reference, which reconstructs one scan per MATLAB call, src/entry_AUMISART.cu:193-207).
+ *    Requests queue in arrival order; a scheduler thread admits one whenever a slot (a stream)
+ *    is free and its device footprint (mcsart_job_bytes) fits in the remaining HBM budget, runs
+ *    it as a device-resident job on that slot, and on completion copies the results into the
+ *    request's output buffers and frees the job.  Jobs on different slots overlap on the device.
+ *    A request's results do not depend on what else runs (same kernels, shapes and order).
+ * ------------------------------------------------------------------------------------------- */
+typedef struct mcsart_server mcsart_server;
+
+typedef struct {
+    int slots;           /* jobs in flight at once, one stream each: 1..16 */
+    size_t hbm_budget;   /* device bytes admitted jobs may hold together; 0: 90% of free memory */
+    int autotune;        /* 1: autotune each job's launch shapes after admission */
+    int proj_split, back_zslab, back_stage;   /* fixed launch shapes when autotune == 0 */
+} mcsart_server_opts;
+
+/* All host buffers must stay valid until the request has completed (mcsart_server_wait). */
+typedef struct {
+    mcsart_geom geom;
+    int n_views;
+    const float *angles, *volumes, *flows;   /* n_views each; copied at submit */
+    mcsart_opts opts;
+    int n_iter;
+    const int *op_iter;                      /* copied at submit */
+    const float *h_proj;                     /* n_views*na*nb measured projections */
+    const float *h_img;                      /* initial image or NULL (zero) */
+    const float *h_fields[6];                /* ax ay az bx by bz, each N or NULL (zero) */
+    float *out_img;                          /* N, or NULL */
+    float *out_norm;                         /* n_iter*n_views, or NULL */
+    float *out_fields[6];                    /* each N or NULL */
+} mcsart_request;
+
+enum { MCSART_REQ_QUEUED = 0, MCSART_REQ_RUNNING = 1, MCSART_REQ_DONE = 2, MCSART_REQ_FAILED = 3 };
+
+typedef struct {
+    int state;          /* MCSART_REQ_* */
+    int status;         /* MCSART_OK or the error that failed the request */
+    int slot;           /* slot it ran on, -1 while queued */
+    size_t bytes;       /* device footprint */
+    double queue_ms;    /* submit -> admission */
+    double run_ms;      /* admission -> results in the output buffers */
+    double total_ms;    /* submit -> results in the output buffers */
+    mcsart_tuning tuning;   /* launch shapes it ran with */
+} mcsart_request_info;
+
+int mcsart_server_create(int device, const mcsart_server_opts *opts, mcsart_server **server);
+/* MCSART_ENOMEM if the request could never fit the budget. */
+int mcsart_server_submit(mcsart_server *server, const mcsart_request *req, long long *id);
+/* Block until request id has completed (timeout_s < 0: no limit; MCSART_EBUSY on timeout);
+ * returns the request's own status. */
+int mcsart_server_wait(mcsart_server *server, long long id, double timeout_s);
+int mcsart_server_drain(mcsart_server *server);   /* wait for every submitted request */
+int mcsart_server_info(mcsart_server *server, long long id, mcsart_request_info *info);
+int mcsart_server_release(mcsart_server *server, long long id);   /* forget a completed request */
+int mcsart_server_memory(mcsart_server *server, size_t *used, size_t *budget);
+/* Runs every queued request to completion, then stops the scheduler and frees the slots. */
+int mcsart_server_destroy(mcsart_server *server);
 
 /* Per-kernel timing of the last run(s) on this context, from HIP events recorded on the stream
  * the kernels run on.  kind: 0 projector, 1 backprojector, 2 warp, 3 motion elementwise. */

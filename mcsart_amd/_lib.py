@@ -71,7 +71,21 @@ SIGNATURES = {
     "mcsart_ctx_kernel_time": ([_P, _I, ctypes.POINTER(ctypes.c_double),
                                 ctypes.POINTER(ctypes.c_longlong)], _I),
     "mcsart_ctx_reset_timing": ([_P], _I),
+    "mcsart_server_create": ([_I, _P, ctypes.POINTER(_P)], _I),
+    "mcsart_server_submit": ([_P, _P, ctypes.POINTER(ctypes.c_longlong)], _I),
+    "mcsart_server_wait": ([_P, ctypes.c_longlong, ctypes.c_double], _I),
+    "mcsart_server_drain": ([_P], _I),
+    "mcsart_server_info": ([_P, ctypes.c_longlong, _P], _I),
+    "mcsart_server_release": ([_P, ctypes.c_longlong], _I),
+    "mcsart_server_memory": ([_P, ctypes.POINTER(_SZ), ctypes.POINTER(_SZ)], _I),
+    "mcsart_server_destroy": ([_P], _I),
 }
+# Asynchronous operators: a context, then the synchronous operator's arguments.
+ASYNC_OPS = ("projection", "backprojection", "deform", "deform2", "update_udvf", "add", "add2",
+             "initial", "initial2", "division", "diff", "diff_forward", "sart_residual",
+             "sart_backprojection", "motion_prep", "motion_update")
+for _op in ASYNC_OPS:
+    SIGNATURES[f"mcsart_{_op}_async"] = ([_P] + SIGNATURES[f"mcsart_{_op}"][0], _I)
 
 _lib = None
 

@@ -18,8 +18,8 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libmcsart_hip.so")
 
-SOURCES = ["capi.hip", "driver.hip", "k_project.hip", "k_backproject.hip", "k_elem.hip",
-           "batch.hip"]
+SOURCES = ["capi.hip", "driver.hip", "serve.hip", "k_project.hip", "k_backproject.hip",
+           "k_elem.hip"]
 
 ARCH = os.environ.get("MCSART_OFFLOAD_ARCH", "gfx950")
 CXXFLAGS = [

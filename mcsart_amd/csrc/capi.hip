@@ -310,16 +310,21 @@ int finish(mcsart_ctx *ctx, int rc) {
     mcsart_ctx *ctx = default_ctx();                                                 \
     if (!ctx) return fail(MCSART_EHIP, "no HIP device / context")
 
+// Asynchronous operators run on the given context's stream (its device becomes current).
+#define CHECK_CTX()                                                                  \
+    if (!ctx) return fail(MCSART_EINVAL, "null context");                           \
+    MCSART_HIP_TRY(hipSetDevice(ctx->device))
+
 }  // namespace
 
-extern "C" int mcsart_projection(float *d_proj, const float *d_img, float angle, float SO,
-                                 float SD, float da, int na, float ai, float db, int nb, float bi,
-                                 int nx, int ny, int nz) {
+extern "C" int mcsart_projection_async(mcsart_ctx *ctx, float *d_proj, const float *d_img,
+                                       float angle, float SO, float SD, float da, int na, float ai,
+                                       float db, int nb, float bi, int nx, int ny, int nz) {
     const Geom g{nx, ny, nz, na, nb, da, db, ai, bi, SO, SD};
     int rc = validate_geom(g);
     if (rc) return rc;
     if (!d_proj || !d_img) return fail(MCSART_EINVAL, "null device pointer");
-    GET_CTX();
+    CHECK_CTX();
     const ViewTrig t = view_trig(angle);
     const ViewInfo vi = view_info_of(g, t);
     ProjArgs a{};
@@ -334,33 +339,50 @@ extern "C" int mcsart_projection(float *d_proj, const float *d_img, float angle,
         a.vol0T = (const float *)tmp;
     }
     TimedRegion tr(ctx, TK_PROJ);
-    return finish(ctx, launch_projector(g, t, vi, PROJ_PLAIN, a, ctx->stream));
+    return launch_projector(g, t, vi, PROJ_PLAIN, a, ctx->stream);
+}
+
+extern "C" int mcsart_projection(float *d_proj, const float *d_img, float angle, float SO, float SD,
+                                 float da, int na, float ai, float db, int nb, float bi, int nx,
+                                 int ny, int nz) {
+    GET_CTX();
+    return finish(ctx, mcsart_projection_async(ctx, d_proj, d_img, angle, SO, SD, da, na, ai, db,
+                                               nb, bi, nx, ny, nz));
+}
+
+extern "C" int mcsart_backprojection_async(mcsart_ctx *ctx, float *d_img, const float *d_proj,
+                                           float angle, float SO, float SD, float da, int na,
+                                           float ai, float db, int nb, float bi, int nx, int ny,
+                                           int nz) {
+    const Geom g{nx, ny, nz, na, nb, da, db, ai, bi, SO, SD};
+    int rc = validate_geom(g);
+    if (rc) return rc;
+    if (!d_proj || !d_img) return fail(MCSART_EINVAL, "null device pointer");
+    CHECK_CTX();
+    const ViewTrig t = view_trig(angle);
+    const ViewInfo vi = view_info_of(g, t);
+    TimedRegion tr(ctx, TK_BACK);
+    return launch_backprojector(g, t, vi, BACK_PLAIN, d_proj, nullptr, d_img,
+                                            ctx->stream);
 }
 
 extern "C" int mcsart_backprojection(float *d_img, const float *d_proj, float angle, float SO,
                                      float SD, float da, int na, float ai, float db, int nb,
                                      float bi, int nx, int ny, int nz) {
-    const Geom g{nx, ny, nz, na, nb, da, db, ai, bi, SO, SD};
-    int rc = validate_geom(g);
-    if (rc) return rc;
-    if (!d_proj || !d_img) return fail(MCSART_EINVAL, "null device pointer");
     GET_CTX();
-    const ViewTrig t = view_trig(angle);
-    const ViewInfo vi = view_info_of(g, t);
-    TimedRegion tr(ctx, TK_BACK);
-    return finish(ctx, launch_backprojector(g, t, vi, BACK_PLAIN, d_proj, nullptr, d_img,
-                                            ctx->stream));
+    return finish(ctx, mcsart_backprojection_async(ctx, d_img, d_proj, angle, SO, SD, da, na, ai,
+                                                   db, nb, bi, nx, ny, nz));
 }
 
-extern "C" int mcsart_deform(float *d_img1, const float *d_img, int nx, int ny, int nz,
-                             float volume, float flow, const float *alpha_x, const float *alpha_y,
-                             const float *alpha_z, const float *beta_x, const float *beta_y,
-                             const float *beta_z) {
+extern "C" int mcsart_deform_async(mcsart_ctx *ctx, float *d_img1, const float *d_img, int nx,
+                                   int ny, int nz, float volume, float flow, const float *alpha_x,
+                                   const float *alpha_y, const float *alpha_z, const float *beta_x,
+                                   const float *beta_y, const float *beta_z) {
     int rc = validate_volume(nx, ny, nz, true);
     if (rc) return rc;
     if (!d_img1 || !d_img || !alpha_x || !alpha_y || !alpha_z || !beta_x || !beta_y || !beta_z)
         return fail(MCSART_EINVAL, "null device pointer");
-    GET_CTX();
+    CHECK_CTX();
     const Fields f{alpha_x, alpha_y, alpha_z, beta_x, beta_y, beta_z};
     const float *src = d_img;
     const long long n = (long long)nx * ny * nz;
@@ -373,18 +395,27 @@ extern "C" int mcsart_deform(float *d_img1, const float *d_img, int nx, int ny, 
         src = (const float *)tmp;
     }
     TimedRegion tr(ctx, TK_WARP);
-    return finish(ctx, launch_warp(d_img1, nullptr, src, nx, ny, nz, volume, flow, f, ctx->stream));
+    return launch_warp(d_img1, nullptr, src, nx, ny, nz, volume, flow, f, ctx->stream);
 }
 
-extern "C" int mcsart_deform2(float *d_img1, const float *d_img, int nx, int ny, int nz,
-                              float volume, float flow, const float *alpha_x,
-                              const float *alpha_y, const float *alpha_z, const float *beta_x,
-                              const float *beta_y, const float *beta_z) {
+extern "C" int mcsart_deform(float *d_img1, const float *d_img, int nx, int ny, int nz,
+                             float volume, float flow, const float *alpha_x, const float *alpha_y,
+                             const float *alpha_z, const float *beta_x, const float *beta_y,
+                             const float *beta_z) {
+    GET_CTX();
+    return finish(ctx, mcsart_deform_async(ctx, d_img1, d_img, nx, ny, nz, volume, flow, alpha_x,
+                                           alpha_y, alpha_z, beta_x, beta_y, beta_z));
+}
+
+extern "C" int mcsart_deform2_async(mcsart_ctx *ctx, float *d_img1, const float *d_img, int nx,
+                                    int ny, int nz, float volume, float flow, const float *alpha_x,
+                                    const float *alpha_y, const float *alpha_z, const float *beta_x,
+                                    const float *beta_y, const float *beta_z) {
     int rc = validate_volume(nx, ny, nz, true);
     if (rc) return rc;
     if (!d_img1 || !d_img || !alpha_x || !alpha_y || !alpha_z || !beta_x || !beta_y || !beta_z)
         return fail(MCSART_EINVAL, "null device pointer");
-    GET_CTX();
+    CHECK_CTX();
     const Fields f{alpha_x, alpha_y, alpha_z, beta_x, beta_y, beta_z};
     const float *src = d_img;
     const long long n = (long long)nx * ny * nz;
@@ -396,102 +427,163 @@ extern "C" int mcsart_deform2(float *d_img1, const float *d_img, int nx, int ny,
                                       ctx->stream));
         src = (const float *)tmp;
     }
-    return finish(ctx, launch_deform2(d_img1, src, nx, ny, nz, volume, flow, f, ctx->stream));
+    return launch_deform2(d_img1, src, nx, ny, nz, volume, flow, f, ctx->stream);
 }
 
-extern "C" int mcsart_update_udvf(float *alpha_x, float *alpha_y, float *alpha_z, float *beta_x,
-                                  float *beta_y, float *beta_z, const float *img,
-                                  const float *img0, float volume_diff, float flow_diff, int nx,
-                                  int ny, int nz, int iter) {
+extern "C" int mcsart_deform2(float *d_img1, const float *d_img, int nx, int ny, int nz,
+                              float volume, float flow, const float *alpha_x, const float *alpha_y,
+                              const float *alpha_z, const float *beta_x, const float *beta_y,
+                              const float *beta_z) {
+    GET_CTX();
+    return finish(ctx, mcsart_deform2_async(ctx, d_img1, d_img, nx, ny, nz, volume, flow, alpha_x,
+                                            alpha_y, alpha_z, beta_x, beta_y, beta_z));
+}
+
+extern "C" int mcsart_update_udvf_async(mcsart_ctx *ctx, float *alpha_x, float *alpha_y,
+                                        float *alpha_z, float *beta_x, float *beta_y, float *beta_z,
+                                        const float *img, const float *img0, float volume_diff,
+                                        float flow_diff, int nx, int ny, int nz, int iter) {
     (void)iter;   // unused in the reference (src/cu_update_udvf.cu:11-62)
     int rc = validate_volume(nx, ny, nz, false);
     if (rc) return rc;
     if (!alpha_x || !alpha_y || !alpha_z || !beta_x || !beta_y || !beta_z || !img || !img0)
         return fail(MCSART_EINVAL, "null device pointer");
-    GET_CTX();
-    return finish(ctx, launch_update_udvf(alpha_x, alpha_y, alpha_z, beta_x, beta_y, beta_z, img,
-                                          img0, volume_diff, flow_diff, nx, ny, nz, ctx->stream));
+    CHECK_CTX();
+    return launch_update_udvf(alpha_x, alpha_y, alpha_z, beta_x, beta_y, beta_z, img,
+                                          img0, volume_diff, flow_diff, nx, ny, nz, ctx->stream);
 }
 
-extern "C" int mcsart_add(float *img1, const float *img, int nx, int ny, int nz, float weight) {
+extern "C" int mcsart_update_udvf(float *alpha_x, float *alpha_y, float *alpha_z, float *beta_x,
+                                  float *beta_y, float *beta_z, const float *img, const float *img0,
+                                  float volume_diff, float flow_diff, int nx, int ny, int nz,
+                                  int iter) {
+    GET_CTX();
+    return finish(ctx, mcsart_update_udvf_async(ctx, alpha_x, alpha_y, alpha_z, beta_x, beta_y,
+                                                beta_z, img, img0, volume_diff, flow_diff, nx, ny,
+                                                nz, iter));
+}
+
+extern "C" int mcsart_add_async(mcsart_ctx *ctx, float *img1, const float *img, int nx, int ny,
+                                int nz, float weight) {
     int rc = validate_volume(nx, ny, nz, false);
     if (rc) return rc;
     if (!img1 || !img) return fail(MCSART_EINVAL, "null device pointer");
-    GET_CTX();
-    return finish(ctx, launch_add(img1, img, (long long)nx * ny * nz, weight, ctx->stream));
+    CHECK_CTX();
+    return launch_add(img1, img, (long long)nx * ny * nz, weight, ctx->stream);
 }
 
-extern "C" int mcsart_add2(float *img1, const float *img, int nx, int ny, int nz,
-                           const float *img0, float weight, int ind) {
+extern "C" int mcsart_add(float *img1, const float *img, int nx, int ny, int nz, float weight) {
+    GET_CTX();
+    return finish(ctx, mcsart_add_async(ctx, img1, img, nx, ny, nz, weight));
+}
+
+extern "C" int mcsart_add2_async(mcsart_ctx *ctx, float *img1, const float *img, int nx, int ny,
+                                 int nz, const float *img0, float weight, int ind) {
     int rc = validate_volume(nx, ny, nz, true);
     if (rc) return rc;
     if (!img1 || !img || !img0) return fail(MCSART_EINVAL, "null device pointer");
     if (ind < 1 || ind > 3) return fail(MCSART_EINVAL, "ind must be 1, 2 or 3");
     if (img1 == img || img1 == img0)
         return fail(MCSART_EINVAL, "add2 output must not alias its inputs");
-    GET_CTX();
-    return finish(ctx, launch_add2(img1, img, nx, ny, nz, img0, weight, ind, ctx->stream));
+    CHECK_CTX();
+    return launch_add2(img1, img, nx, ny, nz, img0, weight, ind, ctx->stream);
 }
 
-extern "C" int mcsart_initial(float *img, int nx, int ny, int nz, float value) {
+extern "C" int mcsart_add2(float *img1, const float *img, int nx, int ny, int nz, const float *img0,
+                           float weight, int ind) {
+    GET_CTX();
+    return finish(ctx, mcsart_add2_async(ctx, img1, img, nx, ny, nz, img0, weight, ind));
+}
+
+extern "C" int mcsart_initial_async(mcsart_ctx *ctx, float *img, int nx, int ny, int nz,
+                                    float value) {
     int rc = validate_volume(nx, ny, nz, false);
     if (rc) return rc;
     if (!img) return fail(MCSART_EINVAL, "null device pointer");
-    GET_CTX();
-    return finish(ctx, launch_fill(img, (long long)nx * ny * nz, value, ctx->stream));
+    CHECK_CTX();
+    return launch_fill(img, (long long)nx * ny * nz, value, ctx->stream);
 }
 
-extern "C" int mcsart_initial2(float *img, int nx, int ny, int nz, const float *img0,
-                               float volume, float flow) {
+extern "C" int mcsart_initial(float *img, int nx, int ny, int nz, float value) {
+    GET_CTX();
+    return finish(ctx, mcsart_initial_async(ctx, img, nx, ny, nz, value));
+}
+
+extern "C" int mcsart_initial2_async(mcsart_ctx *ctx, float *img, int nx, int ny, int nz,
+                                     const float *img0, float volume, float flow) {
     int rc = validate_volume(nx, ny, nz, false);
     if (rc) return rc;
     if (!img || !img0) return fail(MCSART_EINVAL, "null device pointer");
     if (img == img0) return fail(MCSART_EINVAL, "initial2 output must not alias its input");
+    CHECK_CTX();
+    return launch_initial2(img, nx, ny, nz, img0, volume, flow, ctx->stream);
+}
+
+extern "C" int mcsart_initial2(float *img, int nx, int ny, int nz, const float *img0, float volume,
+                               float flow) {
     GET_CTX();
-    return finish(ctx, launch_initial2(img, nx, ny, nz, img0, volume, flow, ctx->stream));
+    return finish(ctx, mcsart_initial2_async(ctx, img, nx, ny, nz, img0, volume, flow));
+}
+
+extern "C" int mcsart_division_async(mcsart_ctx *ctx, float *img1, const float *img, int nx, int ny,
+                                     int nz) {
+    int rc = validate_volume(nx, ny, nz, false);
+    if (rc) return rc;
+    if (!img1 || !img) return fail(MCSART_EINVAL, "null device pointer");
+    CHECK_CTX();
+    return launch_division(img1, img, (long long)nx * ny * nz, ctx->stream);
 }
 
 extern "C" int mcsart_division(float *img1, const float *img, int nx, int ny, int nz) {
+    GET_CTX();
+    return finish(ctx, mcsart_division_async(ctx, img1, img, nx, ny, nz));
+}
+
+extern "C" int mcsart_diff_async(mcsart_ctx *ctx, float *img1, const float *img, int nx, int ny,
+                                 int nz, int ind) {
     int rc = validate_volume(nx, ny, nz, false);
     if (rc) return rc;
     if (!img1 || !img) return fail(MCSART_EINVAL, "null device pointer");
-    GET_CTX();
-    return finish(ctx, launch_division(img1, img, (long long)nx * ny * nz, ctx->stream));
+    if (ind < 1 || ind > 3) return fail(MCSART_EINVAL, "ind must be 1, 2 or 3");
+    if (img1 == img) return fail(MCSART_EINVAL, "diff output must not alias its input");
+    CHECK_CTX();
+    return launch_diff(img1, img, nx, ny, nz, ind, 1, ctx->stream);
 }
 
 extern "C" int mcsart_diff(float *img1, const float *img, int nx, int ny, int nz, int ind) {
-    int rc = validate_volume(nx, ny, nz, false);
-    if (rc) return rc;
-    if (!img1 || !img) return fail(MCSART_EINVAL, "null device pointer");
-    if (ind < 1 || ind > 3) return fail(MCSART_EINVAL, "ind must be 1, 2 or 3");
-    if (img1 == img) return fail(MCSART_EINVAL, "diff output must not alias its input");
     GET_CTX();
-    return finish(ctx, launch_diff(img1, img, nx, ny, nz, ind, 1, ctx->stream));
+    return finish(ctx, mcsart_diff_async(ctx, img1, img, nx, ny, nz, ind));
 }
 
-extern "C" int mcsart_diff_forward(float *img1, const float *img, int nx, int ny, int nz,
-                                   int ind) {
+extern "C" int mcsart_diff_forward_async(mcsart_ctx *ctx, float *img1, const float *img, int nx,
+                                         int ny, int nz, int ind) {
     int rc = validate_volume(nx, ny, nz, false);
     if (rc) return rc;
     if (!img1 || !img) return fail(MCSART_EINVAL, "null device pointer");
     if (ind < 1 || ind > 3) return fail(MCSART_EINVAL, "ind must be 1, 2 or 3");
     if (img1 == img) return fail(MCSART_EINVAL, "diff output must not alias its input");
+    CHECK_CTX();
+    return launch_diff(img1, img, nx, ny, nz, ind, 0, ctx->stream);
+}
+
+extern "C" int mcsart_diff_forward(float *img1, const float *img, int nx, int ny, int nz, int ind) {
     GET_CTX();
-    return finish(ctx, launch_diff(img1, img, nx, ny, nz, ind, 0, ctx->stream));
+    return finish(ctx, mcsart_diff_forward_async(ctx, img1, img, nx, ny, nz, ind));
 }
 
 // ---------------------------------------------------------------------------------------------
 // fused SART steps
 // ---------------------------------------------------------------------------------------------
-extern "C" int mcsart_sart_residual(float *d_rp, float *d_norm, const float *d_vol,
-                                    int motion_pairs, const float *d_meas, float angle, float SO,
-                                    float SD, float da, int na, float ai, float db, int nb,
-                                    float bi, int nx, int ny, int nz) {
+extern "C" int mcsart_sart_residual_async(mcsart_ctx *ctx, float *d_rp, float *d_norm,
+                                          const float *d_vol, int motion_pairs, const float *d_meas,
+                                          float angle, float SO, float SD, float da, int na,
+                                          float ai, float db, int nb, float bi, int nx, int ny,
+                                          int nz) {
     const Geom g{nx, ny, nz, na, nb, da, db, ai, bi, SO, SD};
     int rc = validate_geom(g);
     if (rc) return rc;
     if (!d_rp || !d_norm || !d_vol || !d_meas) return fail(MCSART_EINVAL, "null device pointer");
-    GET_CTX();
+    CHECK_CTX();
     const ViewTrig t = view_trig(angle);
     const ViewInfo vi = view_info_of(g, t);
     ProjArgs a{};
@@ -519,44 +611,73 @@ extern "C" int mcsart_sart_residual(float *d_rp, float *d_norm, const float *d_v
                               &nparts);
         if (rc) return rc;
     }
-    return finish(ctx, launch_norm_finish(NormFinish{a.partials, nparts, d_norm}, ctx->stream));
+    return launch_norm_finish(NormFinish{a.partials, nparts, d_norm}, ctx->stream);
+}
+
+extern "C" int mcsart_sart_residual(float *d_rp, float *d_norm, const float *d_vol,
+                                    int motion_pairs, const float *d_meas, float angle, float SO,
+                                    float SD, float da, int na, float ai, float db, int nb,
+                                    float bi, int nx, int ny, int nz) {
+    GET_CTX();
+    return finish(ctx, mcsart_sart_residual_async(ctx, d_rp, d_norm, d_vol, motion_pairs, d_meas,
+                                                  angle, SO, SD, da, na, ai, db, nb, bi, nx, ny,
+                                                  nz));
+}
+
+extern "C" int mcsart_sart_backprojection_async(mcsart_ctx *ctx, float *d_u, const float *d_rp,
+                                                float angle, float SO, float SD, float da, int na,
+                                                float ai, float db, int nb, float bi, int nx,
+                                                int ny, int nz) {
+    const Geom g{nx, ny, nz, na, nb, da, db, ai, bi, SO, SD};
+    int rc = validate_geom(g);
+    if (rc) return rc;
+    if (!d_u || !d_rp) return fail(MCSART_EINVAL, "null device pointer");
+    CHECK_CTX();
+    const ViewTrig t = view_trig(angle);
+    const ViewInfo vi = view_info_of(g, t);
+    TimedRegion tr(ctx, TK_BACK);
+    return launch_backprojector(g, t, vi, BACK_RATIO, nullptr, (const float2 *)d_rp,
+                                            d_u, ctx->stream);
 }
 
 extern "C" int mcsart_sart_backprojection(float *d_u, const float *d_rp, float angle, float SO,
                                           float SD, float da, int na, float ai, float db, int nb,
                                           float bi, int nx, int ny, int nz) {
-    const Geom g{nx, ny, nz, na, nb, da, db, ai, bi, SO, SD};
-    int rc = validate_geom(g);
-    if (rc) return rc;
-    if (!d_u || !d_rp) return fail(MCSART_EINVAL, "null device pointer");
     GET_CTX();
-    const ViewTrig t = view_trig(angle);
-    const ViewInfo vi = view_info_of(g, t);
-    TimedRegion tr(ctx, TK_BACK);
-    return finish(ctx, launch_backprojector(g, t, vi, BACK_RATIO, nullptr, (const float2 *)d_rp,
-                                            d_u, ctx->stream));
+    return finish(ctx, mcsart_sart_backprojection_async(ctx, d_u, d_rp, angle, SO, SD, da, na, ai,
+                                                        db, nb, bi, nx, ny, nz));
 }
 
-extern "C" int mcsart_motion_prep(float *d_pairs, const float *d_img, const float *alpha_x,
-                                  const float *alpha_y, const float *alpha_z,
-                                  const float *beta_x, const float *beta_y, const float *beta_z,
-                                  float vd, float fd, int nx, int ny, int nz) {
+extern "C" int mcsart_motion_prep_async(mcsart_ctx *ctx, float *d_pairs, const float *d_img,
+                                        const float *alpha_x, const float *alpha_y,
+                                        const float *alpha_z, const float *beta_x,
+                                        const float *beta_y, const float *beta_z, float vd,
+                                        float fd, int nx, int ny, int nz) {
     int rc = validate_volume(nx, ny, nz, true);
     if (rc) return rc;
     if (!d_pairs || !d_img || !alpha_x || !alpha_y || !alpha_z || !beta_x || !beta_y || !beta_z)
         return fail(MCSART_EINVAL, "null device pointer");
     if (d_pairs == d_img) return fail(MCSART_EINVAL, "motion_prep output must not alias the image");
-    GET_CTX();
+    CHECK_CTX();
     const float *f[6] = {alpha_x, alpha_y, alpha_z, beta_x, beta_y, beta_z};
     TimedRegion tr(ctx, TK_MOTION);
-    return finish(ctx, launch_motion_prep((float2 *)d_pairs, nullptr, d_img, f, nx, ny, nz, vd, fd,
-                                          ctx->stream));
+    return launch_motion_prep((float2 *)d_pairs, nullptr, d_img, f, nx, ny, nz, vd, fd,
+                                          ctx->stream);
 }
 
-extern "C" int mcsart_motion_update(float *alpha_x, float *alpha_y, float *alpha_z,
-                                    float *beta_x, float *beta_y, float *beta_z, const float *d_u,
-                                    const float *d_img, float wv, float wf, int nx, int ny,
-                                    int nz) {
+extern "C" int mcsart_motion_prep(float *d_pairs, const float *d_img, const float *alpha_x,
+                                  const float *alpha_y, const float *alpha_z, const float *beta_x,
+                                  const float *beta_y, const float *beta_z, float vd, float fd,
+                                  int nx, int ny, int nz) {
+    GET_CTX();
+    return finish(ctx, mcsart_motion_prep_async(ctx, d_pairs, d_img, alpha_x, alpha_y, alpha_z,
+                                                beta_x, beta_y, beta_z, vd, fd, nx, ny, nz));
+}
+
+extern "C" int mcsart_motion_update_async(mcsart_ctx *ctx, float *alpha_x, float *alpha_y,
+                                          float *alpha_z, float *beta_x, float *beta_y,
+                                          float *beta_z, const float *d_u, const float *d_img,
+                                          float wv, float wf, int nx, int ny, int nz) {
     int rc = validate_volume(nx, ny, nz, true);
     if (rc) return rc;
     if (!alpha_x || !alpha_y || !alpha_z || !beta_x || !beta_y || !beta_z || !d_u || !d_img)
@@ -565,7 +686,16 @@ extern "C" int mcsart_motion_update(float *alpha_x, float *alpha_y, float *alpha
     for (int c = 0; c < 6; ++c)
         if (f[c] == d_u || f[c] == d_img)
             return fail(MCSART_EINVAL, "motion_update fields must not alias u or the image");
-    GET_CTX();
+    CHECK_CTX();
     TimedRegion tr(ctx, TK_MOTION);
-    return finish(ctx, launch_motion_update(f, d_u, d_img, nx, ny, nz, wv, wf, ctx->stream));
+    return launch_motion_update(f, d_u, d_img, nx, ny, nz, wv, wf, ctx->stream);
+}
+
+extern "C" int mcsart_motion_update(float *alpha_x, float *alpha_y, float *alpha_z, float *beta_x,
+                                    float *beta_y, float *beta_z, const float *d_u,
+                                    const float *d_img, float wv, float wf, int nx, int ny,
+                                    int nz) {
+    GET_CTX();
+    return finish(ctx, mcsart_motion_update_async(ctx, alpha_x, alpha_y, alpha_z, beta_x, beta_y,
+                                                  beta_z, d_u, d_img, wv, wf, nx, ny, nz));
 }

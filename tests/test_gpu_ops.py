@@ -217,3 +217,42 @@ def test_invalid_arguments(gpu_lib):
         host_diff(x, y, 8, 6, 4, 7)                  # bad ind
     with pytest.raises(McsartError):
         host_add(x, y, 0, 6, 4, 1.0)                 # empty volume
+
+
+def test_async_operators_on_two_contexts(gpu_lib):
+    """The _async operators enqueue on a context's stream and give exactly the synchronous
+    operators' results; two contexts run the same chain side by side (scratch is per context:
+    the in-place warp and the transposed copy of the projector each use it)."""
+    from mcsart_amd.driver import Context
+    spec = configs.c1_static64()
+    g = spec.geometry
+    vol = smooth_random(g.vol_shape, 3)
+    fields = random_fields(g.vol_shape, 9)
+    ang = np.float32(0.3)
+    geo = (ang, g.SO, g.SD, g.da, g.na, g.ai, g.db, g.nb, g.bi, g.nx, g.ny, g.nz)
+    dF = [DeviceArray.from_host(f) for f in fields]
+    # synchronous chain: warp in place, project, backproject
+    dx = DeviceArray.from_host(vol)
+    host_deform(dx, dx, g.nx, g.ny, g.nz, 0.7, 0.2, *dF)
+    dp = DeviceArray.empty(g.det_shape)
+    gpu_lib.mcsart_projection(dp.ptr, dx.ptr, *geo)
+    db = DeviceArray.empty(g.vol_shape)
+    gpu_lib.mcsart_backprojection(db.ptr, dp.ptr, *geo)
+    want = (dx.to_host(), dp.to_host(), db.to_host())
+    ctxs = [Context(0), Context(0)]
+    bufs = []
+    for c in ctxs:   # enqueue both chains before waiting on either
+        x = DeviceArray.from_host(vol)
+        p = DeviceArray.empty(g.det_shape)
+        b = DeviceArray.empty(g.vol_shape)
+        assert gpu_lib.mcsart_deform_async(c.ptr, x.ptr, x.ptr, g.nx, g.ny, g.nz, 0.7, 0.2,
+                                           *[f.ptr for f in dF]) == 0
+        assert gpu_lib.mcsart_projection_async(c.ptr, p.ptr, x.ptr, *geo) == 0
+        assert gpu_lib.mcsart_backprojection_async(c.ptr, b.ptr, p.ptr, *geo) == 0
+        bufs.append((x, p, b))
+    for c, (x, p, b) in zip(ctxs, bufs):
+        c.synchronize()
+        for got, ref in zip((x.to_host(), p.to_host(), b.to_host()), want):
+            assert np.array_equal(got, ref)
+        c.close()
+    assert gpu_lib.mcsart_projection_async(None, dp.ptr, dx.ptr, *geo) != 0

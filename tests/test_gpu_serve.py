@@ -1,0 +1,130 @@
+"""The reconstruction server: continuous batching of uneven scans on one GPU.
+
+A request's results are the same alone and in any mix (bit for bit with fixed launch shapes:
+the same kernels run in the same order on the request's own stream), the HBM budget bounds what
+runs at once, and requests that cannot fit are refused at submit.
+"""
+import numpy as np
+import pytest
+
+from mcsart_amd._lib import McsartError
+from mcsart_amd.configs import ScanSpec, make_scan_inputs
+from mcsart_amd.driver import Context, Job, job_bytes
+from mcsart_amd.serve import Server
+from mcsart_amd.simulate import measure
+from _util import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+SPECS = [
+    ScanSpec("s32", 32, 32, 32, 4.0, 4.0, 48, 40, 4.0, 24, True, [1, 0, 1], seed=71),
+    ScanSpec("s40", 40, 40, 24, 3.0, 4.0, 64, 32, 3.0, 30, True, [1, 0, 1, 0], seed=72),
+    ScanSpec("s24", 24, 24, 32, 4.0, 3.0, 40, 48, 4.5, 20, False, [1, 1, 0], seed=73),
+    ScanSpec("s48", 48, 48, 16, 2.5, 5.0, 72, 24, 2.5, 16, True, [0, 1, 1], seed=74),
+]
+
+
+@pytest.fixture(scope="module")
+def scans(gpu_lib):
+    out = []
+    for s in SPECS:
+        d = make_scan_inputs(s)
+        out.append((s, d, measure(s, d, s.seed + 5)))
+    return out
+
+
+def _submit(srv, scan, want_fields=False):
+    s, d, p = scan
+    return srv.submit(s.geometry, s.angles, d.alpha, d.beta, s.op_iter, p, fields=d.fields,
+                      lam=s.lam, motion=s.motion, nonneg=s.nonneg, want_fields=want_fields)
+
+
+def _alone(scans):
+    res = []
+    with Server(slots=1) as srv:
+        for sc in scans:
+            t = srv.wait(_submit(srv, sc, want_fields=True))
+            res.append((t.image.copy(), t.norms.copy(), [f.copy() for f in t.fields]))
+    return res
+
+
+def test_alone_equals_plain_job(scans):
+    """One request through the server equals the same scan run as a plain job."""
+    alone = _alone(scans[:2])
+    ctx = Context(0)
+    for (s, d, p), (img, nrm, flds) in zip(scans[:2], alone):
+        job = Job(ctx, s.geometry, s.angles, d.alpha, d.beta, s.op_iter, lam=s.lam,
+                  motion=s.motion, nonneg=s.nonneg)
+        job.set_projections(p)
+        job.set_image(None)
+        job.set_fields(d.fields)
+        job.run()
+        assert np.array_equal(job.image(), img)
+        assert np.array_equal(job.norms(), nrm)
+        for a, b in zip(job.fields(), flds):
+            assert np.array_equal(a, b)
+        job.close()
+    ctx.close()
+
+
+def test_mix_equals_alone(scans):
+    alone = _alone(scans)
+    order = [0, 1, 2, 3, 2, 0, 3, 1]   # duplicates: the same scan twice in flight
+    with Server(slots=3) as srv:
+        tickets = [_submit(srv, scans[k], want_fields=True) for k in order]
+        srv.drain()
+        infos = [srv.info(t) for t in tickets]
+        for k, t, inf in zip(order, tickets, infos):
+            assert inf["state"] == "done", inf
+            img, nrm, flds = alone[k]
+            assert np.array_equal(t.image, img), f"scan {SPECS[k].name} differs in the mix"
+            assert np.array_equal(t.norms, nrm)
+            for a, b in zip(t.fields, flds):
+                assert np.array_equal(a, b)
+    # the three slots really ran jobs side by side
+    iv = [(i["queue_ms"], i["queue_ms"] + i["run_ms"]) for i in infos]
+    assert max(sum(1 for a, b in iv if a <= t0 < b) for t0, _ in iv) >= 2
+
+
+def test_budget_bounds_concurrency(scans):
+    s, d, p = scans[0]
+    nb = job_bytes(s.geometry, s.n_views, len(s.op_iter))
+    with Server(slots=4, hbm_budget=int(2.5 * nb)) as srv:
+        tickets = [_submit(srv, scans[0]) for _ in range(5)]
+        srv.drain()
+        infos = [srv.info(t) for t in tickets]
+        used, budget = srv.memory()
+        assert used == 0 and budget == int(2.5 * nb)
+        ref = tickets[0].image
+        for t in tickets[1:]:
+            assert np.array_equal(t.image, ref)
+    iv = [(i["queue_ms"], i["queue_ms"] + i["run_ms"]) for i in infos]
+    overlap = max(sum(1 for a, b in iv if a <= t0 < b) for t0, _ in iv)
+    assert overlap <= 2, f"{overlap} jobs ran at once under a budget for 2"
+    assert all(i["bytes"] == nb for i in infos)
+
+
+def test_oversize_and_invalid_rejected(scans):
+    s, d, p = scans[0]
+    nb = job_bytes(s.geometry, s.n_views, len(s.op_iter))
+    with Server(slots=2, hbm_budget=nb - 1) as srv:
+        with pytest.raises(McsartError):
+            _submit(srv, scans[0])
+    with Server(slots=1) as srv:
+        with pytest.raises(McsartError):   # nx != ny
+            bad = ScanSpec("bad", 32, 30, 16, 4.0, 4.0, 48, 40, 4.0, 8, False, [1], seed=1)
+            srv.submit(bad.geometry, bad.angles, np.zeros(8), np.zeros(8), [1],
+                       np.zeros((8, 40, 48), np.float32))
+        t = srv.wait(_submit(srv, scans[2]))   # the server keeps serving
+        assert srv.info(t)["state"] == "done"
+
+
+def test_autotuned_requests_match(scans):
+    alone = _alone(scans[:2])
+    with Server(slots=2, autotune=True) as srv:
+        tickets = [_submit(srv, sc) for sc in scans[:2]]
+        srv.drain()
+        for t, (img, nrm, _) in zip(tickets, alone):
+            inf = srv.info(t)
+            assert inf["tuning"]["proj_split"] in (1, 2, 4, 8)
+            assert rel_l2(t.image, img) <= 1e-5

@@ -112,3 +112,39 @@ def test_batch_mix_is_seeded():
     assert [(j.nx, j.nz, j.na, j.n_views) for j in j1] == [(j.nx, j.nz, j.na, j.n_views) for j in j2]
     assert len(j1) == 16 and all(200 <= j.n_views <= 900 for j in j1)
     assert len({(j.nx, j.nz) for j in j1}) >= 2
+
+
+def test_ctypes_structs_match_header(tmp_path):
+    """The Python mirrors of the header's structs have the C layout (sizes and field offsets
+    printed by a program compiled against include/mcsart_hip.h)."""
+    import ctypes
+    import shutil
+    import subprocess
+    from mcsart_amd.driver import CGeom, COpts, CTuning
+    from mcsart_amd.serve import CRequest, CRequestInfo, CServerOpts
+    cc = shutil.which("gcc") or shutil.which("cc")
+    if cc is None:
+        pytest.skip("no C compiler")
+    structs = [(CGeom, "mcsart_geom"), (COpts, "mcsart_opts"), (CTuning, "mcsart_tuning"),
+               (CServerOpts, "mcsart_server_opts"), (CRequest, "mcsart_request"),
+               (CRequestInfo, "mcsart_request_info")]
+    c_names = {"lam": "lambda"}
+    lines = []
+    for py, cname in structs:
+        lines.append(f'printf("%zu\\n", sizeof({cname}));')
+        for f, _ in py._fields_:
+            lines.append(f'printf("%zu\\n", offsetof({cname}, {c_names.get(f, f)}));')
+    src = tmp_path / "layout.c"
+    src.write_text("#include <stdio.h>\n#include <stddef.h>\n#include \"mcsart_hip.h\"\n"
+                   "int main(void) {\n" + "\n".join(lines) + "\nreturn 0;\n}\n")
+    exe = tmp_path / "layout"
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    subprocess.run([cc, "-I", os.path.join(root, "include"), str(src), "-o", str(exe)],
+                   check=True)
+    got = [int(x) for x in subprocess.run([str(exe)], capture_output=True, text=True,
+                                          check=True).stdout.split()]
+    want = []
+    for py, _ in structs:
+        want.append(ctypes.sizeof(py))
+        want.extend(getattr(py, f).offset for f, _ in py._fields_)
+    assert got == want
