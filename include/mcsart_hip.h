@@ -290,9 +290,10 @@ int mcsart_job_destroy(mcsart_job *job);
 /* Launch shapes of the two footprint kernels (the reference has fixed 16x16 blocks,
  * src/cu_projection.cu:140, src/cu_backprojection.cu:186).  Zero fields mean automatic. */
 typedef struct {
-    int proj_split;   /* projector workgroup slices along each ray: 0, 1, 2, 4 or 8 */
+    int proj_split;   /* projector workgroup slices along each ray: 0, 1, 2, 4, 8 or 16 */
     int back_zslab;   /* backprojector voxels per lane along z: 0 or 1..64 */
     int back_stage;   /* 1: backprojector workgroups stage their detector rectangle in LDS */
+    int proj_rows;    /* detector rows (rays of one column) per projector lane: 0, 1, 2 or 4 */
 } mcsart_tuning;
 int mcsart_job_set_tuning(mcsart_job *job, const mcsart_tuning *t);
 int mcsart_job_get_tuning(mcsart_job *job, mcsart_tuning *t);
@@ -316,7 +317,7 @@ typedef struct {
     int slots;           /* jobs in flight at once, one stream each: 1..16 */
     size_t hbm_budget;   /* device bytes admitted jobs may hold together; 0: 90% of free memory */
     int autotune;        /* 1: autotune each job's launch shapes after admission */
-    int proj_split, back_zslab, back_stage;   /* fixed launch shapes when autotune == 0 */
+    int proj_split, back_zslab, back_stage, proj_rows;   /* fixed shapes when autotune == 0 */
 } mcsart_server_opts;
 
 /* All host buffers must stay valid until the request has completed (mcsart_server_wait). */

@@ -47,7 +47,7 @@ struct mcsart_job {
     };
     std::vector<SweepGraph> graphs;
     bool use_graphs = true;
-    int proj_split = 0;         // projector launch shape (0 = automatic)
+    int proj_split = 0, proj_rows = 0;   // projector launch shape (0 = automatic)
     BackTune back_tune{0, 0};   // backprojector launch shape (0 = automatic)
 };
 
@@ -129,7 +129,8 @@ int image_sweep(mcsart_job *j, int it) {
             a.resid_copy = j->d_resid ? j->d_resid + D * iv : nullptr;
             a.partials = j->d_partials;
             TimedRegion tr(j->ctx, TK_PROJ);
-            rc = launch_projector(g, j->trig[iv], vi, PROJ_IMAGE, a, st, &nparts, j->proj_split);
+            rc = launch_projector(g, j->trig[iv], vi, PROJ_IMAGE, a, st, &nparts, j->proj_split,
+                                  j->proj_rows);
             if (rc) return rc;
         }
         {   // u = B(r) / B(P1)  (:212-218); also rho_i from the projector's partials
@@ -156,26 +157,29 @@ int image_sweep(mcsart_job *j, int it) {
 }
 
 // Motion (udvf) sweep (src/AUMISART.cu:236-288; static overload :66-118 steps views by 10).
+// Per view: prep (g, q) -> projector -> backprojector -> field update.  The update of one view
+// and the prep of the next run as one kernel (launch_motion_step), which reads the fields once.
 int motion_sweep(mcsart_job *j, int it) {
     const Geom &g = j->g;
     const long long D = g.ndet(), N = g.nvox();
     hipStream_t st = j->ctx->stream;
     float *fl[6];
     for (int c = 0; c < 6; ++c) fl[c] = j->d_fields + c * N;
-    const float *cfl[6] = {fl[0], fl[1], fl[2], fl[3], fl[4], fl[5]};
     const int stride = j->opts.motion ? 1 : 10;
+    float2 *pair = (float2 *)j->d_w1, *pairT = (float2 *)j->d_w1T;
+    auto vdiff = [&](int iv) { return j->volumes[iv] - j->volumes[iv - 1]; };   // :249
+    auto fdiff = [&](int iv) { return j->flows[iv] - j->flows[iv - 1]; };       // :250
     int rc, nparts = 0;
+    if (j->V > 1) {   // prep of the first view
+        const ViewInfo &vi = j->vinfo[1];
+        TimedRegion tr(j->ctx, TK_MOTION);
+        rc = launch_motion_step(fl, nullptr, j->d_img, vi.has_y ? pair : nullptr,
+                                vi.has_x ? pairT : nullptr, g.nx, g.ny, g.nz, 0.f, 0.f, vdiff(1),
+                                fdiff(1), false, true, st);
+        if (rc) return rc;
+    }
     for (int iv = 1; iv < j->V; iv += stride) {
-        const float vd = j->volumes[iv] - j->volumes[iv - 1];   // :249
-        const float fd = j->flows[iv] - j->flows[iv - 1];       // :250
         const ViewInfo &vi = j->vinfo[iv];
-        float2 *pair = (float2 *)j->d_w1, *pairT = (float2 *)j->d_w1T;
-        {
-            TimedRegion tr(j->ctx, TK_MOTION);
-            rc = launch_motion_prep(vi.has_y ? pair : nullptr, vi.has_x ? pairT : nullptr,
-                                    j->d_img, cfl, g.nx, g.ny, g.nz, vd, fd, st);
-            if (rc) return rc;
-        }
         {   // r = p_i - P(x - img0), P(q), rho_i  (:246-263)
             ProjArgs a{};
             a.pair = pair;
@@ -184,7 +188,8 @@ int motion_sweep(mcsart_job *j, int it) {
             a.rp = j->d_rp;
             a.partials = j->d_partials;
             TimedRegion tr(j->ctx, TK_PROJ);
-            rc = launch_projector(g, j->trig[iv], vi, PROJ_MOTION, a, st, &nparts, j->proj_split);
+            rc = launch_projector(g, j->trig[iv], vi, PROJ_MOTION, a, st, &nparts, j->proj_split,
+                                  j->proj_rows);
             if (rc) return rc;
         }
         {   // u = B(r) / B(P q)  (:265-270); also rho_i
@@ -194,11 +199,16 @@ int motion_sweep(mcsart_job *j, int it) {
                                       st, &nf, &j->back_tune);
             if (rc) return rc;
         }
-        {   // fields += u^T * (-vd | -fd) * D_c x  (:272-277)
+        {   // fields += u^T * (-vd | -fd) * D_c x (:272-277), then the next view's prep
+            const int nv = iv + stride;
+            const bool more = nv < j->V;
+            const ViewInfo &vn = j->vinfo[more ? nv : iv];
             TimedRegion tr(j->ctx, TK_MOTION);
-            rc = launch_motion_update(fl, j->d_w2, j->d_img, g.nx, g.ny, g.nz,
-                                      j->volumes[iv - 1] - j->volumes[iv],
-                                      j->flows[iv - 1] - j->flows[iv], st);
+            rc = launch_motion_step(fl, j->d_w2, j->d_img, more && vn.has_y ? pair : nullptr,
+                                    more && vn.has_x ? pairT : nullptr, g.nx, g.ny, g.nz,
+                                    j->volumes[iv - 1] - j->volumes[iv],
+                                    j->flows[iv - 1] - j->flows[iv], more ? vdiff(nv) : 0.f,
+                                    more ? fdiff(nv) : 0.f, true, more, st);
             if (rc) return rc;
         }
     }
@@ -459,8 +469,10 @@ extern "C" int mcsart_job_device_image(mcsart_job *j, float **d_img) {
 namespace {
 
 int check_tuning(const mcsart_tuning &t) {
-    if (t.proj_split < 0 || t.proj_split > 8 || (t.proj_split & (t.proj_split - 1)))
-        return fail(MCSART_EINVAL, "proj_split must be 0, 1, 2, 4 or 8");
+    if (t.proj_split < 0 || t.proj_split > 16 || (t.proj_split & (t.proj_split - 1)))
+        return fail(MCSART_EINVAL, "proj_split must be 0, 1, 2, 4, 8 or 16");
+    if (t.proj_rows != 0 && t.proj_rows != 1 && t.proj_rows != 2 && t.proj_rows != 4)
+        return fail(MCSART_EINVAL, "proj_rows must be 0, 1, 2 or 4");
     if (t.back_zslab < 0 || t.back_zslab > 64)
         return fail(MCSART_EINVAL, "back_zslab must be in 0..64");
     if (t.back_stage != 0 && t.back_stage != 1)
@@ -469,10 +481,11 @@ int check_tuning(const mcsart_tuning &t) {
 }
 
 void apply_tuning(mcsart_job *j, const mcsart_tuning &t) {
-    if (j->proj_split != t.proj_split || j->back_tune.zslab != t.back_zslab ||
-        j->back_tune.stage != t.back_stage)
+    if (j->proj_split != t.proj_split || j->proj_rows != t.proj_rows ||
+        j->back_tune.zslab != t.back_zslab || j->back_tune.stage != t.back_stage)
         for (auto &sg : j->graphs) drop_graph(sg);   // captured launches carry the old shapes
     j->proj_split = t.proj_split;
+    j->proj_rows = t.proj_rows;
     j->back_tune = BackTune{t.back_zslab, t.back_stage};
 }
 
@@ -513,7 +526,7 @@ extern "C" int mcsart_job_get_tuning(mcsart_job *j, mcsart_tuning *t) {
     int rc = check_job(j);
     if (rc) return rc;
     if (!t) return fail(MCSART_EINVAL, "null tuning");
-    *t = mcsart_tuning{j->proj_split, j->back_tune.zslab, j->back_tune.stage};
+    *t = mcsart_tuning{j->proj_split, j->back_tune.zslab, j->back_tune.stage, j->proj_rows};
     return MCSART_OK;
 }
 
@@ -534,7 +547,7 @@ extern "C" int mcsart_job_autotune(mcsart_job *j, mcsart_tuning *chosen) {
     MCSART_HIP_TRY(hipStreamSynchronize(st));
     rc = launch_transpose(j->d_w1T, j->d_img, g.nx, g.ny, g.nz, st);
     if (rc) return rc;
-    auto project = [&](int split) {
+    auto project = [&](int split, int rows) {
         for (int iv : views) {
             ProjArgs a{};
             a.vol0 = j->d_img;
@@ -544,7 +557,7 @@ extern "C" int mcsart_job_autotune(mcsart_job *j, mcsart_tuning *chosen) {
             a.partials = j->d_partials;
             int np = 0;
             const int r = launch_projector(g, j->trig[iv], j->vinfo[iv], PROJ_IMAGE, a, st, &np,
-                                           split);
+                                           split, rows);
             if (r) return r;
         }
         return (int)MCSART_OK;
@@ -558,21 +571,23 @@ extern "C" int mcsart_job_autotune(mcsart_job *j, mcsart_tuning *chosen) {
         return (int)MCSART_OK;
     };
     const int reps = 3;
-    mcsart_tuning best{0, 0, 0};
+    mcsart_tuning best{0, 0, 0, 0};
     float best_ms = 3.4e38f;
     const int np = std::max(g.nx, g.ny);
-    for (int split = 1; split <= 8; split *= 2) {
-        if (split > 1 && np / split < 8) break;
-        rc = project(split);   // warm-up (code object load, caches)
-        if (rc) return rc;
-        float ms = 0.f;
-        rc = time_on_stream(j, reps, [&] { return project(split); }, &ms);
-        if (rc) return rc;
-        if (ms < best_ms) {
-            best_ms = ms;
-            best.proj_split = split;
+    for (int rows = 1; rows <= 4; rows *= 2)
+        for (int split = 1; split <= 16; split *= 2) {
+            if (split > 1 && np / split < 4) break;
+            rc = project(split, rows);   // warm-up (code object load, caches)
+            if (rc) return rc;
+            float ms = 0.f;
+            rc = time_on_stream(j, reps, [&] { return project(split, rows); }, &ms);
+            if (rc) return rc;
+            if (ms < best_ms) {
+                best_ms = ms;
+                best.proj_split = split;
+                best.proj_rows = rows;
+            }
         }
-    }
     best_ms = 3.4e38f;
     for (int stage = 0; stage <= 1; ++stage)
         for (int zs = 1; zs <= 16 && zs <= g.nz; zs *= 2) {

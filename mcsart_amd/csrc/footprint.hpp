@@ -76,6 +76,81 @@ __host__ __device__ inline void ray_setup(const GeomArgs &g, double c, double s,
     R.KzB = (float)(rz1 * dist + zbase);
 }
 
+// The same constants split by what they depend on: the transverse lines depend on the detector
+// column ia only (the ray's xy direction), the axial lines on (ia, ib).  Lines are ordered for the
+// planes in front of the source (xc of the sign of x21 / y21): there t_lo = rlo*p + Klo is the
+// lower bound and t_hi the upper one; z lines zl/x21 < zr/x21 keep their order there.  Planes
+// behind the source (the reference evaluates every plane) lie outside the volume, where the
+// bounds' order does not matter.
+struct RayT {
+    int xbranch;
+    float rlo, Klo, rhi, Khi;
+};
+struct RayZ {
+    float rlo, Klo, rhi, Khi;
+};
+
+struct RayDir {
+    double x21, y21, dist, s_, c_;
+    int xbranch;
+};
+
+__host__ __device__ inline RayDir ray_dir(const GeomArgs &g, double c, double s, int ia) {
+    const double SO = g.SO, SD = g.SD, da = g.da;
+    const double x1 = -SO * c, y1 = -SO * s;                       // :25-26
+    const double x20 = SD - SO;                                     // :28
+    const double y20 = ((double)ia + (double)g.ai) * da;            // :29
+    const double x2 = x20 * c - y20 * s, y2 = x20 * s + y20 * c;    // :30-31
+    RayDir d;
+    d.x21 = x2 - x1;                                                // :34
+    d.y21 = y2 - y1;                                                // :35
+    d.xbranch = fabs(d.x21) > fabs(d.y21) ? 1 : 0;                  // :39
+    d.dist = d.xbranch ? SO * c : SO * s;
+    d.c_ = c;
+    d.s_ = s;
+    return d;
+}
+
+__host__ __device__ inline void ray_t_setup(const GeomArgs &g, const RayDir &d, RayT &R) {
+    const double SO = g.SO, hda = g.da / 2, c = d.c_, s = d.s_;
+    const double xl = d.x21 - hda * s, xr = d.x21 + hda * s;        // :48-49
+    const double yl = d.y21 - hda * c, yr = d.y21 + hda * c;        // :50-51
+    double r0, r1, r2, r3, base, fwd;
+    if (d.xbranch) {
+        r0 = yl / xl; r1 = yl / xr; r2 = yr / xl; r3 = yr / xr;     // :54-61
+        base = -SO * s + (double)(g.ny / 2);                         // y1 + integer ny/2
+        fwd = d.x21;
+    } else {
+        r0 = xl / yl; r1 = xl / yr; r2 = xr / yl; r3 = xr / yr;     // :107-114
+        base = -SO * c + (double)(g.nx / 2);                         // x1 + integer nx/2
+        fwd = d.y21;
+    }
+    double rmin = fmin(fmin(r0, r1), fmin(r2, r3));
+    double rmax = fmax(fmax(r0, r1), fmax(r2, r3));
+    if (fwd < 0) {   // xc < 0 in front of the source: the larger slope gives the lower bound
+        const double t = rmin;
+        rmin = rmax;
+        rmax = t;
+    }
+    R.xbranch = d.xbranch;
+    R.rlo = (float)rmin;
+    R.Klo = (float)(rmin * d.dist + base);
+    R.rhi = (float)rmax;
+    R.Khi = (float)(rmax * d.dist + base);
+}
+
+__host__ __device__ inline void ray_z_setup(const GeomArgs &g, const RayDir &d, int ib, RayZ &Z) {
+    const double z21 = ((double)ib + (double)g.bi) * (double)g.db;   // :32,:36
+    const double hdb = (double)g.db / 2;
+    const double den = d.xbranch ? d.x21 : d.y21;
+    const double rz0 = (z21 - hdb) / den, rz1 = (z21 + hdb) / den;   // :67-74, :120-127
+    const double zbase = (double)(g.nz / 2);                          // integer nz/2
+    Z.rlo = (float)rz0;
+    Z.Klo = (float)(rz0 * d.dist + zbase);
+    Z.rhi = (float)rz1;
+    Z.Khi = (float)(rz1 * d.dist + zbase);
+}
+
 // ------------------------------------------------------------------------------------------
 // Backprojector column (src/cu_backprojection.cu:62-138).
 //

@@ -77,9 +77,11 @@ struct ProjArgs {
 // (sum p^2, sum r^2) pair per workgroup in a.partials and report their count in *nparts; a later
 // kernel on the same stream turns them into ||r||/||p|| (NormFinish).  No fence or atomic is
 // needed: the kernel boundary orders the stores.
-// split: workgroup slices along the ray (power of two up to 8; 0 = automatic).
+// split: workgroup slices along the ray (power of two up to 16; 0 = automatic); rows: detector
+// rows walked by one lane (1, 2 or 4; 0 = automatic).
 int launch_projector(const Geom &g, const ViewTrig &t, const ViewInfo &vi, ProjMode mode,
-                     const ProjArgs &a, hipStream_t st, int *nparts = nullptr, int split = 0);
+                     const ProjArgs &a, hipStream_t st, int *nparts = nullptr, int split = 0,
+                     int rows = 0);
 ViewInfo view_info_of(const Geom &g, const ViewTrig &t);
 int projector_partials(const Geom &g);   // number of per-block partial slots needed
 
@@ -163,6 +165,11 @@ int launch_motion_prep(float2 *pair, float2 *pairT, const float *x, const float 
 //   alpha_c += u^T * wv * D_c x,  beta_c += u^T * wf * D_c x
 int launch_motion_update(float *const fields[6], const float *u, const float *x, int nx, int ny,
                          int nz, float wv, float wf, hipStream_t st);
+// Both in one pass: the update of one view (u, wv, wf) followed by the prep of the next view
+// (vd, fd) on the updated fields; either half may be switched off.
+int launch_motion_step(float *const fields[6], const float *u, const float *x, float2 *pair,
+                       float2 *pairT, int nx, int ny, int nz, float wv, float wf, float vd,
+                       float fd, bool update, bool prep, hipStream_t st);
 
 // ---- contexts -------------------------------------------------------------------------------
 enum TimingKind { TK_PROJ = 0, TK_BACK = 1, TK_WARP = 2, TK_MOTION = 3, TK_COUNT = 4 };

@@ -1,9 +1,10 @@
 // Warps, elementwise operators and the fused motion-sweep kernels for gfx950.
 //
-// Volume kernels use 256-lane workgroups shaped 64 x 4 over (dim 1, dim 2) with one z-plane per
-// grid row, so every wavefront reads and writes 64 contiguous floats.  Kernels that read a field
-// at the xy-transposed index (src/cu_add.cu:58) stage 32 x 32 tiles through LDS so that both the
-// transposed read and the natural write are coalesced.
+// Simple volume kernels use 256-lane workgroups shaped 64 x 4 over (dim 1, dim 2) with one z-plane
+// per grid row, so every wavefront reads and writes 64 contiguous floats.  The warps and the
+// motion-sweep kernels work on 32 x 32 tiles (below); kernels that read a field at the
+// xy-transposed index (src/cu_add.cu:58) stage the tile through LDS so that both the transposed
+// read and the natural write are coalesced.
 #include "internal.hpp"
 
 namespace mcsart {
@@ -12,62 +13,6 @@ namespace {
 constexpr int EX = 64, EY = 4;
 
 __device__ inline int clampi_d(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
-
-// Clamp-to-edge trilinear sample of img at (i0 + m0, i1 + m1, i2 + m2), the value the
-// reference's texture fetch returns at unnormalised coordinate (i + 1 + m - 0.5)
-// (src/cu_deform.cu:86-89), with exact FP32 weights.  The integer part of each displacement is
-// split off first, so the fraction keeps full precision for any index size.
-__device__ inline float trilinear(const float *__restrict__ img, int n0, int n1, int n2, int i0,
-                                  float m0, int i1, float m1, int i2, float m2) {
-    // A displacement larger than the volume lands outside it on the same side, where clamp-to-edge
-    // addressing returns the edge voxel whatever its size: clamping it to +-(n+1) changes no
-    // result and keeps the float->int conversions in range (a NaN displacement becomes -(n+1)).
-    m0 = fminf(fmaxf(m0, -(float)(n0 + 1)), (float)(n0 + 1));
-    m1 = fminf(fmaxf(m1, -(float)(n1 + 1)), (float)(n1 + 1));
-    m2 = fminf(fmaxf(m2, -(float)(n2 + 1)), (float)(n2 + 1));
-    const float f0 = floorf(m0), f1 = floorf(m1), f2 = floorf(m2);
-    const float t0 = m0 - f0, t1 = m1 - f1, t2 = m2 - f2;
-    const int b0 = i0 + (int)f0, b1 = i1 + (int)f1, b2 = i2 + (int)f2;
-    const int a0 = clampi_d(b0, 0, n0 - 1), c0 = clampi_d(b0 + 1, 0, n0 - 1);
-    const int a1 = clampi_d(b1, 0, n1 - 1), c1 = clampi_d(b1 + 1, 0, n1 - 1);
-    const int a2 = clampi_d(b2, 0, n2 - 1), c2 = clampi_d(b2 + 1, 0, n2 - 1);
-    const long long s1 = n0, s2 = (long long)n0 * n1;
-    const long long r00 = a1 * s1 + a2 * s2, r10 = c1 * s1 + a2 * s2;
-    const long long r01 = a1 * s1 + c2 * s2, r11 = c1 * s1 + c2 * s2;
-    const float u0 = 1.f - t0, u1 = 1.f - t1, u2 = 1.f - t2;
-    const float c00 = img[r00 + a0] * u0 + img[r00 + c0] * t0;
-    const float c10 = img[r10 + a0] * u0 + img[r10 + c0] * t0;
-    const float c01 = img[r01 + a0] * u0 + img[r01 + c0] * t0;
-    const float c11 = img[r11 + a0] * u0 + img[r11 + c0] * t0;
-    const float e0 = c00 * u1 + c10 * t1;
-    const float e1 = c01 * u1 + c11 * t1;
-    return e0 * u2 + e1 * t2;
-}
-
-// out[id] = W(src)[id]: DVF m_c = alpha_c*vol + beta_c*flow formed on the fly
-// (kernel_forwardDVF, src/cu_deform.cu:66-77), sample at (i0 + my, i1 + mx, i2 + mz) (:85-89).
-template <bool ADD>
-__global__ void __launch_bounds__(EX *EY) k_warp(float *__restrict__ out,
-                                                const float *__restrict__ src, int nx, int ny,
-                                                int nz, float vol, float flow, Fields f,
-                                                float lambda, int nonneg) {
-    const int i0 = blockIdx.x * EX + threadIdx.x;
-    const int i1 = blockIdx.y * EY + threadIdx.y;
-    const int i2 = blockIdx.z;
-    if (i0 >= nx || i1 >= ny) return;
-    const long long id = i0 + (long long)i1 * nx + (long long)i2 * nx * ny;
-    const float mx = f.ax[id] * vol + f.bx[id] * flow;
-    const float my = f.ay[id] * vol + f.by[id] * flow;
-    const float mz = f.az[id] * vol + f.bz[id] * flow;
-    const float v = trilinear(src, nx, ny, nz, i0, my, i1, mx, i2, mz);
-    if (ADD) {   // x += lambda * W(u)  (src/AUMISART.cu:222-223), optional clamp
-        float x = out[id] + v * lambda;
-        if (nonneg) x = fmaxf(x, 0.f);
-        out[id] = x;
-    } else {
-        out[id] = v;
-    }
-}
 
 // kernel_deformation2 (src/cu_deform.cu:92-113) with its swapped boundary checks; the linear
 // index is id = iy + ix*ny + iz*nx*ny, i.e. iy is the dim-1 index and ix the dim-2 index.
@@ -189,99 +134,102 @@ __global__ void k_division(float *a, const float *b, long long n) {
     a[i] = v;
 }
 
-// ---- fused motion sweep ----------------------------------------------------------------------
-constexpr int TT = 32, TR = 8;   // 32x32 tile, 32x8 lanes, 4 rows per lane
+// ---- tile kernels: warps and the fused motion sweep ------------------------------------------
+//
+// A workgroup of 32 x 8 lanes owns a 32 x 32 tile of one z-plane; lane (tx, ty) owns the four
+// voxels (X0 + tx, Y0 + ty + 8k), k = 0..3.  Each kernel first issues every load its lane needs
+// (all four voxels), then computes, then stores: nothing it writes can alias what it still has to
+// read, so the loads of all four voxels are in flight together.  Volume indices are 32-bit
+// (volumes are limited to < 2^31 voxels at the C ABI).  Transposed reads and writes go through an
+// LDS tile (+1 padding: conflict-free columns) so that global accesses stay coalesced.
+constexpr int TT = 32, TR = 8, TK = TT / TR;
 
-// Load the xy-transposed 32x32 tile of f at plane i2 into t: afterwards t[j][i] holds
-// f[dim1 = Y0 + i, dim2 = X0 + j], so lane (i0 = X0+tx, i1 = Y0+r) finds f^T at t[tx][r].
+// Clamp-to-edge trilinear sample of img at (i0 + m0, i1 + m1, i2 + m2), the value the
+// reference's texture fetch returns at unnormalised coordinate (i + 1 + m - 0.5)
+// (src/cu_deform.cu:86-89), with exact FP32 weights.  The integer part of each displacement is
+// split off first, so the fraction keeps full precision for any index size.
+__device__ inline float trilinear(const float *__restrict__ img, int n0, int n1, int n2, int i0,
+                                  float m0, int i1, float m1, int i2, float m2) {
+    // A displacement larger than the volume lands outside it on the same side, where clamp-to-edge
+    // addressing returns the edge voxel whatever its size: clamping it to +-(n+1) changes no
+    // result and keeps the float->int conversions in range (a NaN displacement becomes -(n+1)).
+    m0 = fminf(fmaxf(m0, -(float)(n0 + 1)), (float)(n0 + 1));
+    m1 = fminf(fmaxf(m1, -(float)(n1 + 1)), (float)(n1 + 1));
+    m2 = fminf(fmaxf(m2, -(float)(n2 + 1)), (float)(n2 + 1));
+    const float f0 = floorf(m0), f1 = floorf(m1), f2 = floorf(m2);
+    const float t0 = m0 - f0, t1 = m1 - f1, t2 = m2 - f2;
+    const int b0 = i0 + (int)f0, b1 = i1 + (int)f1, b2 = i2 + (int)f2;
+    const int a0 = clampi_d(b0, 0, n0 - 1), c0 = clampi_d(b0 + 1, 0, n0 - 1);
+    const int a1 = clampi_d(b1, 0, n1 - 1), c1 = clampi_d(b1 + 1, 0, n1 - 1);
+    const int a2 = clampi_d(b2, 0, n2 - 1), c2 = clampi_d(b2 + 1, 0, n2 - 1);
+    const int s2 = n0 * n1;
+    const int r00 = a1 * n0 + a2 * s2, r10 = c1 * n0 + a2 * s2;
+    const int r01 = a1 * n0 + c2 * s2, r11 = c1 * n0 + c2 * s2;
+    const float u0 = 1.f - t0, u1 = 1.f - t1, u2 = 1.f - t2;
+    const float c00 = img[r00 + a0] * u0 + img[r00 + c0] * t0;
+    const float c10 = img[r10 + a0] * u0 + img[r10 + c0] * t0;
+    const float c01 = img[r01 + a0] * u0 + img[r01 + c0] * t0;
+    const float c11 = img[r11 + a0] * u0 + img[r11 + c0] * t0;
+    const float e0 = c00 * u1 + c10 * t1;
+    const float e1 = c01 * u1 + c11 * t1;
+    return e0 * u2 + e1 * t2;
+}
+
+// DVF of voxel id (kernel_forwardDVF, src/cu_deform.cu:74-76): (mx, my, mz).
+struct Dvf {
+    float x, y, z;
+};
+__device__ inline Dvf dvf_at(const Fields &f, int id, float vol, float flow) {
+    return Dvf{f.ax[id] * vol + f.bx[id] * flow, f.ay[id] * vol + f.by[id] * flow,
+               f.az[id] * vol + f.bz[id] * flow};
+}
+
+// Forward differences of x at voxel id, zero at the last index (src/cu_diff.cu:2-33).
+struct Diffs {
+    float x, d1, d2, d3;
+};
+__device__ inline Diffs diffs_at(const float *__restrict__ x, int id, int i0, int i1, int i2,
+                                 int nx, int ny, int nz) {
+    Diffs d;
+    d.x = x[id];
+    d.d1 = (i0 == nx - 1) ? 0.f : x[id + 1] - d.x;
+    d.d2 = (i1 == ny - 1) ? 0.f : x[id + nx] - d.x;
+    d.d3 = (i2 == nz - 1) ? 0.f : x[id + nx * ny] - d.x;
+    return d;
+}
+
+// Load the tile of f whose natural coordinates are (dim0 = Y0 + i, dim1 = X0 + j) into t[j][i]
+// (coalesced row reads); lane (tx, r) owning voxel (X0 + tx, Y0 + r) then finds
+// f^T(voxel) = f(dim0 = Y0 + r, dim1 = X0 + tx) at t[tx][r].
 __device__ inline void load_transposed(float (*t)[TT + 1], const float *__restrict__ f, int nx,
                                        int ny, int X0, int Y0, int i2) {
     const int tx = threadIdx.x;
-    for (int r = threadIdx.y; r < TT; r += TR) {
-        const int d0 = Y0 + tx, d1 = X0 + r;
-        t[r][tx] = (d0 < nx && d1 < ny) ? f[d0 + (long long)d1 * nx + (long long)i2 * nx * ny] : 0.f;
-    }
-}
-
-// g = x - img0, img0 = sum over (alpha_y,D1),(alpha_x,D2),(alpha_z,D3) of alpha^T*vd*D x plus the
-// same with beta and fd (src/AUMISART.cu:252-258); q = (D1+D2+D3)x * (-vd - fd) (:267).  The pair
-// (g, q) is written interleaved in the natural layout (pair) and/or, through the LDS tile, in the
-// xy-transposed layout (pairT), whichever the next projection reads.
-__global__ void __launch_bounds__(TT *TR) k_motion_prep(float2 *__restrict__ pair,
-                                                       float2 *__restrict__ pairT,
-                                                       const float *__restrict__ x, Fields f,
-                                                       int nx, int ny, int nz, float vd, float fd) {
-    __shared__ float t[6][TT][TT + 1];
-    const int X0 = blockIdx.x * TT, Y0 = blockIdx.y * TT, i2 = blockIdx.z;
-    load_transposed(t[0], f.ax, nx, ny, X0, Y0, i2);
-    load_transposed(t[1], f.ay, nx, ny, X0, Y0, i2);
-    load_transposed(t[2], f.az, nx, ny, X0, Y0, i2);
-    load_transposed(t[3], f.bx, nx, ny, X0, Y0, i2);
-    load_transposed(t[4], f.by, nx, ny, X0, Y0, i2);
-    load_transposed(t[5], f.bz, nx, ny, X0, Y0, i2);
-    __syncthreads();
-    const int tx = threadIdx.x;
-    const int i0 = X0 + tx;
-    const long long plane = (long long)nx * ny;
-    float gv[TT / TR], qv[TT / TR];
 #pragma unroll
-    for (int k = 0; k < TT / TR; ++k) {
+    for (int k = 0; k < TK; ++k) {
         const int r = threadIdx.y + k * TR;
-        const int i1 = Y0 + r;
-        gv[k] = qv[k] = 0.f;
-        if (i0 >= nx || i1 >= ny) continue;
-        const long long id = i0 + (long long)i1 * nx + (long long)i2 * plane;
-        const float d1 = dfwd(x, id, i0, i1, i2, nx, ny, nz, 1);
-        const float d2 = dfwd(x, id, i0, i1, i2, nx, ny, nz, 2);
-        const float d3 = dfwd(x, id, i0, i1, i2, nx, ny, nz, 3);
-        float img0 = 0.f;
-        img0 += t[1][tx][r] * vd * d1;   // alpha_y, ind 1
-        img0 += t[0][tx][r] * vd * d2;   // alpha_x, ind 2
-        img0 += t[2][tx][r] * vd * d3;   // alpha_z, ind 3
-        img0 += t[4][tx][r] * fd * d1;   // beta_y
-        img0 += t[3][tx][r] * fd * d2;   // beta_x
-        img0 += t[5][tx][r] * fd * d3;   // beta_z
-        gv[k] = x[id] - img0;
-        qv[k] = (d1 + d2 + d3) * (-vd + -fd);
-        if (pair) pair[id] = make_float2(gv[k], qv[k]);
-    }
-    if (pairT) {   // uniform over the workgroup
-        __syncthreads();   // every lane has finished reading the field tiles
-#pragma unroll
-        for (int k = 0; k < TT / TR; ++k) {
-            const int r = threadIdx.y + k * TR;
-            t[0][r][tx] = gv[k];
-            t[1][r][tx] = qv[k];
-        }
-        __syncthreads();
-#pragma unroll
-        for (int k = 0; k < TT / TR; ++k) {
-            const int r = threadIdx.y + k * TR;
-            const int j0 = Y0 + tx, j1 = X0 + r;   // transposed element (iy = j0, ix = j1)
-            if (j0 < ny && j1 < nx)
-                pairT[j0 + (long long)j1 * ny + (long long)i2 * plane] = make_float2(t[0][tx][r], t[1][tx][r]);
-        }
+        const int d0 = Y0 + tx, d1 = X0 + r;
+        t[r][tx] = (d0 < nx && d1 < ny) ? f[d0 + d1 * nx + i2 * nx * ny] : 0.f;
     }
 }
 
-// Store a 32x32 tile held in registers (lane (tx, r) owns voxel (X0 + tx, Y0 + r)) in the
-// xy-transposed layout through LDS; lane (tx, r) then writes element (iy = Y0 + tx, ix = X0 + r).
-__device__ inline void store_transposed(float (*t)[TT + 1], float *__restrict__ outT,
-                                        const float *v, int nx, int ny, int X0, int Y0, int i2) {
+// Store values v[k] of voxels (X0 + tx, Y0 + ty + 8k) in the xy-transposed layout
+// outT[iy + ix*ny + iz*nx*ny] through LDS; every global store is a coalesced row.
+template <class T>
+__device__ inline void store_transposed(T (*t)[TT + 1], T *__restrict__ outT, const T *v,
+                                        int nx, int ny, int X0, int Y0, int i2) {
     const int tx = threadIdx.x;
 #pragma unroll
-    for (int k = 0; k < TT / TR; ++k) t[threadIdx.y + k * TR][tx] = v[k];
+    for (int k = 0; k < TK; ++k) t[threadIdx.y + k * TR][tx] = v[k];
     __syncthreads();
 #pragma unroll
-    for (int k = 0; k < TT / TR; ++k) {
+    for (int k = 0; k < TK; ++k) {
         const int r = threadIdx.y + k * TR;
         const int j0 = Y0 + tx, j1 = X0 + r;
-        if (j0 < ny && j1 < nx)
-            outT[j0 + (long long)j1 * ny + (long long)i2 * nx * ny] = t[tx][r];
+        if (j0 < ny && j1 < nx) outT[j0 + j1 * ny + i2 * nx * ny] = t[tx][r];
     }
 }
 
-// Forward warp over 32x32 tiles: natural output (out) and/or transposed output (outT).
+// Forward warp (src/cu_deform.cu:79-90): natural output (out) and/or transposed output (outT).
 __global__ void __launch_bounds__(TT *TR) k_warp_tile(float *__restrict__ out,
                                                      float *__restrict__ outT,
                                                      const float *__restrict__ src, int nx, int ny,
@@ -289,41 +237,205 @@ __global__ void __launch_bounds__(TT *TR) k_warp_tile(float *__restrict__ out,
     __shared__ float t[TT][TT + 1];
     const int X0 = blockIdx.x * TT, Y0 = blockIdx.y * TT, i2 = blockIdx.z;
     const int i0 = X0 + threadIdx.x;
-    float v[TT / TR];
+    Dvf m[TK];
+    int id[TK];
+    bool ok[TK];
 #pragma unroll
-    for (int k = 0; k < TT / TR; ++k) {
+    for (int k = 0; k < TK; ++k) {
         const int i1 = Y0 + threadIdx.y + k * TR;
-        v[k] = 0.f;
-        if (i0 >= nx || i1 >= ny) continue;
-        const long long id = i0 + (long long)i1 * nx + (long long)i2 * nx * ny;
-        const float mx = f.ax[id] * vol + f.bx[id] * flow;   // src/cu_deform.cu:74-76
-        const float my = f.ay[id] * vol + f.by[id] * flow;
-        const float mz = f.az[id] * vol + f.bz[id] * flow;
-        v[k] = trilinear(src, nx, ny, nz, i0, my, i1, mx, i2, mz);   // :85-89
-        if (out) out[id] = v[k];
+        ok[k] = i0 < nx && i1 < ny;
+        id[k] = i0 + i1 * nx + i2 * nx * ny;
+        m[k] = ok[k] ? dvf_at(f, id[k], vol, flow) : Dvf{0.f, 0.f, 0.f};
+    }
+    float v[TK];
+#pragma unroll
+    for (int k = 0; k < TK; ++k) {
+        const int i1 = Y0 + threadIdx.y + k * TR;
+        // sample at (i0 + my, i1 + mx, i2 + mz) (:85-89)
+        v[k] = ok[k] ? trilinear(src, nx, ny, nz, i0, m[k].y, i1, m[k].x, i2, m[k].z) : 0.f;
+    }
+    if (out) {
+#pragma unroll
+        for (int k = 0; k < TK; ++k)
+            if (ok[k]) out[id[k]] = v[k];
     }
     if (outT) store_transposed(t, outT, v, nx, ny, X0, Y0, i2);
 }
 
-// x += lambda * u (optional clamp at zero) over 32x32 tiles, with the transposed copy of the
-// result when xT is given.
+// x += lambda * W(u; vol, flow) (src/AUMISART.cu:222-223), then the optional clamp at zero.
+__global__ void __launch_bounds__(TT *TR) k_warp_add_tile(float *__restrict__ x,
+                                                         const float *__restrict__ u, int nx,
+                                                         int ny, int nz, float vol, float flow,
+                                                         Fields f, float lambda, int nonneg) {
+    const int X0 = blockIdx.x * TT, Y0 = blockIdx.y * TT, i2 = blockIdx.z;
+    const int i0 = X0 + threadIdx.x;
+    Dvf m[TK];
+    float xv[TK];
+    int id[TK];
+    bool ok[TK];
+#pragma unroll
+    for (int k = 0; k < TK; ++k) {
+        const int i1 = Y0 + threadIdx.y + k * TR;
+        ok[k] = i0 < nx && i1 < ny;
+        id[k] = i0 + i1 * nx + i2 * nx * ny;
+        m[k] = ok[k] ? dvf_at(f, id[k], vol, flow) : Dvf{0.f, 0.f, 0.f};
+        xv[k] = ok[k] ? x[id[k]] : 0.f;
+    }
+    float v[TK];
+#pragma unroll
+    for (int k = 0; k < TK; ++k) {
+        const int i1 = Y0 + threadIdx.y + k * TR;
+        v[k] = ok[k] ? trilinear(u, nx, ny, nz, i0, m[k].y, i1, m[k].x, i2, m[k].z) : 0.f;
+    }
+#pragma unroll
+    for (int k = 0; k < TK; ++k) {
+        float y = xv[k] + v[k] * lambda;
+        if (nonneg) y = fmaxf(y, 0.f);
+        if (ok[k]) x[id[k]] = y;
+    }
+}
+
+// One step of the motion (udvf) sweep, fused over a tile pair.  With A the workgroup's tile and
+// A^T its transpose (nx == ny):
+//   UPD : fields on A += u^T * (wv | wf) * D_c x      (view i,   src/AUMISART.cu:272-277)
+//   PREP: on A^T, g = x - sum_c field_c^T (vd | fd) D_c x, q = (D1+D2+D3)x (-vd - fd)
+//                                                       (view i+1, src/AUMISART.cu:252-258,267)
+// PREP reads the fields at the transposed index, i.e. on A, exactly the values UPD has just
+// written, so one workgroup needs nothing from another and the fields are read once per step.
+// x is constant over a motion sweep (it projects the unwarped image, :246).  The pair (g, q)
+// goes interleaved to `pair` (natural layout) and/or `pairT` (transposed layout).
+template <bool UPD, bool PREP>
+__global__ void __launch_bounds__(TT *TR) k_motion_step(float *__restrict__ ax,
+                                                       float *__restrict__ ay,
+                                                       float *__restrict__ az,
+                                                       float *__restrict__ bx,
+                                                       float *__restrict__ by,
+                                                       float *__restrict__ bz,
+                                                       const float *__restrict__ u,
+                                                       const float *__restrict__ x,
+                                                       float2 *__restrict__ pair,
+                                                       float2 *__restrict__ pairT, int nx,
+                                                       int ny, int nz, float wv, float wf,
+                                                       float vd, float fd) {
+    __shared__ float t[6][TT][TT + 1];
+    const int X0 = blockIdx.x * TT, Y0 = blockIdx.y * TT, i2 = blockIdx.z;
+    const int tx = threadIdx.x, ty = threadIdx.y;
+    const int plane = nx * ny;
+    float *const F[6] = {ax, ay, az, bx, by, bz};
+    float nf[6][TK];   // fields on A after the update, voxel (X0 + tx, Y0 + ty + 8k)
+    int id[TK];
+    bool ok[TK];
+    if (UPD) load_transposed(t[0], u, nx, ny, X0, Y0, i2);
+#pragma unroll
+    for (int k = 0; k < TK; ++k) {
+        const int i1 = Y0 + ty + k * TR;
+        ok[k] = X0 + tx < nx && i1 < ny;
+        id[k] = X0 + tx + i1 * nx + i2 * plane;
+#pragma unroll
+        for (int c = 0; c < 6; ++c) nf[c][k] = ok[k] ? F[c][id[k]] : 0.f;
+    }
+    if (UPD) {
+        Diffs d[TK];
+#pragma unroll
+        for (int k = 0; k < TK; ++k)
+            d[k] = ok[k] ? diffs_at(x, id[k], X0 + tx, Y0 + ty + k * TR, i2, nx, ny, nz)
+                         : Diffs{0.f, 0.f, 0.f, 0.f};
+        __syncthreads();   // the u tile is complete
+#pragma unroll
+        for (int k = 0; k < TK; ++k) {
+            const float ut = t[0][tx][ty + k * TR];
+            nf[1][k] += ut * wv * d[k].d1;   // alpha_y with D1
+            nf[0][k] += ut * wv * d[k].d2;   // alpha_x with D2
+            nf[2][k] += ut * wv * d[k].d3;   // alpha_z with D3
+            nf[4][k] += ut * wf * d[k].d1;   // beta_y
+            nf[3][k] += ut * wf * d[k].d2;   // beta_x
+            nf[5][k] += ut * wf * d[k].d3;   // beta_z
+        }
+#pragma unroll
+        for (int k = 0; k < TK; ++k)
+            if (ok[k])
+#pragma unroll
+                for (int c = 0; c < 6; ++c) F[c][id[k]] = nf[c][k];
+    }
+    if (!PREP) return;
+    if (UPD) __syncthreads();   // every lane is done with the u tile
+#pragma unroll
+    for (int c = 0; c < 6; ++c)
+#pragma unroll
+        for (int k = 0; k < TK; ++k) t[c][ty + k * TR][tx] = nf[c][k];
+    __syncthreads();
+    // lane (tx, r) now owns voxel p = (Y0 + tx, X0 + r) of A^T; field_c^T(p) is t[c][tx][r]
+    float gv[TK], qv[TK];
+    Diffs d[TK];
+    int jd[TK];
+    bool jok[TK];
+#pragma unroll
+    for (int k = 0; k < TK; ++k) {
+        const int j0 = Y0 + tx, j1 = X0 + ty + k * TR;
+        jok[k] = j0 < nx && j1 < ny;
+        jd[k] = j0 + j1 * nx + i2 * plane;
+        d[k] = jok[k] ? diffs_at(x, jd[k], j0, j1, i2, nx, ny, nz) : Diffs{0.f, 0.f, 0.f, 0.f};
+    }
+#pragma unroll
+    for (int k = 0; k < TK; ++k) {
+        const int r = ty + k * TR;
+        float img0 = 0.f;
+        img0 += t[1][tx][r] * vd * d[k].d1;   // alpha_y, ind 1
+        img0 += t[0][tx][r] * vd * d[k].d2;   // alpha_x, ind 2
+        img0 += t[2][tx][r] * vd * d[k].d3;   // alpha_z, ind 3
+        img0 += t[4][tx][r] * fd * d[k].d1;   // beta_y
+        img0 += t[3][tx][r] * fd * d[k].d2;   // beta_x
+        img0 += t[5][tx][r] * fd * d[k].d3;   // beta_z
+        gv[k] = d[k].x - img0;
+        qv[k] = (d[k].d1 + d[k].d2 + d[k].d3) * (-vd + -fd);
+    }
+    if (pair) {
+#pragma unroll
+        for (int k = 0; k < TK; ++k)
+            if (jok[k]) pair[jd[k]] = make_float2(gv[k], qv[k]);
+    }
+    if (pairT) {
+        // pairT at natural position P of A holds the pair of P^T in A^T; lane (tx, r) writes
+        // P = (X0 + tx, Y0 + r), computed by lane (r, tx)
+        __syncthreads();   // all field reads from t are done
+#pragma unroll
+        for (int k = 0; k < TK; ++k) {
+            t[0][ty + k * TR][tx] = gv[k];
+            t[1][ty + k * TR][tx] = qv[k];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < TK; ++k) {
+            const int r = ty + k * TR;
+            if (ok[k]) pairT[id[k]] = make_float2(t[0][tx][r], t[1][tx][r]);
+        }
+    }
+}
+
 __global__ void __launch_bounds__(TT *TR) k_add_tile(float *__restrict__ x, float *__restrict__ xT,
                                                     const float *__restrict__ u, int nx, int ny,
                                                     int nz, float lambda, int nonneg) {
     __shared__ float t[TT][TT + 1];
     const int X0 = blockIdx.x * TT, Y0 = blockIdx.y * TT, i2 = blockIdx.z;
     const int i0 = X0 + threadIdx.x;
-    float v[TT / TR];
+    float xv[TK], uv[TK];
+    int id[TK];
+    bool ok[TK];
 #pragma unroll
-    for (int k = 0; k < TT / TR; ++k) {
+    for (int k = 0; k < TK; ++k) {
         const int i1 = Y0 + threadIdx.y + k * TR;
-        v[k] = 0.f;
-        if (i0 >= nx || i1 >= ny) continue;
-        const long long id = i0 + (long long)i1 * nx + (long long)i2 * nx * ny;
-        float y = x[id] + u[id] * lambda;   // src/cu_add.cu:17
+        ok[k] = i0 < nx && i1 < ny;
+        id[k] = i0 + i1 * nx + i2 * nx * ny;
+        xv[k] = ok[k] ? x[id[k]] : 0.f;
+        uv[k] = ok[k] ? u[id[k]] : 0.f;
+    }
+    float v[TK];
+#pragma unroll
+    for (int k = 0; k < TK; ++k) {
+        float y = xv[k] + uv[k] * lambda;   // src/cu_add.cu:17
         if (nonneg) y = fmaxf(y, 0.f);
-        x[id] = y;
         v[k] = y;
+        if (ok[k]) x[id[k]] = y;
     }
     if (xT) store_transposed(t, xT, v, nx, ny, X0, Y0, i2);
 }
@@ -334,44 +446,13 @@ __global__ void __launch_bounds__(TT *TR) k_transpose(float *__restrict__ dst,
     __shared__ float t[TT][TT + 1];
     const int X0 = blockIdx.x * TT, Y0 = blockIdx.y * TT, i2 = blockIdx.z;
     const int i0 = X0 + threadIdx.x;
-    float v[TT / TR];
+    float v[TK];
 #pragma unroll
-    for (int k = 0; k < TT / TR; ++k) {
+    for (int k = 0; k < TK; ++k) {
         const int i1 = Y0 + threadIdx.y + k * TR;
-        v[k] = (i0 < nx && i1 < ny) ? src[i0 + (long long)i1 * nx + (long long)i2 * nx * ny] : 0.f;
+        v[k] = (i0 < nx && i1 < ny) ? src[i0 + i1 * nx + i2 * nx * ny] : 0.f;
     }
     store_transposed(t, dst, v, nx, ny, X0, Y0, i2);
-}
-
-// alpha_c += u^T * wv * D_c x, beta_c += u^T * wf * D_c x with the pairs (alpha_y, D1),
-// (alpha_x, D2), (alpha_z, D3) (src/AUMISART.cu:272-277).
-__global__ void __launch_bounds__(TT *TR) k_motion_update(float *ax, float *ay, float *az,
-                                                         float *bx, float *by, float *bz,
-                                                         const float *__restrict__ u,
-                                                         const float *__restrict__ x, int nx,
-                                                         int ny, int nz, float wv, float wf) {
-    __shared__ float t[TT][TT + 1];
-    const int X0 = blockIdx.x * TT, Y0 = blockIdx.y * TT, i2 = blockIdx.z;
-    load_transposed(t, u, nx, ny, X0, Y0, i2);
-    __syncthreads();
-    const int tx = threadIdx.x;
-    const int i0 = X0 + tx;
-    if (i0 >= nx) return;
-    for (int r = threadIdx.y; r < TT; r += TR) {
-        const int i1 = Y0 + r;
-        if (i1 >= ny) break;
-        const long long id = i0 + (long long)i1 * nx + (long long)i2 * nx * ny;
-        const float d1 = dfwd(x, id, i0, i1, i2, nx, ny, nz, 1);
-        const float d2 = dfwd(x, id, i0, i1, i2, nx, ny, nz, 2);
-        const float d3 = dfwd(x, id, i0, i1, i2, nx, ny, nz, 3);
-        const float ut = t[tx][r];
-        ay[id] += ut * wv * d1;
-        ax[id] += ut * wv * d2;
-        az[id] += ut * wv * d3;
-        by[id] += ut * wf * d1;
-        bx[id] += ut * wf * d2;
-        bz[id] += ut * wf * d3;
-    }
 }
 
 inline dim3 vgrid(int nx, int ny, int nz) {
@@ -410,8 +491,8 @@ int launch_add_tile(float *x, float *xT, const float *u, int nx, int ny, int nz,
 
 int launch_warp_add(float *x, const float *u, int nx, int ny, int nz, float vol, float flow,
                     const Fields &f, float lambda, int nonneg, hipStream_t st) {
-    hipLaunchKernelGGL(k_warp<true>, vgrid(nx, ny, nz), dim3(EX, EY), 0, st, x, u, nx, ny, nz,
-                       vol, flow, f, lambda, nonneg);
+    hipLaunchKernelGGL(k_warp_add_tile, tile_grid(nx, ny, nz), dim3(TT, TR), 0, st, x, u, nx, ny,
+                       nz, vol, flow, f, lambda, nonneg);
     MCSART_CHECK_LAUNCH();
     return MCSART_OK;
 }
@@ -482,18 +563,31 @@ int launch_update_udvf(float *ax, float *ay, float *az, float *bx, float *by, fl
 
 int launch_motion_prep(float2 *pair, float2 *pairT, const float *x, const float *const fields[6],
                        int nx, int ny, int nz, float vd, float fd, hipStream_t st) {
-    const Fields f{fields[0], fields[1], fields[2], fields[3], fields[4], fields[5]};
-    hipLaunchKernelGGL(k_motion_prep, tile_grid(nx, ny, nz), dim3(TT, TR), 0, st, pair, pairT, x,
-                       f, nx, ny, nz, vd, fd);
-    MCSART_CHECK_LAUNCH();
-    return MCSART_OK;
+    // the PREP half reads the fields only; the kernel's field pointers are writable for UPD
+    float *const f[6] = {(float *)fields[0], (float *)fields[1], (float *)fields[2],
+                         (float *)fields[3], (float *)fields[4], (float *)fields[5]};
+    return launch_motion_step(f, nullptr, x, pair, pairT, nx, ny, nz, 0.f, 0.f, vd, fd, false,
+                              true, st);
 }
 
 int launch_motion_update(float *const fields[6], const float *u, const float *x, int nx, int ny,
                          int nz, float wv, float wf, hipStream_t st) {
-    const dim3 grid((nx + TT - 1) / TT, (ny + TT - 1) / TT, nz);
-    hipLaunchKernelGGL(k_motion_update, grid, dim3(TT, TR), 0, st, fields[0], fields[1],
-                       fields[2], fields[3], fields[4], fields[5], u, x, nx, ny, nz, wv, wf);
+    return launch_motion_step(fields, u, x, nullptr, nullptr, nx, ny, nz, wv, wf, 0.f, 0.f, true,
+                              false, st);
+}
+
+int launch_motion_step(float *const fields[6], const float *u, const float *x, float2 *pair,
+                       float2 *pairT, int nx, int ny, int nz, float wv, float wf, float vd,
+                       float fd, bool update, bool prep, hipStream_t st) {
+    const dim3 grid = tile_grid(nx, ny, nz), block(TT, TR);
+#define MCSART_MOTION_STEP(U, P)                                                                 \
+    hipLaunchKernelGGL((k_motion_step<U, P>), grid, block, 0, st, fields[0], fields[1],          \
+                       fields[2], fields[3], fields[4], fields[5], u, x, pair, pairT, nx, ny, nz, \
+                       wv, wf, vd, fd)
+    if (update && prep) MCSART_MOTION_STEP(true, true);
+    else if (update) MCSART_MOTION_STEP(true, false);
+    else if (prep) MCSART_MOTION_STEP(false, true);
+#undef MCSART_MOTION_STEP
     MCSART_CHECK_LAUNCH();
     return MCSART_OK;
 }

@@ -248,6 +248,231 @@ __global__ void __launch_bounds__(RAYS *MAXSPLIT) k_project(GeomArgs g, double c
     }
 }
 
+// ---- rows kernel ------------------------------------------------------------------------------
+//
+// A lane walks R rays of one detector column (rows ib0 .. ib0+R-1).  Their xy directions are the
+// same, so the transverse half of every plane's work (bounds, window, weights) is done once and
+// shared by the R rays; only the axial half is per ray.  The window of WT (WZ) voxels starts at
+// floor(lo) clamped into [0, n - WT]: every voxel it names exists, so no per-voxel clamp or mask
+// is needed, and a voxel of the window outside the footprint gets weight zero from the overlap
+// formula itself.  Overlaps are differences of the clamped boundary F(b) = min(max(b, lo), hi):
+// w_k = F(T0+k+1) - F(T0+k) = min(T0+k+1, hi) - max(T0+k, lo) whenever the cell meets the
+// footprint (src/cu_projection.cu:83,86), and 0 otherwise.  Plane offsets use 24-bit multiplies
+// (launch requires nx*ny < 2^24).  Requires the xy-transposed layout for x-branch rays.
+template <int WT>
+struct TPlane {
+    float w[WT];
+    float sw, width;
+    bool in;
+    unsigned base;   // T0 + ip * plane stride
+};
+
+// x where `in`, +0 elsewhere, as a bit mask rather than a select: the contribution of a plane
+// outside the volume is dropped even when its (in-volume, clamped) window holds non-finite
+// values, and the compiler keeps every gather unconditional instead of branching around them.
+__device__ inline float masked(float x, bool in) {
+    return __uint_as_float(__float_as_uint(x) & (in ? 0xffffffffu : 0u));
+}
+
+template <int W>
+__device__ inline void window(float lo, float hi, float top, float (&w)[W], float &sw, float &f0) {
+    f0 = fminf(fmaxf(floorf(lo), 0.f), top);   // window start, inside [0, n - W]
+    float F = fmaxf(f0, lo);                    // F(f0): f0 <= hi whenever the plane is inside
+    const float F0 = F;
+#pragma unroll
+    for (int k = 0; k < W; ++k) {
+        const float Fn = (k + 1 < W) ? fminf(fmaxf(f0 + (float)(k + 1), lo), hi)
+                                     : fminf(f0 + (float)W, hi);   // f0 + W >= lo inside
+        w[k] = Fn - F;
+        F = Fn;
+    }
+    sw = F - F0;
+}
+
+template <int MODE, int WT, int WZ, int R>
+__global__ void __launch_bounds__(RAYS * 16) k_project_rows(GeomArgs g, double c, double s,
+                                                           ProjArgs a, int nsplit) {
+    const int tx = threadIdx.x, ty = threadIdx.y;
+    const int ia = blockIdx.x * RAYS + tx;
+    const int ib0 = blockIdx.y * R;
+    const bool live = ia < g.na;
+
+    __shared__ RayT rt[RAYS];
+    __shared__ RayZ rz[R][RAYS];
+    if (ty == 0 && live) {
+        const RayDir d = ray_dir(g, c, s, ia);
+        ray_t_setup(g, d, rt[tx]);
+#pragma unroll
+        for (int r = 0; r < R; ++r) ray_z_setup(g, d, min(ib0 + r, g.nb - 1), rz[r][tx]);
+    }
+    __syncthreads();
+
+    float acc0[R], acc1[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) acc0[r] = acc1[r] = 0.f;
+    if (live) {
+        const RayT T = rt[tx];
+        RayZ Z[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) Z[r] = rz[r][tx];
+        const bool xb = T.xbranch != 0;
+        const int np = xb ? g.nx : g.ny;
+        const int nt = xb ? g.ny : g.nx;
+        const unsigned sp = (unsigned)nt;   // plane stride: T[iy + ix*ny] / img[ix + iy*nx]
+        const unsigned sz = (unsigned)g.nx * (unsigned)g.ny;
+        const float *__restrict__ v = xb ? a.vol0T : a.vol0;
+        const float2 *__restrict__ v2 = xb ? a.pairT : a.pair;
+        const float p0 = 0.5f - 0.5f * (float)np;   // (float)ip + 0.5 - n/2 (:52, :105)
+        const float ntf = (float)nt, nzf = (float)g.nz;
+        const float ttop = (float)(nt - WT), ztop = (float)(g.nz - WZ);
+        const int chunk = (np + nsplit - 1) / nsplit;
+        const int ip_lo = ty * chunk;
+        const int ip_hi = min(np, ip_lo + chunk);
+        // PL planes per iteration, all their gathers in flight together (one plane when a lane
+        // walks four rays: sixteen or more gathers already)
+        constexpr int PL = R >= 4 ? 1 : 2;
+        for (int ip = ip_lo; ip < ip_hi; ip += PL) {
+            const int npl = (ip + 1 < ip_hi) ? PL : 1;   // uniform over the wavefront
+            TPlane<WT> P[PL];
+            float wz[PL][R][WZ], swz[PL][R], sc[PL][R];
+            bool in[PL][R];
+            unsigned off[PL][R];
+#pragma unroll
+            for (int q = 0; q < PL; ++q) {
+                const float p = (float)(ip + q) + p0;
+                const float t1 = fmaf(T.rlo, p, T.Klo), t2 = fmaf(T.rhi, p, T.Khi);
+                float f0;
+                window<WT>(t1, t2, ttop, P[q].w, P[q].sw, f0);
+                P[q].width = t2 - t1;
+                P[q].in = (t2 > 0.f) & (t1 < ntf) & (q < npl);
+                P[q].base = (unsigned)(int)f0 + (unsigned)(ip + q) * sp;
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    const float z1 = fmaf(Z[r].rlo, p, Z[r].Klo), z2 = fmaf(Z[r].rhi, p, Z[r].Khi);
+                    float g0;
+                    window<WZ>(z1, z2, ztop, wz[q][r], swz[q][r], g0);
+                    in[q][r] = P[q].in & (z2 > 0.f) & (z1 < nzf);
+                    sc[q][r] = __builtin_amdgcn_rcpf(P[q].width * (z2 - z1));
+                    off[q][r] = P[q].base + __umul24((unsigned)(int)g0, sz);
+                }
+            }
+            if (MODE == PROJ_MOTION) {
+                float2 val[PL][R][WT][WZ];
+#pragma unroll
+                for (int q = 0; q < PL; ++q)
+#pragma unroll
+                    for (int r = 0; r < R; ++r) {
+                        // the second plane of a final odd chunk re-reads the first plane's voxels
+                        const float2 *qv = v2 + (q < npl ? off[q][r] : off[0][r]);
+#pragma unroll
+                        for (int k = 0; k < WT; ++k)
+#pragma unroll
+                            for (int j = 0; j < WZ; ++j) val[q][r][k][j] = qv[k + j * sz];
+                    }
+#pragma unroll
+                for (int q = 0; q < PL; ++q)
+#pragma unroll
+                    for (int r = 0; r < R; ++r) {
+                        float s0 = 0.f, s1 = 0.f;
+#pragma unroll
+                        for (int k = 0; k < WT; ++k) {
+                            float u0 = 0.f, u1 = 0.f;
+#pragma unroll
+                            for (int j = 0; j < WZ; ++j) {
+                                u0 = fmaf(wz[q][r][j], val[q][r][k][j].x, u0);
+                                u1 = fmaf(wz[q][r][j], val[q][r][k][j].y, u1);
+                            }
+                            s0 = fmaf(P[q].w[k], u0, s0);
+                            s1 = fmaf(P[q].w[k], u1, s1);
+                        }
+                        acc0[r] += masked(sc[q][r] * s0, in[q][r]);
+                        acc1[r] += masked(sc[q][r] * s1, in[q][r]);
+                    }
+            } else {
+                float val[PL][R][WT][WZ];
+#pragma unroll
+                for (int q = 0; q < PL; ++q)
+#pragma unroll
+                    for (int r = 0; r < R; ++r) {
+                        const float *qv = v + (q < npl ? off[q][r] : off[0][r]);
+#pragma unroll
+                        for (int k = 0; k < WT; ++k)
+#pragma unroll
+                            for (int j = 0; j < WZ; ++j) val[q][r][k][j] = qv[k + j * sz];
+                    }
+#pragma unroll
+                for (int q = 0; q < PL; ++q)
+#pragma unroll
+                    for (int r = 0; r < R; ++r) {
+                        float s0 = 0.f;
+#pragma unroll
+                        for (int k = 0; k < WT; ++k) {
+                            float u0 = 0.f;
+#pragma unroll
+                            for (int j = 0; j < WZ; ++j) u0 = fmaf(wz[q][r][j], val[q][r][k][j], u0);
+                            s0 = fmaf(P[q].w[k], u0, s0);
+                        }
+                        acc0[r] += masked(sc[q][r] * s0, in[q][r]);
+                        if (MODE == PROJ_IMAGE)   // P(1): the same weights on a volume of ones
+                            acc1[r] += masked(sc[q][r] * (P[q].sw * swz[q][r]), in[q][r]);
+                    }
+            }
+        }
+    }
+
+    // Combine the plane chunks (fixed order: deterministic).
+    extern __shared__ float red[];   // [2][nsplit][R][RAYS]
+    if (nsplit > 1) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            red[((0 * nsplit + ty) * R + r) * RAYS + tx] = acc0[r];
+            red[((1 * nsplit + ty) * R + r) * RAYS + tx] = acc1[r];
+        }
+        __syncthreads();
+        if (ty == 0) {
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                acc0[r] = red[(0 * R + r) * RAYS + tx];
+                acc1[r] = red[((1 * nsplit) * R + r) * RAYS + tx];
+                for (int k = 1; k < nsplit; ++k) {
+                    acc0[r] += red[((0 * nsplit + k) * R + r) * RAYS + tx];
+                    acc1[r] += red[((1 * nsplit + k) * R + r) * RAYS + tx];
+                }
+            }
+        }
+    }
+    if (ty != 0) return;
+
+    if (MODE == PROJ_PLAIN) {
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+            if (live && ib0 + r < g.nb) a.out[(long long)(ib0 + r) * g.na + ia] = acc0[r];
+        return;
+    }
+    // SART epilogue: residual, normaliser, norms.
+    double pp = 0.0, rr = 0.0;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        if (!live || ib0 + r >= g.nb) continue;
+        const long long id = (long long)(ib0 + r) * g.na + ia;
+        const float pm = a.meas[id];
+        const float res = pm - acc0[r];
+        a.rp[id] = make_float2(res, acc1[r]);
+        if (a.resid_copy) a.resid_copy[id] = res;
+        pp += (double)pm * (double)pm;
+        rr += (double)res * (double)res;
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        pp += __shfl_xor(pp, o, 64);
+        rr += __shfl_xor(rr, o, 64);
+    }
+    if (tx == 0) {   // finished by the next kernel on the stream (NormFinish)
+        const int bid = blockIdx.y * gridDim.x + blockIdx.x;
+        a.partials[2 * bid] = pp;
+        a.partials[2 * bid + 1] = rr;
+    }
+}
+
 // Footprints wider than the fixed windows (coarse voxels against a fine detector): the same
 // weights with data-dependent loops, as in the reference (src/cu_projection.cu:81-89).
 template <int MODE>
@@ -277,6 +502,60 @@ int launch_mode(const Geom &g, const ViewTrig &t, const ViewInfo &vi, const Proj
         case 3: hipLaunchKernelGGL((k_project<MODE, 3, 2>), grid, block, 0, st, ga, t.c, t.s, a, s); break;
         case 4: hipLaunchKernelGGL((k_project<MODE, 3, 3>), grid, block, 0, st, ga, t.c, t.s, a, s); break;
         default: hipLaunchKernelGGL((k_project_wide<MODE>), grid, block, 0, st, ga, t.c, t.s, a, s); break;
+    }
+    MCSART_CHECK_LAUNCH();
+    return MCSART_OK;
+}
+
+// Rows kernel shape: R rays per lane and S plane chunks per workgroup, enough wavefronts to fill
+// the device (>= 4096) with at least 4 planes per chunk.
+void pick_rows(const Geom &g, int rows, int split, int &R, int &S) {
+    const int np = g.nx > g.ny ? g.nx : g.ny;
+    const long long cols = (g.na + RAYS - 1) / RAYS;
+    auto split_for = [&](int r) {
+        const long long tiles = cols * ((g.nb + r - 1) / r);
+        int sp = 1;
+        while (sp < 16 && tiles * sp < 4096 && np / (2 * sp) >= 4) sp *= 2;
+        return sp;
+    };
+    R = rows > 0 ? rows : 0;
+    if (!R) {   // the most rows per lane that still fills the device
+        R = 1;
+        for (int r = 4; r >= 2; r /= 2)
+            if (cols * ((g.nb + r - 1) / r) * split_for(r) >= 4096) {
+                R = r;
+                break;
+            }
+    }
+    S = split > 0 ? split : split_for(R);
+}
+
+template <int MODE, int WT, int WZ, int R>
+void launch_rows(const GeomArgs &ga, const ViewTrig &t, const ProjArgs &a, hipStream_t st, int S) {
+    const dim3 block(RAYS, S);
+    const dim3 grid((ga.na + RAYS - 1) / RAYS, (ga.nb + R - 1) / R);
+    const size_t lds = S > 1 ? sizeof(float) * 2 * S * R * RAYS : 0;
+    hipLaunchKernelGGL((k_project_rows<MODE, WT, WZ, R>), grid, block, lds, st, ga, t.c, t.s, a, S);
+}
+
+template <int MODE, int WT, int WZ>
+void launch_rows_r(const GeomArgs &ga, const ViewTrig &t, const ProjArgs &a, hipStream_t st,
+                   int R, int S) {
+    if (R == 4) launch_rows<MODE, WT, WZ, 4>(ga, t, a, st, S);
+    else if (R == 2) launch_rows<MODE, WT, WZ, 2>(ga, t, a, st, S);
+    else launch_rows<MODE, WT, WZ, 1>(ga, t, a, st, S);
+}
+
+template <int MODE>
+int launch_mode_rows(const Geom &g, const ViewTrig &t, const ViewInfo &vi, const ProjArgs &a,
+                     hipStream_t st, int R, int S) {
+    const GeomArgs ga{g.nx, g.ny, g.nz, g.na, g.nb, g.da, g.db, g.ai, g.bi, g.SO, g.SD};
+    const int code = (vi.wt <= 2 ? 0 : 1) * 2 + (vi.wz <= 2 ? 0 : 1);
+    switch (code) {
+        case 0: launch_rows_r<MODE, 2, 2>(ga, t, a, st, R, S); break;
+        case 1: launch_rows_r<MODE, 2, 3>(ga, t, a, st, R, S); break;
+        case 2: launch_rows_r<MODE, 3, 2>(ga, t, a, st, R, S); break;
+        default: launch_rows_r<MODE, 3, 3>(ga, t, a, st, R, S); break;
     }
     MCSART_CHECK_LAUNCH();
     return MCSART_OK;
@@ -402,16 +681,35 @@ int projector_partials(const Geom &g) {
 }
 
 int launch_projector(const Geom &g, const ViewTrig &t, const ViewInfo &vi, ProjMode mode,
-                     const ProjArgs &a, hipStream_t st, int *nparts, int split) {
+                     const ProjArgs &a, hipStream_t st, int *nparts, int split, int rows) {
     const bool wide = !(vi.wt <= 3 && vi.wz <= 3);
-    if (split < 0 || split > MAXSPLIT || (split & (split - 1)))
-        return fail(MCSART_EINVAL, "projector split must be 0 (automatic) or a power of two <= " +
-                                       std::to_string(MAXSPLIT));
+    if (split < 0 || split > 16 || (split & (split - 1)))
+        return fail(MCSART_EINVAL, "projector split must be 0 (automatic) or a power of two <= 16");
+    if (rows != 0 && rows != 1 && rows != 2 && rows != 4)
+        return fail(MCSART_EINVAL, "projector rows must be 0 (automatic), 1, 2 or 4");
+    // The rows kernel needs contiguous footprints for every ray (the transposed layout when the
+    // view has x-branch rays), windows that fit in the volume and 24-bit plane offsets.
+    const bool have_t = mode == PROJ_MOTION ? a.pairT != nullptr : a.vol0T != nullptr;
+    const bool rows_ok = !wide && (!vi.has_x || have_t) && g.nx >= vi.wt && g.ny >= vi.wt &&
+                         g.nz >= vi.wz && (long long)g.nx * g.ny < (1LL << 24);
     int rc;
+    if (rows_ok) {
+        int R, S;
+        pick_rows(g, rows, split, R, S);
+        if (mode == PROJ_MOTION && vi.wt == 3 && vi.wz == 3 && R == 4) R = 2;   // registers
+        switch (mode) {
+            case PROJ_PLAIN: rc = launch_mode_rows<PROJ_PLAIN>(g, t, vi, a, st, R, S); break;
+            case PROJ_IMAGE: rc = launch_mode_rows<PROJ_IMAGE>(g, t, vi, a, st, R, S); break;
+            default: rc = launch_mode_rows<PROJ_MOTION>(g, t, vi, a, st, R, S); break;
+        }
+        if (nparts) *nparts = ((g.na + RAYS - 1) / RAYS) * ((g.nb + R - 1) / R);
+        return rc;
+    }
+    const int s8 = split > MAXSPLIT ? MAXSPLIT : split;
     switch (mode) {
-        case PROJ_PLAIN: rc = launch_mode<PROJ_PLAIN>(g, t, vi, a, st, split); break;
-        case PROJ_IMAGE: rc = launch_mode<PROJ_IMAGE>(g, t, vi, a, st, split); break;
-        default: rc = launch_mode<PROJ_MOTION>(g, t, vi, a, st, split); break;
+        case PROJ_PLAIN: rc = launch_mode<PROJ_PLAIN>(g, t, vi, a, st, s8); break;
+        case PROJ_IMAGE: rc = launch_mode<PROJ_IMAGE>(g, t, vi, a, st, s8); break;
+        default: rc = launch_mode<PROJ_MOTION>(g, t, vi, a, st, s8); break;
     }
     if (nparts) *nparts = ((g.na + RAYS - 1) / RAYS) * g.nb;
     if (rc || !wide || mode == PROJ_PLAIN) return rc;

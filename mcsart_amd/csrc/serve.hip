@@ -42,7 +42,7 @@ struct Request {
     int status = MCSART_OK;
     int slot = -1;
     int bypassed = 0;   // times a later request was admitted ahead of this one
-    mcsart_tuning tuning{0, 0, 0};
+    mcsart_tuning tuning{0, 0, 0, 0};
     mcsart_job *job = nullptr;
     Clock::time_point t_submit, t_admit, t_done;
 };
@@ -87,7 +87,8 @@ int start_request(mcsart_server *sv, Slot &s, Request *r) {
     if (sv->opts.autotune) {
         if ((rc = mcsart_job_autotune(j, &r->tuning))) return rc;
     } else {
-        mcsart_tuning t{sv->opts.proj_split, sv->opts.back_zslab, sv->opts.back_stage};
+        mcsart_tuning t{sv->opts.proj_split, sv->opts.back_zslab, sv->opts.back_stage,
+                        sv->opts.proj_rows};
         if ((rc = mcsart_job_set_tuning(j, &t))) return rc;
         r->tuning = t;
     }
@@ -225,8 +226,9 @@ extern "C" int mcsart_server_create(int device, const mcsart_server_opts *o,
     *out = nullptr;
     if (!o) return fail(MCSART_EINVAL, "null options");
     if (o->slots < 1 || o->slots > 16) return fail(MCSART_EINVAL, "slots must be in 1..16");
-    const mcsart_tuning t{o->proj_split, o->back_zslab, o->back_stage};
-    if (t.proj_split < 0 || t.proj_split > 8 || (t.proj_split & (t.proj_split - 1)) ||
+    const mcsart_tuning t{o->proj_split, o->back_zslab, o->back_stage, o->proj_rows};
+    if (t.proj_split < 0 || t.proj_split > 16 || (t.proj_split & (t.proj_split - 1)) ||
+        (t.proj_rows != 0 && t.proj_rows != 1 && t.proj_rows != 2 && t.proj_rows != 4) ||
         t.back_zslab < 0 || t.back_zslab > 64 || (t.back_stage != 0 && t.back_stage != 1))
         return fail(MCSART_EINVAL, "bad fixed launch shapes");
     MCSART_HIP_TRY(hipSetDevice(device));

@@ -3,11 +3,11 @@ hardware-counter passes).
 
     python -m mcsart_amd.profile_sweep --workload mcsart128 --sweeps 1,0 --copy-mb 64
 
-Builds the named workload's scan (synthetic inputs, measured projections made on the GPU), then
-runs the given sweeps (1 = image sweep, 0 = motion sweep) kernel by kernel (no graphs, so every
-kernel is its own dispatch), and finally one device-to-device copy of --copy-mb MiB: the
-profiler's FETCH_SIZE / WRITE_SIZE for that copy calibrate the counters against a known byte
-count.
+Builds the named workload's scan (synthetic inputs, measured projections made on the GPU),
+autotunes the footprint kernels' launch shapes as the benchmark does, then runs the given sweeps
+(1 = image sweep, 0 = motion sweep) kernel by kernel (no graphs, so every kernel is its own
+dispatch), and finally one device-to-device copy of --copy-mb MiB: the profiler's FETCH_SIZE /
+WRITE_SIZE for that copy calibrate the counters against a known byte count.
 """
 import argparse
 import sys
@@ -18,29 +18,12 @@ import numpy as np
 from . import configs
 from .configs import make_scan_inputs
 from .driver import Context, Job
-from .ops import DeviceArray, host_deform, host_projection
+from .ops import DeviceArray
+from .simulate import measure
 from ._lib import check, load
 
 WORKLOADS = {"mcsart128": configs.c2_mcsart128, "clinical256": configs.c3_clinical,
              "static64": configs.c1_static64}
-
-
-def measured_projections(spec, data):
-    g = spec.geometry
-    d_ph = DeviceArray.from_host(data.phantom)
-    d_w = DeviceArray.empty(g.vol_shape)
-    d_f = [DeviceArray.from_host(f) for f in data.fields] if data.fields else None
-    d_p = DeviceArray.empty(g.det_shape)
-    out = np.empty((spec.n_views,) + g.det_shape, np.float32)
-    for i, a in enumerate(spec.angles):
-        src = d_ph
-        if d_f is not None:
-            host_deform(d_w, d_ph, g.nx, g.ny, g.nz, float(data.alpha[i]), float(data.beta[i]), *d_f)
-            src = d_w
-        host_projection(d_p, src, float(a), g.SO, g.SD, g.da, g.na, g.ai, g.db, g.nb, g.bi,
-                        g.nx, g.ny, g.nz)
-        out[i] = d_p.to_host()
-    return out
 
 
 def main(argv=None):
@@ -49,6 +32,7 @@ def main(argv=None):
     ap.add_argument("--sweeps", default="1,0")
     ap.add_argument("--views", type=int, default=0, help="use only the first V views (0: all)")
     ap.add_argument("--copy-mb", type=int, default=64)
+    ap.add_argument("--no-autotune", action="store_true")
     args = ap.parse_args(argv)
     spec = WORKLOADS[args.workload]()
     if args.views:
@@ -56,7 +40,7 @@ def main(argv=None):
     sweeps = [int(s) for s in args.sweeps.split(",") if s != ""]
     g = spec.geometry
     data = make_scan_inputs(spec)
-    proj = measured_projections(spec, data)
+    proj = measure(spec, data, spec.seed + 404)
     ctx = Context(0)
     job = Job(ctx, g, spec.angles, data.alpha, data.beta, sweeps, lam=spec.lam,
               motion=spec.motion, nonneg=spec.nonneg)
@@ -64,6 +48,8 @@ def main(argv=None):
     job.set_projections(proj)
     job.set_image(None)
     job.set_fields(data.fields)
+    if not args.no_autotune:
+        print(f"[profile_sweep] autotuned launch shapes: {job.autotune()}", flush=True)
     t0 = time.perf_counter()
     job.run()
     ctx.synchronize()

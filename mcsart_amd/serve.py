@@ -24,7 +24,8 @@ _FP = ctypes.POINTER(ctypes.c_float)
 class CServerOpts(ctypes.Structure):
     _fields_ = [("slots", ctypes.c_int), ("hbm_budget", ctypes.c_size_t),
                 ("autotune", ctypes.c_int), ("proj_split", ctypes.c_int),
-                ("back_zslab", ctypes.c_int), ("back_stage", ctypes.c_int)]
+                ("back_zslab", ctypes.c_int), ("back_stage", ctypes.c_int),
+                ("proj_rows", ctypes.c_int)]
 
 
 class CRequest(ctypes.Structure):
@@ -61,7 +62,7 @@ class Ticket:
 
 class Server:
     def __init__(self, slots: int = 4, hbm_budget: int = 0, autotune: bool = False,
-                 tuning: Sequence[int] = (0, 0, 0), device: int = 0):
+                 tuning: Sequence[int] = (0, 0, 0, 0), device: int = 0):
         self._lib = load()
         o = CServerOpts(int(slots), int(hbm_budget), int(autotune), *[int(t) for t in tuning])
         p = ctypes.c_void_p()

@@ -126,5 +126,5 @@ def test_autotuned_requests_match(scans):
         srv.drain()
         for t, (img, nrm, _) in zip(tickets, alone):
             inf = srv.info(t)
-            assert inf["tuning"]["proj_split"] in (1, 2, 4, 8)
+            assert inf["tuning"]["proj_split"] in (1, 2, 4, 8, 16)
             assert rel_l2(t.image, img) <= 1e-5

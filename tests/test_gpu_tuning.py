@@ -16,7 +16,9 @@ from mcsart_amd.synth import ellipsoid_phantom
 from _util import rel_l2
 
 pytestmark = pytest.mark.gpu
-SHAPES = [(1, 1, 0), (1, 1, 1), (8, 16, 0), (8, 16, 1), (2, 4, 1), (4, 2, 0), (0, 64, 1)]
+# (proj_split, back_zslab, back_stage, proj_rows)
+SHAPES = [(1, 1, 0, 1), (1, 1, 1, 1), (8, 16, 0, 2), (8, 16, 1, 2), (2, 4, 1, 4), (4, 2, 0, 4),
+          (16, 64, 1, 4), (16, 8, 0, 1), (0, 0, 0, 1), (0, 0, 0, 2)]
 
 
 @pytest.fixture(scope="module")
@@ -45,7 +47,7 @@ def _job(scan, op_iter=(1, 1)):
 def test_shapes_agree(scan):
     g, ang, proj, ctx = scan
     job = _job(scan)
-    assert job.tuning() == {"proj_split": 0, "back_zslab": 0, "back_stage": 0}
+    assert job.tuning() == {"proj_split": 0, "back_zslab": 0, "back_stage": 0, "proj_rows": 0}
     job.run()
     base = job.image()
     imgs = {}
@@ -60,9 +62,9 @@ def test_shapes_agree(scan):
             e = rel_l2(imgs[sh], base)
             assert e <= 1e-5, f"shape {sh}: rel-L2 {e:.3e} against the automatic shape"
     job.close()
-    for sp, zs in {(s[0], s[1]) for s in SHAPES}:
-        if (sp, zs, 0) in imgs and (sp, zs, 1) in imgs:
-            assert np.array_equal(imgs[(sp, zs, 0)], imgs[(sp, zs, 1)]), (sp, zs)
+    for sp, zs, rw in {(s[0], s[1], s[3]) for s in SHAPES}:
+        if (sp, zs, 0, rw) in imgs and (sp, zs, 1, rw) in imgs:
+            assert np.array_equal(imgs[(sp, zs, 0, rw)], imgs[(sp, zs, 1, rw)]), (sp, zs, rw)
     zeros = np.zeros(len(ang), np.float32)
     ref, _, _, _ = oracle.aumisart(np.zeros(g.vol_shape), proj, g, ang, zeros, zeros,
                                    np.array([1, 1], np.int32), lam=1.0, nonneg=True)
@@ -75,7 +77,7 @@ def test_autotune_keeps_state(scan):
     job.run()   # non-trivial image and norms
     img0, nrm0 = job.image(), job.norms()
     chosen = job.autotune()
-    assert chosen["proj_split"] in (1, 2, 4, 8)
+    assert chosen["proj_split"] in (1, 2, 4, 8, 16) and chosen["proj_rows"] in (1, 2, 4)
     assert 1 <= chosen["back_zslab"] <= 16 and chosen["back_stage"] in (0, 1)
     assert job.tuning() == chosen
     assert np.array_equal(job.image(), img0)
@@ -88,8 +90,9 @@ def test_autotune_keeps_state(scan):
 
 def test_bad_shapes_rejected(scan):
     job = _job(scan, op_iter=(1,))
-    for bad in [(3, 0, 0), (16, 0, 0), (0, -1, 0), (0, 65, 0), (0, 0, 2)]:
+    for bad in [(3, 0, 0, 0), (32, 0, 0, 0), (0, -1, 0, 0), (0, 65, 0, 0), (0, 0, 2, 0),
+                (0, 0, 0, 3), (0, 0, 0, 8)]:
         with pytest.raises(McsartError):
             job.set_tuning(*bad)
-    assert job.tuning() == {"proj_split": 0, "back_zslab": 0, "back_stage": 0}
+    assert job.tuning() == {"proj_split": 0, "back_zslab": 0, "back_stage": 0, "proj_rows": 0}
     job.close()
