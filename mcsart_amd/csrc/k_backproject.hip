@@ -17,6 +17,8 @@
 //
 // The SART mode reads (r, P1) interleaved as float2 and writes B(r)/B(P1) with NaN/Inf -> 0,
 // fusing src/AUMISART.cu:212-218 (two backprojections and a division) into one pass.
+#include <type_traits>
+
 #include "footprint.hpp"
 #include "internal.hpp"
 
@@ -53,6 +55,16 @@ __device__ inline BLines blines_at(const Column &C, int nz, int iz0) {
     return b;
 }
 
+// Detector value at byte offset `off` from `base`: (r, P1) pairs in the SART mode, (proj, 0) in
+// the plain mode.  32-bit byte offsets (detectors are limited to < 2^29 cells at launch) let the
+// compiler address every load from one scalar base plus one vector register.
+template <int MODE>
+__device__ inline float2 det_at(const void *base, unsigned off) {
+    if (MODE == BACK_PLAIN)
+        return make_float2(*(const float *)((const char *)base + off), 0.f);
+    return *(const float2 *)((const char *)base + off);
+}
+
 template <int MODE, int WA, bool STAGE>
 __global__ void __launch_bounds__(BX *BY) k_backproject(GeomArgs g, double c, double s,
                                                        const float *__restrict__ proj,
@@ -85,11 +97,16 @@ __global__ void __launch_bounds__(BX *BY) k_backproject(GeomArgs g, double c, do
         acount = alim > 0.f ? (int)ceilf(alim) : 0;   // cells ia_lo .. ia_lo + acount - 1
         bl = blines_at(C, g.nz, iz0);
     }
-    // weights of the first WA cells (zero beyond acount); :156
+    // weights of the first WA cells (zero beyond acount); :156.  Window slots past the footprint
+    // read its last cell again (weight zero): every value read is one the footprint uses.
     float wa[WA];
+    int col[WA];
+    const int last = max(ia_lo + acount - 1, ia_lo);
 #pragma unroll
-    for (int k = 0; k < WA; ++k)
+    for (int k = 0; k < WA; ++k) {
         wa[k] = (k < acount) ? fminf(k + 1.f, amaxL) - fmaxf((float)k, aminL) : 0.f;
+        col[k] = min(ia_lo + k, last);
+    }
     const bool wide = acount > WA;   // rare: handled by the generic loop below
 
     // ---- optional LDS staging of the workgroup's detector rectangle
@@ -107,6 +124,8 @@ __global__ void __launch_bounds__(BX *BY) k_backproject(GeomArgs g, double c, do
         __syncthreads();
         int a0 = g.na, a1 = 0, b0 = g.nb, b1 = 0;
         if (live && acount > 0 && iz0 < iz1) {
+            // the slab's rows: its first voxel's lower bound to its last voxel's upper bound
+            // (both bounds grow with z), with the same arithmetic as the voxel loop below
             const float d = (float)(iz1 - 1 - iz0);
             const float lo = fminf(fminf(bl.l0, bl.l1), bl.l3);
             const float hi = fmaxf(fmaxf(fmaf(bl.q0, d, bl.l0), fmaf(bl.q1, d, bl.l1)),
@@ -114,7 +133,7 @@ __global__ void __launch_bounds__(BX *BY) k_backproject(GeomArgs g, double c, do
             a0 = ia_lo;
             a1 = min(g.na, ia_lo + acount);
             b0 = clampi(bl.anchor + (int)floorf(lo), 0, g.nb);
-            b1 = clampi(bl.anchor + (int)ceilf(fminf(hi, 4.f * g.nb)), 0, g.nb);
+            b1 = clampi(bl.anchor + (int)ceilf(fminf(hi, (float)(g.nb - bl.anchor))), 0, g.nb);
         }
         for (int off = 32; off > 0; off >>= 1) {
             a0 = min(a0, __shfl_xor(a0, off, 64));
@@ -133,7 +152,7 @@ __global__ void __launch_bounds__(BX *BY) k_backproject(GeomArgs g, double c, do
         B0 = rB0;
         RA = max(rA1 - A0, 0);
         RB = max(rB1 - B0, 0);
-        staged = RA > 0 && RB > 0 && RA * RB <= LDS_CELLS;
+        staged = RA > 0 && RB > 0 && RA * RB <= LDS_CELLS;   // uniform over the workgroup
         if (staged) {   // coalesced copy of the rectangle, 8 loads in flight per lane
             const int total = RA * RB;
             const float inv = 1.0f / (float)RA;
@@ -143,15 +162,14 @@ __global__ void __launch_bounds__(BX *BY) k_backproject(GeomArgs g, double c, do
 #pragma unroll
                 for (int k = 0; k < 8; ++k) {
                     e[k] = base + k * BX * BY;
-                    int row = (int)((float)e[k] * inv);   // e < 4096: off by at most one
+                    int row = (int)((float)e[k] * inv);   // e < 4096 + 2048: off by at most one
                     row -= (row * RA > e[k]) ? 1 : 0;
                     row += ((row + 1) * RA <= e[k]) ? 1 : 0;
-                    const int col = e[k] - row * RA;
-                    const long long src = (long long)(B0 + min(row, RB - 1)) * g.na + A0 + col;
-                    if (e[k] < total) {
-                        if (MODE == BACK_PLAIN) v[k] = make_float2(proj[src], 0.f);
-                        else v[k] = rp[src];
-                    }
+                    const int cl = e[k] - row * RA;
+                    const unsigned src = (unsigned)((B0 + min(row, RB - 1)) * g.na + A0 + cl);
+                    if (e[k] < total)
+                        v[k] = MODE == BACK_PLAIN ? det_at<MODE>(proj, src * 4u)
+                                                  : det_at<MODE>(rp, src * 8u);
                 }
 #pragma unroll
                 for (int k = 0; k < 8; ++k)
@@ -162,65 +180,92 @@ __global__ void __launch_bounds__(BX *BY) k_backproject(GeomArgs g, double c, do
     }
     if (!live) return;
 
-    auto cell = [&](int ib, int ia) -> float2 {
-        if (STAGE) {
-            const unsigned sr = (unsigned)(ib - B0), sc = (unsigned)(ia - A0);
-            if (staged && sr < (unsigned)RB && sc < (unsigned)RA) return stage[sr * RA + sc];
-        }
-        const long long k = (long long)ib * g.na + ia;
-        if (MODE == BACK_PLAIN) return make_float2(proj[k], 0.f);
-        return rp[k];
-    };
-
-    const long long plane = (long long)g.nx * g.ny;
-    const int last = max(ia_lo + acount - 1, ia_lo);   // clamp for masked window slots
-    for (int iz = iz0; iz < iz1; ++iz) {
-        // axial bounds (:131-132,:137-138) in anchor-relative FP32
-        const float d = (float)(iz - iz0);
-        const float L0 = fmaf(bl.q0, d, bl.l0), L1 = fmaf(bl.q1, d, bl.l1);
-        const float L2 = fmaf(bl.q2, d, bl.l2), L3 = fmaf(bl.q2, d, bl.l3);
-        const float lBmin = fminf(fminf(L0, L1), L3), lBmax = fmaxf(fmaxf(L0, L1), L2);
-        const float flo = floorf(lBmin);
-        const int ib_lo = max(0, bl.anchor + (int)flo);   // :142,:158
-        const float off = (float)(bl.anchor - ib_lo);     // small integer
-        const float bminL = off + lBmin, bmaxL = off + lBmax;
-        // ib < MIN(nb, b_max) (:158); an empty a range contributes nothing
-        float blim = ((float)(g.nb - bl.anchor) < lBmax) ? (float)(g.nb - ib_lo) : bmaxL;
-        if (acount == 0) blim = 0.f;
-        float acc = 0.f, acc1 = 0.f;
-        for (int lb = 0; (float)lb < blim; ++lb) {
-            const float wb = fminf(lb + 1.f, bmaxL) - fmaxf((float)lb, bminL);   // :164
-            const int ib = ib_lo + lb;
+    // Every value the voxels read lies in the staged rectangle when `staged` (uniform over the
+    // workgroup), so the source is chosen once, outside the loops: element
+    // e(ib, ia) = (ib - rb0) * rs + (ia - ra0) of the LDS rectangle or of the detector.
+    const void *gsrc = MODE == BACK_PLAIN ? (const void *)proj : (const void *)rp;
+    constexpr unsigned ES = MODE == BACK_PLAIN ? 4u : 8u;   // bytes per detector element
+    const int plane = g.nx * g.ny;
+    auto run = [&](auto from_lds) {
+        constexpr bool LDSRC = decltype(from_lds)::value;
+        const int rs = LDSRC ? RA : g.na, rb0 = LDSRC ? B0 : 0, ra0 = LDSRC ? A0 : 0;
+        int cof[WA];
+#pragma unroll
+        for (int k = 0; k < WA; ++k) cof[k] = col[k] - ra0;
+        auto value = [&](int e) -> float2 {
+            if (LDSRC) return stage[e];
+            return det_at<MODE>(gsrc, (unsigned)e * ES);
+        };
+        auto row_sum = [&](int r) -> float2 {   // sum_k wa[k] * value(row r, cell col[k])
+            float2 v[WA];
+            const int rowe = r * rs;
+#pragma unroll
+            for (int k = 0; k < WA; ++k) v[k] = value(rowe + cof[k]);
+            float s0 = 0.f, s1 = 0.f;
+#pragma unroll
+            for (int k = 0; k < WA; ++k) {
+                s0 = fmaf(wa[k], v[k].x, s0);
+                s1 = fmaf(wa[k], v[k].y, s1);
+            }
+            return make_float2(s0, s1);
+        };
+        for (int iz = iz0; iz < iz1; ++iz) {
+            // axial bounds (:131-132,:137-138) in anchor-relative FP32
+            const float d = (float)(iz - iz0);
+            const float L0 = fmaf(bl.q0, d, bl.l0), L1 = fmaf(bl.q1, d, bl.l1);
+            const float L2 = fmaf(bl.q2, d, bl.l2), L3 = fmaf(bl.q2, d, bl.l3);
+            const float lBmin = fminf(fminf(L0, L1), L3), lBmax = fmaxf(fmaxf(L0, L1), L2);
+            const int ib_lo = max(0, bl.anchor + (int)floorf(lBmin));   // :142,:158
+            const float off = (float)(bl.anchor - ib_lo);               // small integer
+            const float bminL = off + lBmin, bmaxL = off + lBmax;
+            // rows ib < MIN(nb, b_max) (:158); an empty a range contributes nothing
+            const float top = (float)(g.nb - ib_lo);
+            const float blim = (top < bmaxL) ? top : bmaxL;
+            const int n = (acount > 0 && blim > 0.f) ? (int)ceilf(blim) : 0;
+            // row weight (:164) as a difference of the clamped boundary F(b) = min(max(b, lo), hi)
+            const float bhi = fminf(bmaxL, top);
+            auto F = [&](int b) { return fminf(fmaxf((float)b, bminL), bhi); };
+            float acc = 0.f, acc1 = 0.f;
             if (!wide) {
-                float2 v[WA];
-#pragma unroll
-                for (int k = 0; k < WA; ++k) v[k] = cell(ib, min(ia_lo + k, last));
-                float s0 = 0.f, s1 = 0.f;
-#pragma unroll
-                for (int k = 0; k < WA; ++k) {
-                    s0 = fmaf(wa[k], v[k].x, s0);
-                    s1 = fmaf(wa[k], v[k].y, s1);
+                // two rows per step, all their loads in flight; a second row past the end reads
+                // the first again with weight F(lb+2) - F(lb+1) = 0
+                for (int lb = 0; lb < n; lb += 2) {
+                    const int r0 = ib_lo + lb - rb0;
+                    const int r1 = ib_lo + min(lb + 1, n - 1) - rb0;
+                    const float Fa = F(lb), Fb = F(lb + 1), Fc = F(lb + 2);
+                    const float2 s0 = row_sum(r0), s1 = row_sum(r1);
+                    const float w0 = Fb - Fa, w1 = Fc - Fb;
+                    acc = fmaf(w0, s0.x, acc);
+                    acc1 = fmaf(w0, s0.y, acc1);
+                    acc = fmaf(w1, s1.x, acc);
+                    acc1 = fmaf(w1, s1.y, acc1);
                 }
-                acc = fmaf(wb, s0, acc);
-                acc1 = fmaf(wb, s1, acc1);
             } else {
-                for (int la = 0; la < acount; ++la) {
-                    const float w = fminf(la + 1.f, amaxL) - fmaxf((float)la, aminL);
-                    const float2 v = cell(ib, ia_lo + la);
-                    acc = fmaf(wb, w * v.x, acc);
-                    acc1 = fmaf(wb, w * v.y, acc1);
+                for (int lb = 0; lb < n; ++lb) {
+                    const float wb = F(lb + 1) - F(lb);
+                    const int rowe = (ib_lo + lb - rb0) * rs;
+                    for (int la = 0; la < acount; ++la) {
+                        const float w = fminf(la + 1.f, amaxL) - fmaxf((float)la, aminL);
+                        const float2 v = value(rowe + ia_lo + la - ra0);
+                        acc = fmaf(wb, w * v.x, acc);
+                        acc1 = fmaf(wb, w * v.y, acc1);
+                    }
                 }
             }
+            const int id = ix + iy * g.nx + iz * plane;
+            if (MODE == BACK_PLAIN) {
+                out[id] = acc;
+            } else {
+                float u = acc / acc1;   // src/cu_division.cu:20-24
+                if (isnan(u) || isinf(u)) u = 0.f;
+                out[id] = u;
+            }
         }
-        const long long id = ix + (long long)iy * g.nx + iz * plane;
-        if (MODE == BACK_PLAIN) {
-            out[id] = acc;
-        } else {
-            float u = acc / acc1;   // src/cu_division.cu:20-24
-            if (isnan(u) || isinf(u)) u = 0.f;
-            out[id] = u;
-        }
-    }
+    };
+    if (STAGE && __builtin_amdgcn_readfirstlane((int)staged))
+        run(std::true_type{});
+    else
+        run(std::false_type{});
 }
 
 int pick_zslab(const Geom &g) {
@@ -246,9 +291,14 @@ template <int MODE>
 void launch_mode(dim3 grid, dim3 block, hipStream_t st, const GeomArgs &ga, const ViewTrig &t,
                  int wa, const float *proj, const float2 *rp, float *out, int zs,
                  const NormFinish &nf, bool stage) {
-    if (wa <= 4) launch_wa<MODE, 4>(grid, block, st, ga, t, proj, rp, out, zs, nf, stage);
-    else if (wa <= 6) launch_wa<MODE, 6>(grid, block, st, ga, t, proj, rp, out, zs, nf, stage);
-    else launch_wa<MODE, 8>(grid, block, st, ga, t, proj, rp, out, zs, nf, stage);
+    switch (wa <= 3 ? 3 : wa >= 8 ? 8 : wa) {
+        case 3: launch_wa<MODE, 3>(grid, block, st, ga, t, proj, rp, out, zs, nf, stage); break;
+        case 4: launch_wa<MODE, 4>(grid, block, st, ga, t, proj, rp, out, zs, nf, stage); break;
+        case 5: launch_wa<MODE, 5>(grid, block, st, ga, t, proj, rp, out, zs, nf, stage); break;
+        case 6: launch_wa<MODE, 6>(grid, block, st, ga, t, proj, rp, out, zs, nf, stage); break;
+        case 7: launch_wa<MODE, 7>(grid, block, st, ga, t, proj, rp, out, zs, nf, stage); break;
+        default: launch_wa<MODE, 8>(grid, block, st, ga, t, proj, rp, out, zs, nf, stage); break;
+    }
 }
 
 }  // namespace
@@ -257,6 +307,8 @@ int launch_backprojector(const Geom &g, const ViewTrig &t, const ViewInfo &vi, B
                          const float *proj, const float2 *rp, float *out, hipStream_t st,
                          const NormFinish *nfp, const BackTune *tune) {
     const GeomArgs ga{g.nx, g.ny, g.nz, g.na, g.nb, g.da, g.db, g.ai, g.bi, g.SO, g.SD};
+    if (g.ndet() >= (1LL << 29))
+        return fail(MCSART_ENOTSUP, "backprojector: detectors of 2^29 cells or more");
     const int zs = (tune && tune->zslab > 0) ? tune->zslab : pick_zslab(g);
     const bool stage = tune ? tune->stage != 0 : false;
     const dim3 block(BX, BY);
