@@ -157,6 +157,29 @@ int mcsart_motion_update(float *alpha_x, float *alpha_y, float *alpha_z, float *
                          float *beta_y, float *beta_z, const float *d_u, const float *d_img,
                          float wv, float wf, int nx, int ny, int nz);
 
+/* Replaces host_multiple (src/cu_multiple.cu:2-9, kernel :11-43): img1 *= weight * D_ind(img0),
+ * D_ind the forward difference along dimension ind (1, 2, 3), zero at the last index. */
+int mcsart_multiple(float *img1, int nx, int ny, int nz, const float *img0, float weight, int ind);
+
+/* The DVF inverse of waste/kernel_invertDVF.cu:1-18 (no host code for it ships with the
+ * reference): per voxel, starting from d = 0, niter Gauss-Seidel sweeps
+ * x = -mx(p + d), y = -my(p + d), z = -mz(p + d), each a clamp-to-edge trilinear sample in index
+ * space (dim 0 fastest) at the current d.  Outputs must not alias inputs. */
+int mcsart_invert_dvf(float *mx2, float *my2, float *mz2, const float *mx, const float *my,
+                      const float *mz, int nx, int ny, int nz, int niter);
+
+/* Forward simulator after include/generateProj.h:21-24 (its MEX source is not in the reference):
+ * for each view i, the DVF alpha*volumes[i] + beta*flows[i] (kernel_forwardDVF), replaced by its
+ * inverse (mcsart_invert_dvf, niter_inverse steps) when niter_inverse > 0, warps d_img
+ * (kernel_deformation) and the warped image is projected at angles[i]; d_proj receives
+ * n_views*na*nb floats.  volumes, flows, angles are host arrays; the rest device pointers. */
+int mcsart_generate_proj(float *d_proj, const float *d_img, const float *alpha_x,
+                         const float *alpha_y, const float *alpha_z, const float *beta_x,
+                         const float *beta_y, const float *beta_z, const float *volumes,
+                         const float *flows, const float *angles, int n_views, int niter_inverse,
+                         float SO, float SD, float da, int na, float ai, float db, int nb,
+                         float bi, int nx, int ny, int nz);
+
 /* ---------------------------------------------------------------------------------------------
  * 2b. Asynchronous operators: the same operators enqueued on a context's stream (the context owns
  *     the stream, the scratch buffers and the kernel timers; its device becomes current).  They
@@ -209,6 +232,18 @@ int mcsart_motion_prep_async(mcsart_ctx *ctx, float *d_pairs, const float *d_img
 int mcsart_motion_update_async(mcsart_ctx *ctx, float *alpha_x, float *alpha_y, float *alpha_z,
                                float *beta_x, float *beta_y, float *beta_z, const float *d_u,
                                const float *d_img, float wv, float wf, int nx, int ny, int nz);
+
+int mcsart_multiple_async(mcsart_ctx *ctx, float *img1, int nx, int ny, int nz,
+                          const float *img0, float weight, int ind);
+int mcsart_invert_dvf_async(mcsart_ctx *ctx, float *mx2, float *my2, float *mz2, const float *mx,
+                            const float *my, const float *mz, int nx, int ny, int nz, int niter);
+int mcsart_generate_proj_async(mcsart_ctx *ctx, float *d_proj, const float *d_img,
+                               const float *alpha_x, const float *alpha_y, const float *alpha_z,
+                               const float *beta_x, const float *beta_y, const float *beta_z,
+                               const float *volumes, const float *flows, const float *angles,
+                               int n_views, int niter_inverse, float SO, float SD, float da,
+                               int na, float ai, float db, int nb, float bi, int nx, int ny,
+                               int nz);
 
 /* ---------------------------------------------------------------------------------------------
  * 3. Drivers: host_AUMISART (include/AUMISART.h:17,19), host pointers, same argument order.
@@ -301,6 +336,11 @@ int mcsart_job_get_tuning(mcsart_job *job, mcsart_tuning *t);
  * current image, backprojector in ratio mode into scratch; the job's image, fields and norms are
  * untouched), keep the fastest and report it in *chosen (may be NULL).  Synchronous. */
 int mcsart_job_autotune(mcsart_job *job, mcsart_tuning *chosen);
+/* Average duration (microseconds) of one launch of the projector (kind 0, residual mode) or the
+ * backprojector (kind 1, ratio mode) with the job's launch shapes: every view of the scan is
+ * launched back to back `reps` times between two HIP events on the job's stream (scratch outputs
+ * only; the job's image, fields and norms are untouched).  Synchronous. */
+int mcsart_job_kernel_time(mcsart_job *job, int kind, int reps, double *avg_us);
 
 /* ---------------------------------------------------------------------------------------------
  * 5. Reconstruction server: continuous batching of jobs on one device (no counterpart in the
@@ -361,6 +401,49 @@ int mcsart_server_release(mcsart_server *server, long long id);   /* forget a co
 int mcsart_server_memory(mcsart_server *server, size_t *used, size_t *budget);
 /* Runs every queued request to completion, then stops the scheduler and frees the slots. */
 int mcsart_server_destroy(mcsart_server *server);
+
+/* ---------------------------------------------------------------------------------------------
+ * 6. Gateway parameter parsing: the MATLAB gateways' struct fields, aliases, defaults and
+ *    "MATLAB:badInput" errors (src/entry_AUMISART.cu:18-160, src/mex_backprojection.cu:13-59,
+ *    src/mex_diff.cu:17-37, waste/8888mex_projection.cu:13-126), over a caller-supplied lookup so
+ *    any host (MEX wrapper, file driver, Python) shares one implementation.  A missing required
+ *    field returns MCSART_EINVAL with the reference's message in mcsart_last_error().
+ * ------------------------------------------------------------------------------------------- */
+/* Return 1 and set *value (may be NULL: presence check) if the struct has field `name` (its
+ * first element for arrays, as mxGetScalar), 0 otherwise. */
+typedef int (*mcsart_scalar_fn)(void *user, const char *name, double *value);
+
+enum {
+    MCSART_GW_AUMISART = 0,        /* entry_AUMISART: volume, detector, dx, da, db, ai, bi, SO, SD */
+    MCSART_GW_BACKPROJECTION = 1,  /* mex_backprojection: no dx, plus angle */
+    MCSART_GW_PROJECTION = 2,      /* standalone projection: plus angle; bi shifted only if > -1 */
+    MCSART_GW_DIFF = 3             /* mex_diff: nx, ny, nz, ind */
+};
+enum {   /* bits of `defaulted`: fields absent from the struct that took their default */
+    MCSART_DEF_DX = 1, MCSART_DEF_DA = 2, MCSART_DEF_DB = 4, MCSART_DEF_AI = 8, MCSART_DEF_BI = 16,
+    MCSART_DEF_NITER = 32
+};
+
+typedef struct {
+    int nx, ny, nz, na, nb;
+    float dx, da, db;
+    float ai, bi;      /* internal offsets: user offset - (n/2 - 0.5), in float */
+    float SO, SD;
+    float angle;       /* projection / backprojection gateways */
+    int ind;           /* diff gateway */
+    unsigned defaulted;
+} mcsart_geo_params;
+
+typedef struct {
+    int n_iter, n_views, outIter;
+    float lambda;
+    int motion;        /* 1 if alpha_x is present: the motion overload of host_AUMISART */
+    unsigned defaulted;
+} mcsart_iter_params;
+
+int mcsart_parse_geo(int gateway, mcsart_scalar_fn get, void *user, mcsart_geo_params *out);
+/* Scalars of ITER_PARA; presence of op_iter, volumes, flows, err_weights and angles. */
+int mcsart_parse_iter(mcsart_scalar_fn get, void *user, mcsart_iter_params *out);
 
 /* Per-kernel timing of the last run(s) on this context, from HIP events recorded on the stream
  * the kernels run on.  kind: 0 projector, 1 backprojector, 2 warp, 3 motion elementwise. */

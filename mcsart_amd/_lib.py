@@ -42,6 +42,9 @@ SIGNATURES = {
     "mcsart_division": ([_P, _P, _I, _I, _I], _I),
     "mcsart_diff": ([_P, _P, _I, _I, _I, _I], _I),
     "mcsart_diff_forward": ([_P, _P, _I, _I, _I, _I], _I),
+    "mcsart_multiple": ([_P, _I, _I, _I, _P, _F, _I], _I),
+    "mcsart_invert_dvf": ([_P] * 6 + [_I, _I, _I, _I], _I),
+    "mcsart_generate_proj": ([_P] * 11 + [_I, _I, _F, _F, _F, _I, _F, _F, _I, _F, _I, _I, _I], _I),
     "mcsart_sart_residual": ([_P, _P, _P, _I, _P] + [_F, _F, _F, _F, _I, _F, _F, _I, _F, _I, _I, _I], _I),
     "mcsart_sart_backprojection": ([_P, _P, _F, _F, _F, _F, _I, _F, _F, _I, _F, _I, _I, _I], _I),
     "mcsart_motion_prep": ([_P] * 8 + [_F, _F, _I, _I, _I], _I),
@@ -67,6 +70,7 @@ SIGNATURES = {
     "mcsart_job_set_tuning": ([_P, _P], _I),
     "mcsart_job_get_tuning": ([_P, _P], _I),
     "mcsart_job_autotune": ([_P, _P], _I),
+    "mcsart_job_kernel_time": ([_P, _I, _I, ctypes.POINTER(ctypes.c_double)], _I),
     "mcsart_ctx_enable_timing": ([_P, _I], _I),
     "mcsart_ctx_kernel_time": ([_P, _I, ctypes.POINTER(ctypes.c_double),
                                 ctypes.POINTER(ctypes.c_longlong)], _I),
@@ -79,11 +83,14 @@ SIGNATURES = {
     "mcsart_server_release": ([_P, ctypes.c_longlong], _I),
     "mcsart_server_memory": ([_P, ctypes.POINTER(_SZ), ctypes.POINTER(_SZ)], _I),
     "mcsart_server_destroy": ([_P], _I),
+    "mcsart_parse_geo": ([_I, _P, _P, _P], _I),
+    "mcsart_parse_iter": ([_P, _P, _P], _I),
 }
 # Asynchronous operators: a context, then the synchronous operator's arguments.
 ASYNC_OPS = ("projection", "backprojection", "deform", "deform2", "update_udvf", "add", "add2",
              "initial", "initial2", "division", "diff", "diff_forward", "sart_residual",
-             "sart_backprojection", "motion_prep", "motion_update")
+             "sart_backprojection", "motion_prep", "motion_update", "multiple", "invert_dvf",
+             "generate_proj")
 for _op in ASYNC_OPS:
     SIGNATURES[f"mcsart_{_op}_async"] = ([_P] + SIGNATURES[f"mcsart_{_op}"][0], _I)
 

@@ -18,8 +18,8 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libmcsart_hip.so")
 
-SOURCES = ["capi.hip", "driver.hip", "serve.hip", "k_project.hip", "k_backproject.hip",
-           "k_elem.hip"]
+SOURCES = ["capi.hip", "driver.hip", "serve.hip", "params.hip", "k_project.hip",
+           "k_backproject.hip", "k_elem.hip"]
 
 ARCH = os.environ.get("MCSART_OFFLOAD_ARCH", "gfx950")
 CXXFLAGS = [
@@ -58,9 +58,27 @@ def build(verbose: bool = False) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
+    _build_cli()
     if verbose:
-        print(f"built {LIB}")
+        print(f"built {LIB} and {CLI}")
     return LIB
+
+
+CLI = os.path.join(HERE, "mcsart_recon")
+
+
+def _build_cli() -> None:
+    """The file-driven driver (host code only) next to the library, found through $ORIGIN."""
+    cxx = shutil.which("g++") or shutil.which("c++")
+    if cxx is None:
+        raise RuntimeError("no host C++ compiler for mcsart_recon")
+    cmd = [cxx, "-O2", "-std=c++17", "-Wall", "-I", os.path.join(ROOT, "include"),
+           os.path.join(CSRC, "recon_cli.cpp"), "-o", CLI + ".tmp", "-L", HERE, "-lmcsart_hip",
+           "-Wl,-rpath,$ORIGIN"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"build failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    os.replace(CLI + ".tmp", CLI)
 
 
 if __name__ == "__main__":

@@ -574,6 +574,114 @@ extern "C" int mcsart_diff_forward(float *img1, const float *img, int nx, int ny
 // ---------------------------------------------------------------------------------------------
 // fused SART steps
 // ---------------------------------------------------------------------------------------------
+extern "C" int mcsart_multiple_async(mcsart_ctx *ctx, float *img1, int nx, int ny, int nz,
+                                     const float *img0, float weight, int ind) {
+    int rc = validate_volume(nx, ny, nz, false);
+    if (rc) return rc;
+    if (!img1 || !img0) return fail(MCSART_EINVAL, "null device pointer");
+    if (ind < 1 || ind > 3) return fail(MCSART_EINVAL, "ind must be 1, 2 or 3");
+    CHECK_CTX();
+    return launch_multiple(img1, img0, nx, ny, nz, weight, ind, ctx->stream);
+}
+
+extern "C" int mcsart_multiple(float *img1, int nx, int ny, int nz, const float *img0,
+                               float weight, int ind) {
+    GET_CTX();
+    return finish(ctx, mcsart_multiple_async(ctx, img1, nx, ny, nz, img0, weight, ind));
+}
+
+extern "C" int mcsart_invert_dvf_async(mcsart_ctx *ctx, float *mx2, float *my2, float *mz2,
+                                       const float *mx, const float *my, const float *mz, int nx,
+                                       int ny, int nz, int niter) {
+    int rc = validate_volume(nx, ny, nz, false);
+    if (rc) return rc;
+    if (!mx2 || !my2 || !mz2 || !mx || !my || !mz) return fail(MCSART_EINVAL, "null device pointer");
+    if (niter < 0) return fail(MCSART_EINVAL, "niter must be >= 0");
+    const float *out[3] = {mx2, my2, mz2}, *in[3] = {mx, my, mz};
+    for (const float *o : out)
+        for (const float *i : in)
+            if (o == i) return fail(MCSART_EINVAL, "outputs must not alias inputs");
+    CHECK_CTX();
+    return launch_invert_dvf(mx2, my2, mz2, mx, my, mz, nx, ny, nz, niter, ctx->stream);
+}
+
+extern "C" int mcsart_invert_dvf(float *mx2, float *my2, float *mz2, const float *mx,
+                                 const float *my, const float *mz, int nx, int ny, int nz,
+                                 int niter) {
+    GET_CTX();
+    return finish(ctx, mcsart_invert_dvf_async(ctx, mx2, my2, mz2, mx, my, mz, nx, ny, nz, niter));
+}
+
+extern "C" int mcsart_generate_proj_async(mcsart_ctx *ctx, float *d_proj, const float *d_img,
+                                          const float *alpha_x, const float *alpha_y,
+                                          const float *alpha_z, const float *beta_x,
+                                          const float *beta_y, const float *beta_z,
+                                          const float *volumes, const float *flows,
+                                          const float *angles, int n_views, int niter_inverse,
+                                          float SO, float SD, float da, int na, float ai,
+                                          float db, int nb, float bi, int nx, int ny, int nz) {
+    const Geom g{nx, ny, nz, na, nb, da, db, ai, bi, SO, SD};
+    int rc = validate_geom(g);
+    if (rc) return rc;
+    rc = validate_volume(nx, ny, nz, true);
+    if (rc) return rc;
+    if (!d_proj || !d_img || !alpha_x || !alpha_y || !alpha_z || !beta_x || !beta_y || !beta_z)
+        return fail(MCSART_EINVAL, "null device pointer");
+    if (!volumes || !flows || !angles || n_views <= 0)
+        return fail(MCSART_EINVAL, "n_views and the per-view arrays are required");
+    if (niter_inverse < 0) return fail(MCSART_EINVAL, "niter_inverse must be >= 0");
+    CHECK_CTX();
+    // scratch: forward DVF (3N), inverse DVF (3N), warped image (N), its transpose (N)
+    const size_t N = (size_t)g.nvox();
+    void *tmp = nullptr;
+    rc = ctx_scratch(ctx, sizeof(float) * 8 * N, &tmp);
+    if (rc) return rc;
+    float *m = (float *)tmp, *mi = m + 3 * N, *w = m + 6 * N, *wT = m + 7 * N;
+    const Fields f{alpha_x, alpha_y, alpha_z, beta_x, beta_y, beta_z};
+    for (int iv = 0; iv < n_views; ++iv) {
+        // DVF of the view's breathing state (kernel_forwardDVF)
+        rc = launch_forward_dvf(m, m + N, m + 2 * N, f, nx, ny, nz, volumes[iv], flows[iv],
+                                ctx->stream);
+        if (rc) return rc;
+        const float *u = m;
+        if (niter_inverse > 0) {   // its fixed-point inverse (kernel_invertDVF)
+            rc = launch_invert_dvf(mi, mi + N, mi + 2 * N, m, m + N, m + 2 * N, nx, ny, nz,
+                                   niter_inverse, ctx->stream);
+            if (rc) return rc;
+            u = mi;
+        }
+        // warp by that DVF (kernel_deformation: components by name, dim 1 by my, dim 2 by mx)
+        const Fields uf{u, u + N, u + 2 * N, u, u + N, u + 2 * N};
+        const ViewTrig t = view_trig(angles[iv]);
+        const ViewInfo vi = view_info_of(g, t);
+        rc = launch_warp(vi.has_y ? w : nullptr, vi.has_x ? wT : nullptr, d_img, nx, ny, nz, 1.f,
+                         0.f, uf, ctx->stream);
+        if (rc) return rc;
+        ProjArgs a{};
+        a.vol0 = w;
+        a.vol0T = vi.has_x ? wT : nullptr;
+        a.out = d_proj + (size_t)iv * g.ndet();
+        TimedRegion tr(ctx, TK_PROJ);
+        rc = launch_projector(g, t, vi, PROJ_PLAIN, a, ctx->stream);
+        if (rc) return rc;
+    }
+    return MCSART_OK;
+}
+
+extern "C" int mcsart_generate_proj(float *d_proj, const float *d_img, const float *alpha_x,
+                                    const float *alpha_y, const float *alpha_z,
+                                    const float *beta_x, const float *beta_y, const float *beta_z,
+                                    const float *volumes, const float *flows, const float *angles,
+                                    int n_views, int niter_inverse, float SO, float SD, float da,
+                                    int na, float ai, float db, int nb, float bi, int nx, int ny,
+                                    int nz) {
+    GET_CTX();
+    return finish(ctx, mcsart_generate_proj_async(ctx, d_proj, d_img, alpha_x, alpha_y, alpha_z,
+                                                  beta_x, beta_y, beta_z, volumes, flows, angles,
+                                                  n_views, niter_inverse, SO, SD, da, na, ai, db,
+                                                  nb, bi, nx, ny, nz));
+}
+
 extern "C" int mcsart_sart_residual_async(mcsart_ctx *ctx, float *d_rp, float *d_norm,
                                           const float *d_vol, int motion_pairs, const float *d_meas,
                                           float angle, float SO, float SD, float da, int na,

@@ -530,6 +530,51 @@ extern "C" int mcsart_job_get_tuning(mcsart_job *j, mcsart_tuning *t) {
     return MCSART_OK;
 }
 
+extern "C" int mcsart_job_kernel_time(mcsart_job *j, int kind, int reps, double *avg_us) {
+    int rc = check_job(j);
+    if (rc) return rc;
+    if (!avg_us || reps < 1 || (kind != 0 && kind != 1))
+        return fail(MCSART_EINVAL, "kind must be 0 (projector) or 1 (backprojector), reps >= 1");
+    mcsart_ctx *ctx = j->ctx;
+    if (ctx->timing) return fail(MCSART_EINVAL, "kernel timing while per-kernel events are on");
+    MCSART_HIP_TRY(hipSetDevice(ctx->device));
+    const Geom &g = j->g;
+    const long long D = g.ndet();
+    hipStream_t st = ctx->stream;
+    MCSART_HIP_TRY(hipStreamSynchronize(st));
+    rc = launch_transpose(j->d_w1T, j->d_img, g.nx, g.ny, g.nz, st);   // scratch copy
+    if (rc) return rc;
+    // every view of the scan once per rep, back to back, as the image sweep launches them
+    auto all_views = [&]() {
+        for (int iv = 0; iv < j->V; ++iv) {
+            int r;
+            if (kind == 0) {
+                ProjArgs a{};
+                a.vol0 = j->d_img;
+                a.vol0T = j->vinfo[iv].has_x ? j->d_w1T : nullptr;
+                a.meas = j->d_proj + D * iv;
+                a.rp = j->d_rp;
+                a.partials = j->d_partials;
+                int np = 0;
+                r = launch_projector(g, j->trig[iv], j->vinfo[iv], PROJ_IMAGE, a, st, &np,
+                                     j->proj_split, j->proj_rows);
+            } else {
+                r = launch_backprojector(g, j->trig[iv], j->vinfo[iv], BACK_RATIO, nullptr,
+                                         j->d_rp, j->d_w2, st, nullptr, &j->back_tune);
+            }
+            if (r) return r;
+        }
+        return (int)MCSART_OK;
+    };
+    rc = all_views();   // warm-up
+    if (rc) return rc;
+    float ms = 0.f;
+    rc = time_on_stream(j, reps, all_views, &ms);
+    if (rc) return rc;
+    *avg_us = 1e3 * (double)ms / ((double)reps * j->V);
+    return MCSART_OK;
+}
+
 extern "C" int mcsart_job_autotune(mcsart_job *j, mcsart_tuning *chosen) {
     int rc = check_job(j);
     if (rc) return rc;

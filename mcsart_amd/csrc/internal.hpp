@@ -51,6 +51,17 @@ struct ViewTrig {
 };
 ViewTrig view_trig(float angle);
 
+// ---- XCD-aware workgroup order ----------------------------------------------------------------
+// The device hands consecutive workgroups of a launch to its 8 XCDs in turn, and each XCD has its
+// own L2.  xcd_order maps the hardware workgroup index L (0 <= L < T) to a logical tile index so
+// that every XCD receives one contiguous range of logical tiles (a bijection of [0, T)); kernels
+// lay tiles out so that neighbouring tiles share data.
+constexpr int kXcds = 8;
+__device__ inline int xcd_order(int L, int T) {
+    const int q = T / kXcds, r = T % kXcds, x = L % kXcds;
+    return x * q + min(x, r) + L / kXcds;
+}
+
 // ---- projector ------------------------------------------------------------------------------
 enum ProjMode {
     PROJ_PLAIN = 0,   // out[id] = P(x)
@@ -170,6 +181,16 @@ int launch_motion_update(float *const fields[6], const float *u, const float *x,
 int launch_motion_step(float *const fields[6], const float *u, const float *x, float2 *pair,
                        float2 *pairT, int nx, int ny, int nz, float wv, float wf, float vd,
                        float fd, bool update, bool prep, hipStream_t st);
+
+// img1 *= w * D_ind(img0) (src/cu_multiple.cu:11-43)
+int launch_multiple(float *a, const float *b, int nx, int ny, int nz, float w, int ind,
+                    hipStream_t st);
+// m_c = alpha_c * vol + beta_c * flow into three volumes (src/cu_deform.cu:66-77)
+int launch_forward_dvf(float *mx, float *my, float *mz, const Fields &f, int nx, int ny, int nz,
+                       float vol, float flow, hipStream_t st);
+// fixed-point DVF inverse (waste/kernel_invertDVF.cu:1-18); outputs must not alias inputs
+int launch_invert_dvf(float *mx2, float *my2, float *mz2, const float *mx, const float *my,
+                      const float *mz, int nx, int ny, int nz, int niter, hipStream_t st);
 
 // ---- contexts -------------------------------------------------------------------------------
 enum TimingKind { TK_PROJ = 0, TK_BACK = 1, TK_WARP = 2, TK_MOTION = 3, TK_COUNT = 4 };

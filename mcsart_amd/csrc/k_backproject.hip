@@ -77,10 +77,16 @@ __global__ void __launch_bounds__(BX *BY) k_backproject(GeomArgs g, double c, do
     if (nf.out && blockIdx.x == 0 && blockIdx.y == 0 && blockIdx.z == 0 && tid < 64)
         finish_norm_wave(nf, tid);
 
-    const int ix = blockIdx.x * BX + threadIdx.x;
-    const int iy = blockIdx.y * BY + threadIdx.y;
+    // logical tile, z fastest: each XCD walks whole columns of tiles (a band of detector
+    // columns, every z), which keeps its share of the work even (work grows with |z|)
+    const int tile = xcd_order(blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z),
+                               gridDim.x * gridDim.y * gridDim.z);
+    const int bzl = tile % gridDim.z, rest = tile / gridDim.z;
+    const int bxl = rest % gridDim.x, byl = rest / gridDim.x;
+    const int ix = bxl * BX + threadIdx.x;
+    const int iy = byl * BY + threadIdx.y;
     const bool live = ix < g.nx && iy < g.ny;
-    const int iz0 = blockIdx.z * zslab;
+    const int iz0 = bzl * zslab;
     const int iz1 = min(g.nz, iz0 + zslab);
 
     // ---- column: transverse footprint (a) once, in FP64, clamped so conversions stay in range

@@ -134,6 +134,25 @@ __global__ void k_division(float *a, const float *b, long long n) {
     a[i] = v;
 }
 
+// kernel_multiple (src/cu_multiple.cu:11-43): img1 *= weight * D_ind(img0), forward difference,
+// zero at the last index.
+__global__ void k_multiple(float *__restrict__ a, const float *__restrict__ b, int nx, int ny,
+                           int nz, float w, int ind) {
+    VOXEL_PROLOGUE
+    a[id] *= w * dfwd(b, id, i0, i1, i2, nx, ny, nz, ind);
+}
+
+// kernel_forwardDVF (src/cu_deform.cu:66-77, waste/kernel_forwardDVF.cu): m_c = alpha_c * vol +
+// beta_c * flow.
+__global__ void k_forward_dvf(float *__restrict__ mx, float *__restrict__ my,
+                              float *__restrict__ mz, Fields f, int nx, int ny, int nz, float vol,
+                              float flow) {
+    VOXEL_PROLOGUE
+    mx[id] = f.ax[id] * vol + f.bx[id] * flow;
+    my[id] = f.ay[id] * vol + f.by[id] * flow;
+    mz[id] = f.az[id] * vol + f.bz[id] * flow;
+}
+
 // ---- tile kernels: warps and the fused motion sweep ------------------------------------------
 //
 // A workgroup of 32 x 8 lanes owns a 32 x 32 tile of one z-plane; lane (tx, ty) owns the four
@@ -455,6 +474,30 @@ __global__ void __launch_bounds__(TT *TR) k_transpose(float *__restrict__ dst,
     store_transposed(t, dst, v, nx, ny, X0, Y0, i2);
 }
 
+// Fixed-point inverse of a DVF (waste/kernel_invertDVF.cu:1-18): starting from 0, niter
+// Gauss-Seidel sweeps x <- -m_x(p + d), y <- -m_y(p + d), z <- -m_z(p + d), each component
+// sampled (clamp-to-edge trilinear, index space, dim 0 fastest) at the current estimate
+// d = (x, y, z) with the components updated so far.
+__global__ void k_invert_dvf(float *__restrict__ mx2, float *__restrict__ my2,
+                             float *__restrict__ mz2, const float *__restrict__ mx,
+                             const float *__restrict__ my, const float *__restrict__ mz, int nx,
+                             int ny, int nz, int niter) {
+    const int i0 = blockIdx.x * EX + threadIdx.x;
+    const int i1 = blockIdx.y * EY + threadIdx.y;
+    const int i2 = blockIdx.z;
+    if (i0 >= nx || i1 >= ny) return;
+    float x = 0.f, y = 0.f, z = 0.f;
+    for (int it = 0; it < niter; ++it) {
+        x = -trilinear(mx, nx, ny, nz, i0, x, i1, y, i2, z);
+        y = -trilinear(my, nx, ny, nz, i0, x, i1, y, i2, z);
+        z = -trilinear(mz, nx, ny, nz, i0, x, i1, y, i2, z);
+    }
+    const int id = i0 + i1 * nx + i2 * nx * ny;
+    mx2[id] = x;
+    my2[id] = y;
+    mz2[id] = z;
+}
+
 inline dim3 vgrid(int nx, int ny, int nz) {
     return dim3((nx + EX - 1) / EX, (ny + EY - 1) / EY, nz);
 }
@@ -588,6 +631,30 @@ int launch_motion_step(float *const fields[6], const float *u, const float *x, f
     else if (update) MCSART_MOTION_STEP(true, false);
     else if (prep) MCSART_MOTION_STEP(false, true);
 #undef MCSART_MOTION_STEP
+    MCSART_CHECK_LAUNCH();
+    return MCSART_OK;
+}
+
+int launch_multiple(float *a, const float *b, int nx, int ny, int nz, float w, int ind,
+                    hipStream_t st) {
+    hipLaunchKernelGGL(k_multiple, vgrid(nx, ny, nz), dim3(EX, EY), 0, st, a, b, nx, ny, nz, w,
+                       ind);
+    MCSART_CHECK_LAUNCH();
+    return MCSART_OK;
+}
+
+int launch_forward_dvf(float *mx, float *my, float *mz, const Fields &f, int nx, int ny, int nz,
+                       float vol, float flow, hipStream_t st) {
+    hipLaunchKernelGGL(k_forward_dvf, vgrid(nx, ny, nz), dim3(EX, EY), 0, st, mx, my, mz, f, nx,
+                       ny, nz, vol, flow);
+    MCSART_CHECK_LAUNCH();
+    return MCSART_OK;
+}
+
+int launch_invert_dvf(float *mx2, float *my2, float *mz2, const float *mx, const float *my,
+                      const float *mz, int nx, int ny, int nz, int niter, hipStream_t st) {
+    hipLaunchKernelGGL(k_invert_dvf, vgrid(nx, ny, nz), dim3(EX, EY), 0, st, mx2, my2, mz2, mx,
+                       my, mz, nx, ny, nz, niter);
     MCSART_CHECK_LAUNCH();
     return MCSART_OK;
 }

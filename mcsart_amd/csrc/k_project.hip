@@ -293,8 +293,11 @@ template <int MODE, int WT, int WZ, int R>
 __global__ void __launch_bounds__(RAYS * 16) k_project_rows(GeomArgs g, double c, double s,
                                                            ProjArgs a, int nsplit) {
     const int tx = threadIdx.x, ty = threadIdx.y;
-    const int ia = blockIdx.x * RAYS + tx;
-    const int ib0 = blockIdx.y * R;
+    // logical tile: detector-row bands stay on one XCD (their rays share the same z-slab)
+    const int tile = xcd_order(blockIdx.x + blockIdx.y * gridDim.x, gridDim.x * gridDim.y);
+    const int bxl = tile % gridDim.x, byl = tile / gridDim.x;
+    const int ia = bxl * RAYS + tx;
+    const int ib0 = byl * R;
     const bool live = ia < g.na;
 
     __shared__ RayT rt[RAYS];

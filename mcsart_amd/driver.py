@@ -28,63 +28,79 @@ def _get(d: Mapping, *names):
     return None
 
 
-def parse_geo(geo: Mapping) -> Geometry:
-    """GEO_PARA -> Geometry (src/entry_AUMISART.cu:18-108)."""
-    vals = {}
-    for k in ("nx", "ny", "nz"):
-        v = _get(geo, k)
+_SCALAR_FN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.c_char_p,
+                              ctypes.POINTER(ctypes.c_double))
+
+
+class CGeoParams(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int) for n in ("nx", "ny", "nz", "na", "nb")] + \
+               [(n, ctypes.c_float) for n in ("dx", "da", "db", "ai", "bi", "SO", "SD", "angle")] + \
+               [("ind", ctypes.c_int), ("defaulted", ctypes.c_uint)]
+
+
+class CIterParams(ctypes.Structure):
+    _fields_ = [("n_iter", ctypes.c_int), ("n_views", ctypes.c_int), ("outIter", ctypes.c_int),
+                ("lam", ctypes.c_float), ("motion", ctypes.c_int), ("defaulted", ctypes.c_uint)]
+
+
+GATEWAYS = {"aumisart": 0, "backprojection": 1, "projection": 2, "diff": 3}
+
+
+def _lookup(d: Mapping):
+    """A mcsart_scalar_fn over a mapping: present if the key exists and is not None; the value
+    is the first element, as MATLAB's mxGetScalar."""
+    def fn(_user, name, out):
+        v = d.get(name.decode())
         if v is None:
-            raise MexBadInput(f"Can't found valid volume resolution {k}.")
-        vals[k] = int(v)
-    na = _get(geo, "na", "nv")
-    if na is None:
-        raise MexBadInput("Can't found valid number of detector in plane, which is denoted as na or nu.")
-    nb = _get(geo, "nb", "nu")
-    if nb is None:
-        raise MexBadInput("Can't found valid number of detector across plane, which is denoted as nb or nv.")
-    da = _get(geo, "da")
-    db = _get(geo, "db")
-    SO = _get(geo, "SO", "SI")
-    if SO is None:
-        raise MexBadInput("Can't found valid distance between source and isocenter, which is denoted with para.SO or para.DI.")
-    SD = _get(geo, "SD")
-    if SD is None:
-        DI = _get(geo, "DI")
-        if DI is None:
-            raise MexBadInput("Can't found valid distance between source and detector plane, which is denoted with para.SD or para.SI + para.DI.")
-        SD = np.float32(DI) + np.float32(SO)
-    return Geometry.from_user(vals["nx"], vals["ny"], vals["nz"], int(na), int(nb),
-                              da=1.0 if da is None else da, db=1.0 if db is None else db,
-                              ai=_get(geo, "ai"), bi=_get(geo, "bi"), SO=SO, SD=SD)
+            return 0
+        if out:
+            a = np.asarray(v, dtype=np.float64).ravel()
+            out[0] = float(a[0]) if a.size else 0.0
+        return 1
+    return _SCALAR_FN(fn)
+
+
+def _raise_bad_input(rc, what):
+    msg = load().mcsart_last_error().decode()
+    if msg.startswith("MATLAB:badInput: "):
+        raise MexBadInput(msg[len("MATLAB:badInput: "):])
+    check(rc, what)
+
+
+def parse_geo_params(geo: Mapping, gateway: str = "aumisart") -> CGeoParams:
+    """GEO_PARA of one of the gateways, parsed by the library (mcsart_parse_geo)."""
+    p = CGeoParams()
+    cb = _lookup(geo)
+    rc = load().mcsart_parse_geo(GATEWAYS[gateway], ctypes.cast(cb, ctypes.c_void_p), None,
+                                 ctypes.byref(p))
+    if rc:
+        _raise_bad_input(rc, "mcsart_parse_geo")
+    return p
+
+
+def parse_geo(geo: Mapping, gateway: str = "aumisart") -> Geometry:
+    """GEO_PARA -> Geometry (src/entry_AUMISART.cu:18-108; aliases na|nv, nb|nu, SO|SI,
+    SD|DI+SO; defaults da = db = dx = 1, ai = bi = 0)."""
+    p = parse_geo_params(geo, gateway)
+    return Geometry(p.nx, p.ny, p.nz, p.na, p.nb, float(p.da), float(p.db), float(p.ai),
+                    float(p.bi), float(p.SO), float(p.SD))
 
 
 def parse_iter(it: Mapping):
-    """ITER_PARA (src/entry_AUMISART.cu:111-160)."""
-    n_iter = int(_get(it, "n_iter") or 1)
-    op = _get(it, "op_iter")
-    if op is None:
-        raise MexBadInput("Can't found valid iteration distribution as iter_para.op_iter.")
-    n_views = _get(it, "n_views")
-    if n_views is None:
-        raise MexBadInput("Can't found valid number bins, which is denoted as iter_para.n_views.")
-    lam = _get(it, "lambda")
-    if lam is None:
-        raise MexBadInput("Can't found valid coefficience iter_para.lambda.")
-    if _get(it, "outIter") is None:
-        raise MexBadInput("Can't found valid coefficience iter_para.outIter.")
-    out = {}
-    for k, msg in (("volumes", "volume in iter_para.volumes"), ("flows", "flow in iter_para.flows"),
-                   ("err_weights", "flow in iter_para.err_weights"),
-                   ("angles", "angles iter_para.angles")):
-        v = _get(it, k)
-        if v is None:
-            raise MexBadInput(f"Can't found valid {msg}.")
-        out[k] = np.ascontiguousarray(v, dtype=np.float32)
-    op = np.ascontiguousarray(op, dtype=np.int32)
-    if op.size < n_iter:
+    """ITER_PARA (src/entry_AUMISART.cu:111-160), scalars and presence checks by the library
+    (mcsart_parse_iter); the arrays are taken from the mapping."""
+    p = CIterParams()
+    cb = _lookup(it)
+    rc = load().mcsart_parse_iter(ctypes.cast(cb, ctypes.c_void_p), None, ctypes.byref(p))
+    if rc:
+        _raise_bad_input(rc, "mcsart_parse_iter")
+    out = {k: np.ascontiguousarray(it[k], dtype=np.float32)
+           for k in ("volumes", "flows", "err_weights", "angles")}
+    op = np.ascontiguousarray(it["op_iter"], dtype=np.int32).ravel()
+    if op.size < p.n_iter:
         raise MexBadInput("op_iter has fewer than n_iter entries")
-    return dict(n_iter=n_iter, op_iter=op[:n_iter], n_views=int(n_views), lam=float(lam),
-                outIter=int(_get(it, "outIter")), **out)
+    return dict(n_iter=p.n_iter, op_iter=op[:p.n_iter], n_views=p.n_views, lam=float(p.lam),
+                outIter=p.outIter, motion=bool(p.motion), **out)
 
 
 def entry_aumisart(img0: np.ndarray, proj: np.ndarray, geo: Mapping, it: Mapping):
@@ -102,9 +118,9 @@ def entry_aumisart(img0: np.ndarray, proj: np.ndarray, geo: Mapping, it: Mapping
     args = [out_img.ctypes.data, out_proj.ctypes.data, out_norm.ctypes.data, out_ax.ctypes.data,
             img0.ctypes.data, proj.ctypes.data, g.nx, g.ny, g.nz, g.na, g.nb, p["outIter"], V,
             p["n_iter"], p["op_iter"].ctypes.data, g.da, g.db, g.ai, g.bi, g.SO, g.SD,
-            float(_get(geo, "dx") or 1.0), p["lam"], p["volumes"].ctypes.data,
+            float(parse_geo_params(geo).dx), p["lam"], p["volumes"].ctypes.data,
             p["flows"].ctypes.data, p["err_weights"].ctypes.data, p["angles"].ctypes.data]
-    if _get(it, "alpha_x") is None:
+    if not p["motion"]:
         check(lib.mcsart_aumisart(*args), "mcsart_aumisart")
     else:
         fs = [np.ascontiguousarray(it[k], dtype=np.float32).reshape(g.vol_shape)
@@ -261,6 +277,15 @@ class Job:
         t = CTuning()
         check(self._lib.mcsart_job_autotune(self.ptr, ctypes.byref(t)), "mcsart_job_autotune")
         return t.as_dict()
+
+    def kernel_time(self, kind: str, reps: int = 3) -> float:
+        """Average microseconds per launch of 'projector' or 'backprojector' over all views,
+        launched back to back between two HIP events on the job's stream."""
+        us = ctypes.c_double()
+        check(self._lib.mcsart_job_kernel_time(self.ptr, {"projector": 0, "backprojector": 1}[kind],
+                                               int(reps), ctypes.byref(us)),
+              "mcsart_job_kernel_time")
+        return us.value
 
     def run(self):
         check(self._lib.mcsart_job_run(self.ptr), "mcsart_job_run")

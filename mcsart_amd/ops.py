@@ -215,3 +215,43 @@ def deform(img: np.ndarray, volume, flow, fields: Sequence[np.ndarray]) -> np.nd
     for a in d:
         a.close()
     return out
+
+
+def host_multiple(img1, nx, ny, nz, img0, weight, ind):
+    """``host_multiple`` (``src/cu_multiple.cu:2-9``): img1 *= weight * D_ind(img0)."""
+    check(load().mcsart_multiple(_p(img1), nx, ny, nz, _p(img0), float(weight), int(ind)),
+          "mcsart_multiple")
+
+
+def invert_dvf(mx: np.ndarray, my: np.ndarray, mz: np.ndarray, niter: int):
+    """Fixed-point DVF inverse on the device (mcsart_invert_dvf); returns (mx2, my2, mz2)."""
+    nz, ny, nx = mx.shape
+    ins = [DeviceArray.from_host(a) for a in (mx, my, mz)]
+    outs = [DeviceArray.empty(mx.shape) for _ in range(3)]
+    try:
+        check(load().mcsart_invert_dvf(*[o.ptr for o in outs], *[i.ptr for i in ins], nx, ny, nz,
+                                       int(niter)), "mcsart_invert_dvf")
+        return [o.to_host() for o in outs]
+    finally:
+        for a in ins + outs:
+            a.close()
+
+
+def generate_proj(img: np.ndarray, fields: Sequence[np.ndarray], volumes, flows, angles, g,
+                  niter_inverse: int = 0) -> np.ndarray:
+    """Forward simulator on the device (mcsart_generate_proj); returns (V, nb, na)."""
+    vol, flo, ang = (np.ascontiguousarray(x, dtype=np.float32).ravel()
+                     for x in (volumes, flows, angles))
+    V = int(ang.size)
+    d_f = [DeviceArray.from_host(f) for f in fields]
+    d_img = DeviceArray.from_host(img)
+    d_p = DeviceArray.empty((V, g.nb, g.na))
+    try:
+        check(load().mcsart_generate_proj(
+            d_p.ptr, d_img.ptr, *[f.ptr for f in d_f], vol.ctypes.data, flo.ctypes.data,
+            ang.ctypes.data, V, int(niter_inverse), g.SO, g.SD, g.da, g.na, g.ai, g.db, g.nb,
+            g.bi, g.nx, g.ny, g.nz), "mcsart_generate_proj")
+        return d_p.to_host()
+    finally:
+        for a in d_f + [d_img, d_p]:
+            a.close()

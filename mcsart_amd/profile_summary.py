@@ -1,0 +1,155 @@
+"""Summarise rocprofv3 output of ``mcsart_amd.profile_sweep`` into a per-kernel table.
+
+    python -m mcsart_amd.profile_summary --workload mcsart128 \\
+        --trace gpurun_out/trace_mcsart128 --fetch gpurun_out/pmc_FETCH_SIZE_mcsart128 \\
+        --write gpurun_out/pmc_WRITE_SIZE_mcsart128 --md profiles/x.md --json profiles/x.json
+
+Inputs are rocprofv3 CSV directories: a ``--kernel-trace --stats`` run and one ``--pmc`` run per
+counter (FETCH_SIZE, WRITE_SIZE; KiB per dispatch).  The counters are calibrated against the last
+device-to-device copy of the run (profile_sweep ends with one of --copy-mb MiB): factor =
+true bytes / counted bytes for reads and for writes.  For every kernel the table gives launches,
+average duration, algorithmic bytes per launch (the minimum HBM traffic of its job, 4-byte
+floats, N voxels, D detector cells), achieved GB/s = algorithmic bytes / duration, the fraction
+of the 8 TB/s HBM peak, and the calibrated counter traffic per launch.
+"""
+from __future__ import annotations
+
+import argparse
+import collections
+import csv
+import json
+import os
+import re
+
+from . import configs
+
+HBM_PEAK_GBS = 8000.0
+WORKLOADS = {"mcsart128": configs.c2_mcsart128, "clinical256": configs.c3_clinical,
+             "static64": configs.c1_static64}
+
+
+def short(name: str) -> str:
+    n = name.replace("mcsart::(anonymous namespace)::", "").replace("mcsart::", "")
+    n = n.replace("void ", "")
+    return re.sub(r"\(.*", "", n).strip()
+
+
+def algorithmic_bytes(kernel: str, N: int, D: int):
+    """Minimum bytes one launch must move (reads + writes, 4-byte floats)."""
+    m = re.match(r"k_project(?:_rows)?<(\d)", kernel)
+    if m:
+        mode = int(m.group(1))
+        return {0: 4 * N + 4 * D, 1: 4 * N + 12 * D, 2: 8 * N + 12 * D}[mode]
+    m = re.match(r"k_backproject<(\d)", kernel)
+    if m:
+        return 4 * N + (4 * D if m.group(1) == "0" else 8 * D)
+    table = {
+        "k_warp_tile": 32 * N,        # 6 fields + x read, warped image written
+        "k_warp_add_tile": 36 * N,    # 6 fields + u + x read, x written
+        "k_motion_step<true, true>": 72 * N,    # 6 fields read+written, u, x, pairs
+        "k_motion_step<true, false>": 56 * N,
+        "k_motion_step<false, true>": 40 * N,
+        "k_transpose": 8 * N,
+        "k_add_tile": 12 * N,
+    }
+    return table.get(kernel)
+
+
+def load_stats(d):
+    out = {}
+    for r in csv.DictReader(open(os.path.join(d, "run_kernel_stats.csv"))):
+        out[short(r["Name"])] = (int(r["Calls"]), float(r["AverageNs"]) / 1e3)
+    return out
+
+
+def load_pmc(d):
+    per = collections.defaultdict(list)
+    copies = []
+    for r in csv.DictReader(open(os.path.join(d, "run_counter_collection.csv"))):
+        k = short(r["Kernel_Name"])
+        v = float(r["Counter_Value"]) * 1024.0   # KiB -> bytes
+        per[k].append(v)
+        if "copyBuffer" in k:
+            copies.append((int(r["Dispatch_Id"]), v))
+    calib = max(copies)[1] if copies else None   # the run's last copy is the calibration copy
+    return {k: sum(v) / len(v) for k, v in per.items()}, calib
+
+
+def summarise(workload, trace, fetch, write, copy_mb=64, views=0):
+    spec = WORKLOADS[workload]()
+    g = spec.geometry
+    N, D = g.nvox, g.ndet
+    stats = load_stats(trace)
+    fe, fcal = load_pmc(fetch) if fetch else ({}, None)
+    wr, wcal = load_pmc(write) if write else ({}, None)
+    true_bytes = copy_mb << 20
+    ffac = true_bytes / fcal if fcal else None
+    wfac = true_bytes / wcal if wcal else None
+    rows = []
+    for k, (calls, us) in sorted(stats.items(), key=lambda kv: -kv[1][0] * kv[1][1]):
+        if k.startswith("__amd"):
+            continue
+        alg = algorithmic_bytes(k, N, D)
+        row = {"kernel": k, "launches": calls, "avg_us": round(us, 2),
+               "algorithmic_bytes": alg}
+        if alg:
+            gbs = alg / (us * 1e-6) / 1e9
+            row.update({"achieved_gbs": round(gbs, 1), "frac_hbm_peak": round(gbs / HBM_PEAK_GBS, 4)})
+        if k in fe and ffac:
+            row["fetch_bytes"] = int(fe[k] * ffac)
+        if k in wr and wfac:
+            row["write_bytes"] = int(wr[k] * wfac)
+        if "fetch_bytes" in row and "write_bytes" in row:
+            row["traffic_bytes"] = row["fetch_bytes"] + row["write_bytes"]
+            if alg:
+                row["traffic_over_algorithmic"] = round(row["traffic_bytes"] / alg, 2)
+        rows.append(row)
+    return {"workload": workload, "N": N, "D": D, "views": views or spec.n_views,
+            "fetch_factor": ffac and round(ffac, 4), "write_factor": wfac and round(wfac, 4),
+            "kernels": rows}
+
+
+def to_markdown(s, command_lines):
+    out = [f"# Kernel profile: {s['workload']} (N = {s['N']} voxels, D = {s['D']} detector cells)",
+           "", "Commands (on the MI355X box, from /tmp, PYTHONPATH = repository root):", ""]
+    out += ["    " + c for c in command_lines]
+    out += ["", f"Counter calibration against the run's final 64 MiB device copy: FETCH_SIZE x "
+            f"{s['fetch_factor']}, WRITE_SIZE x {s['write_factor']}.", "",
+            "| kernel | launches | avg us | algorithmic MB | achieved GB/s | frac of 8 TB/s | "
+            "fetch MB | write MB | traffic / algorithmic |",
+            "|---|---:|---:|---:|---:|---:|---:|---:|---:|"]
+    for r in s["kernels"]:
+        mb = lambda b: "" if b is None else f"{b / 1e6:.2f}"
+        out.append(f"| `{r['kernel']}` | {r['launches']} | {r['avg_us']} | "
+                   f"{mb(r.get('algorithmic_bytes'))} | {r.get('achieved_gbs', '')} | "
+                   f"{r.get('frac_hbm_peak', '')} | {mb(r.get('fetch_bytes'))} | "
+                   f"{mb(r.get('write_bytes'))} | {r.get('traffic_over_algorithmic', '')} |")
+    return "\n".join(out) + "\n"
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--workload", required=True, choices=sorted(WORKLOADS))
+    ap.add_argument("--trace", required=True)
+    ap.add_argument("--fetch")
+    ap.add_argument("--write")
+    ap.add_argument("--views", type=int, default=0)
+    ap.add_argument("--md")
+    ap.add_argument("--json")
+    ap.add_argument("--command", action="append", default=[],
+                    help="command line that produced the inputs (repeatable)")
+    args = ap.parse_args(argv)
+    s = summarise(args.workload, args.trace, args.fetch, args.write, views=args.views)
+    if args.json:
+        with open(args.json, "w") as f:
+            json.dump(s, f, indent=1)
+    md = to_markdown(s, args.command)
+    if args.md:
+        with open(args.md, "w") as f:
+            f.write(md)
+    else:
+        print(md)
+
+
+if __name__ == "__main__":
+    main()

@@ -33,6 +33,9 @@ def main(argv=None):
     ap.add_argument("--views", type=int, default=0, help="use only the first V views (0: all)")
     ap.add_argument("--copy-mb", type=int, default=64)
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--tuning", default="",
+                    help="fixed launch shapes proj_split,back_zslab,back_stage,proj_rows "
+                         "(no autotune: the profile then holds the sweep's kernels only)")
     args = ap.parse_args(argv)
     spec = WORKLOADS[args.workload]()
     if args.views:
@@ -48,7 +51,10 @@ def main(argv=None):
     job.set_projections(proj)
     job.set_image(None)
     job.set_fields(data.fields)
-    if not args.no_autotune:
+    if args.tuning:
+        job.set_tuning(*[int(v) for v in args.tuning.split(",")])
+        print(f"[profile_sweep] launch shapes: {job.tuning()}", flush=True)
+    elif not args.no_autotune:
         print(f"[profile_sweep] autotuned launch shapes: {job.autotune()}", flush=True)
     t0 = time.perf_counter()
     job.run()

@@ -65,6 +65,10 @@ def _declare(lib):
         f = getattr(lib, f"oracle_norm2_{sfx}")
         f.argtypes = [_P, _LL]
         f.restype = R
+        getattr(lib, f"oracle_multiple_{sfx}").argtypes = [_P, _I, _I, _I, _P, R, _I]
+        getattr(lib, f"oracle_invert_dvf_{sfx}").argtypes = [_P] * 6 + [_I] * 4
+        getattr(lib, f"oracle_generate_proj_{sfx}").argtypes = (
+            [_P] * 11 + [_I, _I, R, R, R, _I, R, R, _I, R, _I, _I, _I])
         getattr(lib, f"oracle_aumisart_{sfx}").argtypes = (
             [_P] * 6 + [_I] * 7 + [_P] + [R] * 7 + [_P] * 3 + [_P] * 6 + [_I, _I])
     lib.oracle_num_threads.restype = _I
@@ -167,6 +171,42 @@ def add2(a, b, c, w, ind, dtype=np.float64):
     c = _arr(c, dtype).reshape(a.shape)
     getattr(_load(), f"oracle_add2_{_sfx(dtype)}")(_p(a), _p(b), nx, ny, nz, _p(c), float(w), int(ind))
     return a
+
+
+def multiple(a, b, w, ind, dtype=np.float64):
+    """``a * (w * D_ind(b))``, D_ind the forward difference (``src/cu_multiple.cu:11-43``)."""
+    a = _arr(a, dtype).copy()
+    nz, ny, nx = a.shape
+    b = _arr(b, dtype).reshape(a.shape)
+    getattr(_load(), f"oracle_multiple_{_sfx(dtype)}")(_p(a), nx, ny, nz, _p(b), float(w), int(ind))
+    return a
+
+
+def invert_dvf(mx, my, mz, niter, dtype=np.float64):
+    """Fixed-point DVF inverse (``waste/kernel_invertDVF.cu:1-18``); returns (mx2, my2, mz2)."""
+    mx = _arr(mx, dtype)
+    nz, ny, nx = mx.shape
+    my = _arr(my, dtype).reshape(mx.shape)
+    mz = _arr(mz, dtype).reshape(mx.shape)
+    out = [np.empty_like(mx) for _ in range(3)]
+    getattr(_load(), f"oracle_invert_dvf_{_sfx(dtype)}")(
+        *[_p(o) for o in out], _p(mx), _p(my), _p(mz), nx, ny, nz, int(niter))
+    return out
+
+
+def generate_proj(img, fields, volumes, flows, angles, g, niter, dtype=np.float64):
+    """Forward simulator: per view, forwardDVF -> invertDVF (niter > 0) -> warp -> projection
+    (the kernel order of ``include/generateProj.h:21-24``); returns (V, nb, na)."""
+    img = _arr(img, dtype).reshape(g.nz, g.ny, g.nx)
+    fs = _fields(fields, img.shape, dtype)
+    vol, flo, ang = (_arr(x, dtype).ravel() for x in (volumes, flows, angles))
+    V = ang.size
+    out = np.empty((V, g.nb, g.na), dtype=dtype)
+    getattr(_load(), f"oracle_generate_proj_{_sfx(dtype)}")(
+        _p(out), _p(img), *[_p(f) for f in fs], _p(vol), _p(flo), _p(ang), V, int(niter),
+        float(g.SO), float(g.SD), float(g.da), int(g.na), float(g.ai), float(g.db), int(g.nb),
+        float(g.bi), int(g.nx), int(g.ny), int(g.nz))
+    return out
 
 
 def initial2(img0, volume, flow, dtype=np.float64):

@@ -497,6 +497,67 @@ void aumisart(T* outimg, T* outproj, T* outnorm, T* outalphax, const T* h_img, c
 // ---------------------------------------------------------------------------------------------
 // extern "C" surface used by the Python test harness (ctypes).  All arrays are T.
 // ---------------------------------------------------------------------------------------------
+// kernel_multiple (src/cu_multiple.cu:11-43): img1[id] *= weight * D_ind(img0)[id].
+template <typename T>
+void multiple(T* a, int nx, int ny, int nz, const T* b, T w, int ind) {
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int iz = 0; iz < nz; ++iz)
+        for (int iy = 0; iy < ny; ++iy)
+            for (int ix = 0; ix < nx; ++ix) {
+                const long long id = lin(ix, iy, iz, nx, ny);
+                a[id] *= w * dfwd(b, ix, iy, iz, nx, ny, nz, ind);
+            }
+}
+
+// kernel_invertDVF (waste/kernel_invertDVF.cu:1-18): the texture fetch tex3D(m, c + 0.5) with
+// linear filtering and clamp addressing samples m at index-space point c, dim 0 = x fastest.
+// Each component is updated in turn with the components already updated (Gauss-Seidel).
+template <typename T>
+void invert_dvf(T* mx2, T* my2, T* mz2, const T* mx, const T* my, const T* mz, int nx, int ny,
+                int nz, int niter) {
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int iz = 0; iz < nz; ++iz)
+        for (int iy = 0; iy < ny; ++iy)
+            for (int ix = 0; ix < nx; ++ix) {
+                T x = 0, y = 0, z = 0;
+                for (int it = 0; it < niter; ++it) {
+                    x = -trilinear(mx, nx, ny, nz, (T)ix + x, (T)iy + y, (T)iz + z);   // :11
+                    y = -trilinear(my, nx, ny, nz, (T)ix + x, (T)iy + y, (T)iz + z);   // :12
+                    z = -trilinear(mz, nx, ny, nz, (T)ix + x, (T)iy + y, (T)iz + z);   // :13
+                }
+                const long long id = lin(ix, iy, iz, nx, ny);
+                mx2[id] = x;
+                my2[id] = y;
+                mz2[id] = z;
+            }
+}
+
+// Forward simulator in the order include/generateProj.h:21-24 lists its kernels: forwardDVF ->
+// (optional) invertDVF -> deformation -> projection, per view.
+template <typename T>
+void generate_proj(T* proj, const T* img, const T* const fl[6], const T* volumes, const T* flows,
+                   const T* angles, int n_views, int niter, T SO, T SD, T da, int na, T ai, T db,
+                   int nb, T bi, int nx, int ny, int nz) {
+    const long long N = (long long)nx * ny * nz, D = (long long)na * nb;
+    std::vector<T> m(3 * N), mi(3 * N), w(N);
+    const std::vector<T> zero(N, T(0));
+    for (int iv = 0; iv < n_views; ++iv) {
+        for (long long id = 0; id < N; ++id)
+            for (int c = 0; c < 3; ++c)   // kernel_forwardDVF (src/cu_deform.cu:73-76)
+                m[c * N + id] = fl[c][id] * volumes[iv] + fl[c + 3][id] * flows[iv];
+        const T* u = m.data();
+        if (niter > 0) {
+            invert_dvf(mi.data(), mi.data() + N, mi.data() + 2 * N, m.data(), m.data() + N,
+                       m.data() + 2 * N, nx, ny, nz, niter);
+            u = mi.data();
+        }
+        deform(w.data(), img, nx, ny, nz, T(1), T(0), u, u + N, u + 2 * N, zero.data(),
+               zero.data(), zero.data());
+        projection(proj + iv * D, w.data(), angles[iv], SO, SD, da, na, ai, db, nb, bi, nx, ny,
+                   nz);
+    }
+}
+
 #define ORACLE_EXPORTS(T, SFX)                                                                     \
     extern "C" void oracle_projection_##SFX(T* proj, const T* img, T angle, T SO, T SD, T da,      \
                                             int na, T ai, T db, int nb, T bi, int nx, int ny,      \
@@ -541,6 +602,21 @@ void aumisart(T* outimg, T* outproj, T* outnorm, T* outalphax, const T* h_img, c
         diff_backward<T>(o, i, nx, ny, nz, ind);                                                   \
     }                                                                                              \
     extern "C" T oracle_norm2_##SFX(const T* x, long long n) { return norm2<T>(x, n); }            \
+    extern "C" void oracle_multiple_##SFX(T* a, int nx, int ny, int nz, const T* b, T w, int ind) { \
+        multiple<T>(a, nx, ny, nz, b, w, ind);                                                     \
+    }                                                                                              \
+    extern "C" void oracle_invert_dvf_##SFX(T* mx2, T* my2, T* mz2, const T* mx, const T* my,      \
+                                            const T* mz, int nx, int ny, int nz, int niter) {      \
+        invert_dvf<T>(mx2, my2, mz2, mx, my, mz, nx, ny, nz, niter);                               \
+    }                                                                                              \
+    extern "C" void oracle_generate_proj_##SFX(                                                    \
+        T* proj, const T* img, const T* ax, const T* ay, const T* az, const T* bx, const T* by,    \
+        const T* bz, const T* volumes, const T* flows, const T* angles, int n_views, int niter,    \
+        T SO, T SD, T da, int na, T ai, T db, int nb, T bi, int nx, int ny, int nz) {              \
+        const T* const fl[6] = {ax, ay, az, bx, by, bz};                                           \
+        generate_proj<T>(proj, img, fl, volumes, flows, angles, n_views, niter, SO, SD, da, na,    \
+                         ai, db, nb, bi, nx, ny, nz);                                              \
+    }                                                                                              \
     extern "C" void oracle_aumisart_##SFX(                                                         \
         T* outimg, T* outproj, T* outnorm, T* outalphax, const T* h_img, const T* h_proj, int nx,  \
         int ny, int nz, int na, int nb, int n_views, int n_iter, const int* op_iter, T da, T db,   \

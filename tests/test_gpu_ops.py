@@ -256,3 +256,45 @@ def test_async_operators_on_two_contexts(gpu_lib):
             assert np.array_equal(got, ref)
         c.close()
     assert gpu_lib.mcsart_projection_async(None, dp.ptr, dx.ptr, *geo) != 0
+
+
+def test_multiple_parity(gpu_lib):
+    from mcsart_amd.ops import host_multiple
+    shape = (12, 20, 20)
+    a, b = smooth_random(shape, 31), smooth_random(shape, 32)
+    nz, ny, nx = shape
+    for ind in (1, 2, 3):
+        with DeviceArray.from_host(a) as da, DeviceArray.from_host(b) as db:
+            host_multiple(da, nx, ny, nz, db, 0.75, ind)
+            got = da.to_host()
+        assert rel_l2(got, oracle.multiple(a, b, 0.75, ind)) <= TOL
+    with DeviceArray.from_host(a) as da, DeviceArray.from_host(b) as db:
+        assert gpu_lib.mcsart_multiple(da.ptr, nx, ny, nz, db.ptr, 1.0, 4) != 0   # ind 1..3
+
+
+def test_invert_dvf_parity(gpu_lib):
+    from mcsart_amd.ops import invert_dvf
+    shape = (10, 24, 24)
+    m = [2.0 * smooth_random(shape, 40 + k) for k in range(3)]
+    for niter in (1, 5):
+        got = invert_dvf(*m, niter)
+        ref = oracle.invert_dvf(*m, niter)
+        for g_, r_ in zip(got, ref):
+            assert rel_l2(g_, r_) <= TOL, niter
+    c = np.full(shape, 0.75, np.float32)
+    assert np.array_equal(invert_dvf(c, c, c, 3)[0], -c)
+
+
+def test_generate_proj_parity(gpu_lib):
+    from mcsart_amd.ops import generate_proj
+    g = Geometry.from_user(32, 32, 24, 48, 40, da=1.1, db=1.1, SO=120.0, SD=200.0)
+    img = smooth_random(g.vol_shape, 50)
+    fields = random_fields(g.vol_shape, 51)
+    vol = np.array([0.4, -0.8, 1.2, 0.0], np.float32)
+    flo = np.array([0.1, -0.2, 0.3, 0.0], np.float32)
+    ang = np.array([0.0, 0.7, 2.0, 4.4], np.float32)
+    for niter in (0, 3):
+        got = generate_proj(img, fields, vol, flo, ang, g, niter)
+        ref = oracle.generate_proj(img, fields, vol, flo, ang, g, niter)
+        for i in range(len(ang)):
+            assert rel_l2(got[i], ref[i]) <= TOL, (niter, i)

@@ -181,3 +181,30 @@ def test_zero_fields_motion_equals_static(ctx, static64):
         imgs.append(job.image())
         job.close()
     assert np.array_equal(imgs[0], imgs[1])
+
+
+def test_recon_cli_matches_gateway(tmp_path, gpu_lib):
+    """The file driver (raw FP32 + JSON sidecar) gives exactly the entry_AUMISART outputs."""
+    from mcsart_amd import scanio
+    from _util import random_fields, smooth_random
+    V = 10
+    geo = dict(nx=24, ny=24, nz=16, na=40, nb=28, da=1.2, db=1.2, SO=120.0, SD=200.0)
+    g = Geometry.from_user(24, 24, 16, 40, 28, da=1.2, db=1.2, SO=120.0, SD=200.0)
+    ang = uniform_angles(V)
+    vol = np.linspace(-0.5, 0.5, V).astype(np.float32)
+    flo = np.linspace(0.2, -0.2, V).astype(np.float32)
+    fields = [0.2 * f for f in random_fields(g.vol_shape, 61)]
+    ph = smooth_random(g.vol_shape, 62)
+    proj = np.stack([oracle.projection(ph, g, a) for a in ang]).astype(np.float32)
+    it = dict(n_iter=3, op_iter=np.array([1, 0, 1], np.int32), n_views=V, outIter=1,
+              volumes=vol, flows=flo, err_weights=np.ones(V, np.float32), angles=ang,
+              **{"lambda": 1.0})
+    js = scanio.write_scan(str(tmp_path / "scan"), geo, it, proj, fields=fields)
+    r = scanio.run_cli(js, str(tmp_path / "out"))
+    assert r.returncode == 0, r.stderr
+    got = scanio.read_outputs(str(tmp_path / "out"), 24, 24, 16, 40, 28, V, 3)
+    it_m = dict(it, alpha_x=fields[0], alpha_y=fields[1], alpha_z=fields[2], beta_x=fields[3],
+                beta_y=fields[4], beta_z=fields[5])
+    ref = entry_aumisart(np.zeros(g.vol_shape, np.float32), proj, geo, it_m)
+    for a, b in zip(got, ref):
+        assert np.array_equal(a, b.reshape(a.shape))

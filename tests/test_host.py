@@ -1,6 +1,7 @@
 """Host-side checks that need no GPU: the C-ABI library loads and exports every declared symbol,
 the entry_AUMISART parameter parsing, geometry conversion and the seeded synthetic data."""
 import ctypes
+import json
 import os
 import re
 
@@ -148,3 +149,69 @@ def test_ctypes_structs_match_header(tmp_path):
         want.append(ctypes.sizeof(py))
         want.extend(getattr(py, f).offset for f, _ in py._fields_)
     assert got == want
+
+
+def _scan_dicts(V=6, motion=False):
+    geo = dict(nx=16, ny=16, nz=8, nv=24, nu=20, da=1.5, db=1.25, SI=90.0, DI=60.0, ai=0.5,
+               dx=2.0)
+    it = dict(n_iter=3, op_iter=[1, 0, 1], n_views=V, **{"lambda": 0.8}, outIter=1,
+              volumes=np.linspace(0, 1, V), flows=np.zeros(V), err_weights=np.ones(V),
+              angles=np.linspace(0, 3, V))
+    return geo, it
+
+
+def test_recon_cli_dry_run(tmp_path):
+    """The file driver parses the sidecar with the gateway's rules (aliases nv/nu, SI/DI,
+    offsets, defaults) and checks the raw file sizes, without touching the GPU."""
+    from mcsart_amd import scanio
+    from mcsart_amd.driver import parse_geo
+    geo, it = _scan_dicts()
+    proj = np.zeros((6, 20, 24), np.float32)
+    js = scanio.write_scan(str(tmp_path / "s"), geo, it, proj)
+    r = scanio.run_cli(js, str(tmp_path / "o"), dry_run=True)
+    assert r.returncode == 0, r.stderr
+    got = json.loads(r.stdout)
+    g = parse_geo(geo)
+    assert (got["nx"], got["na"], got["nb"]) == (16, 24, 20)
+    assert np.float32(got["ai"]) == np.float32(g.ai) and np.float32(got["bi"]) == np.float32(g.bi)
+    assert np.float32(got["SD"]) == np.float32(150.0) and got["dx"] == 2.0
+    assert got["op_iter"] == [1, 0, 1] and got["motion"] == 0
+    assert got["defaulted"] == 16   # bi only
+    # a missing required field: the gateway's message, non-zero exit
+    del geo["nz"]
+    js = scanio.write_scan(str(tmp_path / "s2"), geo, it, proj)
+    r = scanio.run_cli(js, str(tmp_path / "o2"), dry_run=True)
+    assert r.returncode != 0 and "nz" in r.stderr
+    # a projection file of the wrong size
+    geo, it = _scan_dicts()
+    js = scanio.write_scan(str(tmp_path / "s3"), geo, it, proj[:5])
+    r = scanio.run_cli(js, str(tmp_path / "o3"), dry_run=True)
+    assert r.returncode != 0 and "bytes" in r.stderr
+
+
+def test_gateway_parser_rules():
+    """mcsart_parse_geo / mcsart_parse_iter: per-gateway fields, the projection gateway's
+    'bi > -1' rule, default bits, and the reference's check order for ITER_PARA."""
+    from mcsart_amd.driver import MexBadInput, parse_geo_params, parse_iter
+    base = dict(nx=8, ny=8, nz=4, na=10, nb=12, SO=100.0, SD=150.0)
+    p = parse_geo_params(base)
+    assert p.defaulted == 1 | 2 | 4 | 8 | 16 and p.dx == 1.0
+    assert p.ai == np.float32(-(10 / 2 - 0.5)) and p.bi == np.float32(-(12 / 2 - 0.5))
+    for gw, bi_user, want in (("projection", -2.0, -2.0), ("projection", 0.0, -5.5),
+                              ("backprojection", -2.0, -7.5), ("aumisart", -2.0, -7.5)):
+        q = parse_geo_params(dict(base, bi=bi_user, angle=0.5), gw)
+        assert q.bi == np.float32(want), (gw, bi_user)
+    with pytest.raises(MexBadInput, match="angle"):
+        parse_geo_params(base, "backprojection")
+    assert parse_geo_params(dict(nx=3, ny=4, nz=5, ind=2), "diff").ind == 2
+    it = dict(op_iter=[1], n_views=2, outIter=1, volumes=[0, 0], flows=[0, 0],
+              err_weights=[1, 1], angles=[0, 1])
+    with pytest.raises(MexBadInput, match="lambda"):
+        parse_iter(it)
+    it["lambda"] = 1.0
+    p = parse_iter(it)
+    assert p["n_iter"] == 1 and not p["motion"]
+    del it["op_iter"]
+    del it["n_views"]   # op_iter is checked first (src/entry_AUMISART.cu:119-128)
+    with pytest.raises(MexBadInput, match="op_iter"):
+        parse_iter(it)

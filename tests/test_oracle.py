@@ -341,3 +341,73 @@ def test_zero_fields_overloads_agree():
     a = oracle.aumisart(np.zeros(g.vol_shape), proj, g, ang, alpha, beta, [1, 1])
     b = oracle.aumisart(np.zeros(g.vol_shape), proj, g, ang, alpha, beta, [1, 1], fields=zero)
     assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+
+
+# ---- host_multiple, DVF inverse, forward simulator ---------------------------------------------
+
+def test_multiple_closed_form():
+    """img1 * (w * forward difference of img0 along ind, zero at the last index)."""
+    rng = np.random.default_rng(3)
+    a, b = rng.random((5, 6, 7)), rng.random((5, 6, 7))
+    for ind, axis in ((1, 2), (2, 1), (3, 0)):
+        d = np.diff(b, axis=axis, append=np.take(b, [-1], axis=axis))   # zero at the end
+        np.testing.assert_allclose(oracle.multiple(a, b, 0.7, ind), a * (0.7 * d), rtol=0,
+                                   atol=1e-15)
+
+
+def test_invert_dvf_zero_and_constant():
+    z = np.zeros((4, 5, 6))
+    for out in oracle.invert_dvf(z, z, z, 4):
+        assert np.array_equal(out, z)
+    c = np.ones((4, 5, 6))
+    mx2, my2, mz2 = oracle.invert_dvf(0.3 * c, -1.25 * c, 2.0 * c, 3)
+    assert np.array_equal(mx2, -0.3 * c) and np.array_equal(my2, 1.25 * c)
+    assert np.array_equal(mz2, -2.0 * c)
+
+
+def test_invert_dvf_linear_field_geometric_series():
+    """m_x = a*(ix - c0): x_k = -a*(ix + x_{k-1} - c0) from x_0 = 0, so
+    x_k = x* (1 - (-a)^k) with x* = -a (ix - c0) / (1 + a), exactly where no sample is clamped
+    (trilinear interpolation reproduces linear functions)."""
+    nz, ny, nx = 3, 4, 24
+    a, c0, k = 0.2, 11.0, 5
+    ix = np.arange(nx, dtype=np.float64)
+    mx = np.broadcast_to(a * (ix - c0), (nz, ny, nx)).copy()
+    z = np.zeros_like(mx)
+    got = oracle.invert_dvf(mx, z, z, k)[0][1, 1]
+    xs = -a * (ix - c0) / (1 + a) * (1 - (-a) ** k)
+    interior = slice(4, 20)
+    np.testing.assert_allclose(got[interior], xs[interior], rtol=0, atol=1e-12)
+
+
+def test_invert_dvf_gauss_seidel_order():
+    """y is sampled at the x just computed (not the previous sweep's): with m_x = c and
+    m_y = b*ix, one sweep gives x = -c and y = -b*(ix - c)."""
+    nz, ny, nx = 2, 3, 16
+    b, c = 0.25, 0.5
+    ix = np.arange(nx, dtype=np.float64)
+    mx = np.full((nz, ny, nx), c)
+    my = np.broadcast_to(b * ix, (nz, ny, nx)).copy()
+    x, y, _ = oracle.invert_dvf(mx, my, np.zeros_like(mx), 1)
+    np.testing.assert_array_equal(x, -mx)
+    np.testing.assert_allclose(y[0, 1, 2:-2], -b * (ix[2:-2] - c), rtol=0, atol=1e-14)
+
+
+def test_generate_proj_composition():
+    """Without inversion the simulator is the projection of the warped image per view; with zero
+    fields it is the plain projection."""
+    g = Geometry.from_user(16, 16, 12, 24, 20, da=1.2, db=1.2, SO=60.0, SD=100.0)
+    rng = np.random.default_rng(5)
+    img = rng.random(g.vol_shape)
+    fields = [0.3 * rng.standard_normal(g.vol_shape) for _ in range(6)]
+    vol, flo = np.array([0.5, -1.0, 1.5]), np.array([0.1, 0.2, -0.3])
+    ang = np.array([0.0, 1.1, 2.5])
+    p = oracle.generate_proj(img, fields, vol, flo, ang, g, 0)
+    for i in range(3):
+        w = oracle.deform(img, vol[i], flo[i], fields)
+        np.testing.assert_allclose(p[i], oracle.projection(w, g, ang[i]), rtol=1e-12, atol=1e-12)
+    zf = [np.zeros(g.vol_shape)] * 6
+    p0 = oracle.generate_proj(img, zf, vol, flo, ang, g, 4)
+    for i in range(3):
+        np.testing.assert_allclose(p0[i], oracle.projection(img, g, ang[i]), rtol=1e-12,
+                                   atol=1e-12)
