@@ -64,6 +64,69 @@ def profiled_traffic(workload, kind):
     return int(best[0]["traffic_bytes"])
 
 
+def concurrent_scans(ctx, spec, g, data, proj, tuning, jobs, rounds):
+    """Throughput with `jobs` copies of the scan resident on separate streams, all started
+    together, `rounds` times; returns scans/hour and the per-round wall time."""
+    if jobs < 2:
+        return None
+    from mcsart_amd.driver import Context, Job
+    ctxs = [Context(0) for _ in range(jobs)]
+    js = []
+    for c in ctxs:
+        j = Job(c, g, spec.angles, data.alpha, data.beta, spec.op_iter, lam=spec.lam,
+                motion=spec.motion, nonneg=spec.nonneg)
+        j.set_projections(proj)
+        j.set_image(None)
+        j.set_fields(data.fields)
+        j.set_tuning(**tuning)
+        js.append(j)
+    for j in js:   # warm-up: captures every job's graphs
+        j.reset()
+        j.run()
+    for c in ctxs:
+        c.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(rounds):
+        for j in js:
+            j.reset()
+            j.run()
+        for c in ctxs:
+            c.synchronize()
+    el = time.perf_counter() - t0
+    for j in js:
+        j.close()
+    for c in ctxs:
+        c.close()
+    return {"jobs_in_flight": jobs, "scans_per_hour": round(3600.0 * jobs * rounds / el, 1),
+            "ms_per_round": round(el * 1e3 / rounds, 2)}
+
+
+def host_boundary_ms(spec, g, data, proj):
+    """Wall time of one mcsart_aumisart_motion call (the host_AUMISART replacement): upload,
+    schedule, download, with allocation inside the call."""
+    import ctypes
+    from mcsart_amd._lib import check, load
+    lib = load()
+    V = spec.n_views
+    op = np.ascontiguousarray(spec.op_iter, np.int32)
+    f32 = lambda a: np.ascontiguousarray(a, np.float32)
+    img0 = np.zeros(g.vol_shape, np.float32)
+    out_img, out_ax = np.empty_like(img0), np.empty_like(img0)
+    out_nrm = np.empty(len(op) * V, np.float32)
+    out_res = np.empty((V,) + g.det_shape, np.float32)
+    vol, flo, ang = f32(data.alpha), f32(data.beta), f32(spec.angles)
+    ew = np.ones(V, np.float32)
+    fl = [f32(f) for f in data.fields]
+    t0 = time.perf_counter()
+    check(lib.mcsart_aumisart_motion(
+        out_img.ctypes.data, out_res.ctypes.data, out_nrm.ctypes.data, out_ax.ctypes.data,
+        img0.ctypes.data, f32(proj).ctypes.data, g.nx, g.ny, g.nz, g.na, g.nb, 1, V, len(op),
+        op.ctypes.data, g.da, g.db, g.ai, g.bi, g.SO, g.SD, 1.0, spec.lam, vol.ctypes.data,
+        flo.ctypes.data, ew.ctypes.data, ang.ctypes.data, *[f.ctypes.data for f in fl]),
+        "mcsart_aumisart_motion")
+    return (time.perf_counter() - t0) * 1e3
+
+
 def launches_per_scan(spec):
     n_img = sum(1 for o in spec.op_iter if o == 1)
     n_mot = len(spec.op_iter) - n_img
@@ -108,6 +171,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--cpu-sample-views", type=int, default=12)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--concurrent", type=int, default=4,
+                    help="also measure this many scans in flight at once (0: skip)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -195,6 +260,16 @@ def main():
     kernel_ms = {k: {"total_ms": round(v[0], 2), "launches": v[1],
                      "avg_us": round(v[0] * 1e3 / max(v[1], 1), 2)} for k, v in times.items()}
 
+    # Several copies of the scan in flight at once, each a resident job on its own stream.
+    conc = concurrent_scans(ctx, spec, g, data, proj, tuning, args.concurrent, K)
+    if conc:
+        log(f"[bench] {args.concurrent} scans in flight: {conc}")
+
+    # The drop-in boundary (host_AUMISART: host buffers in and out, device memory allocated and
+    # freed per call): one whole scan through it, PCIe transfers included.  Not the headline value.
+    host_ms = host_boundary_ms(spec, g, data, proj)
+    log(f"[bench] host-buffer driver call: {host_ms:.1f} ms per scan")
+
     cpu = None
     if not args.no_cpu_baseline:
         log("[bench] timing the FP64 CPU oracle on a bounded sample ...")
@@ -224,6 +299,8 @@ def main():
         "kernels_note": "eager pass with HIP events around every launch (includes event overhead)",
         "eager_ms_per_scan_with_events": round(eager_ms, 2),
         "isolated_launch_us": {k: round(v, 2) for k, v in iso_us.items()},
+        "host_boundary_ms_per_scan": round(host_ms, 1),
+        "concurrent": conc,
         "launch": "one HIP graph per sweep",
         "tuning": tuning,
         "roofline": roofline,

@@ -94,6 +94,12 @@ inline void host_diff(float *img1, float *img, int nx, int ny, int nz, int ind) 
     MCSART_COMPAT_CALL(mcsart_diff(img1, img, nx, ny, nz, ind), "host_diff");
 }
 
+// include/cu_multiple.h (src/cu_multiple.cu:2)
+inline void host_multiple(float *img1, int nx, int ny, int nz, float *img0, float weight,
+                          int ind) {
+    MCSART_COMPAT_CALL(mcsart_multiple(img1, nx, ny, nz, img0, weight, ind), "host_multiple");
+}
+
 // include/AUMISART.h:17 (overload 1, static) and :19 (overload 2, motion-compensated)
 inline void host_AUMISART(float *h_outimg, float *h_outproj, float *h_outnorm, float *h_outalphax,
                           float *h_img, float *h_proj, int nx, int ny, int nz, int na, int nb,

@@ -323,11 +323,14 @@ int mcsart_job_device_image(mcsart_job *job, float **d_img);
 int mcsart_job_destroy(mcsart_job *job);
 
 /* Launch shapes of the two footprint kernels (the reference has fixed 16x16 blocks,
- * src/cu_projection.cu:140, src/cu_backprojection.cu:186).  Zero fields mean automatic. */
+ * src/cu_projection.cu:140, src/cu_backprojection.cu:186).  Zero fields mean automatic: shapes
+ * chosen from the geometry alone, so a given scan always runs the same kernels.  The shapes
+ * change floating-point rounding (plane-chunk sums, slab anchors) but not the mathematics. */
 typedef struct {
     int proj_split;   /* projector workgroup slices along each ray: 0, 1, 2, 4, 8 or 16 */
     int back_zslab;   /* backprojector voxels per lane along z: 0 or 1..64 */
-    int back_stage;   /* 1: backprojector workgroups stage their detector rectangle in LDS */
+    int back_stage;   /* backprojector detector reads: 0 automatic (LDS-staged rectangle),
+                         1 staged in LDS, 2 straight from global memory */
     int proj_rows;    /* detector rows (rays of one column) per projector lane: 0, 1, 2 or 4 */
 } mcsart_tuning;
 int mcsart_job_set_tuning(mcsart_job *job, const mcsart_tuning *t);

@@ -475,8 +475,8 @@ int check_tuning(const mcsart_tuning &t) {
         return fail(MCSART_EINVAL, "proj_rows must be 0, 1, 2 or 4");
     if (t.back_zslab < 0 || t.back_zslab > 64)
         return fail(MCSART_EINVAL, "back_zslab must be in 0..64");
-    if (t.back_stage != 0 && t.back_stage != 1)
-        return fail(MCSART_EINVAL, "back_stage must be 0 or 1");
+    if (t.back_stage < 0 || t.back_stage > 2)
+        return fail(MCSART_EINVAL, "back_stage must be 0, 1 or 2");
     return MCSART_OK;
 }
 
@@ -634,7 +634,7 @@ extern "C" int mcsart_job_autotune(mcsart_job *j, mcsart_tuning *chosen) {
             }
         }
     best_ms = 3.4e38f;
-    for (int stage = 0; stage <= 1; ++stage)
+    for (int stage = 1; stage <= 2; ++stage)
         for (int zs = 1; zs <= 16 && zs <= g.nz; zs *= 2) {
             const BackTune bt{zs, stage};
             rc = backproject(bt);

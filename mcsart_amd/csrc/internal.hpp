@@ -123,8 +123,9 @@ enum BackMode {
     BACK_PLAIN = 0,  // out = B(proj)
     BACK_RATIO = 1,  // out = B(rp.x) / B(rp.y), NaN/Inf -> 0
 };
-// Launch shape of the backprojector: voxels per workgroup along z (0 = automatic) and whether
-// the workgroup stages its detector rectangle in LDS.
+// Launch shape of the backprojector: voxels per lane along z (0 = automatic) and whether the
+// workgroup stages its detector rectangle in LDS (0 = automatic: staged, 1 = staged, 2 = read
+// the detector from global memory).
 struct BackTune {
     int zslab;
     int stage;

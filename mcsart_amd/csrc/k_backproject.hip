@@ -274,10 +274,11 @@ __global__ void __launch_bounds__(BX *BY) k_backproject(GeomArgs g, double c, do
         run(std::false_type{});
 }
 
+// Voxels per lane along z: the longest slab (up to 8) that still leaves >= 1024 workgroups.
 int pick_zslab(const Geom &g) {
     const long long cols = (long long)((g.nx + BX - 1) / BX) * ((g.ny + BY - 1) / BY);
-    int zs = 4;
-    while (zs > 1 && cols * ((g.nz + zs - 1) / zs) < 2048) zs /= 2;
+    int zs = 8;
+    while (zs > 1 && cols * ((g.nz + zs - 1) / zs) < 1024) zs /= 2;
     return zs;
 }
 
@@ -316,7 +317,7 @@ int launch_backprojector(const Geom &g, const ViewTrig &t, const ViewInfo &vi, B
     if (g.ndet() >= (1LL << 29))
         return fail(MCSART_ENOTSUP, "backprojector: detectors of 2^29 cells or more");
     const int zs = (tune && tune->zslab > 0) ? tune->zslab : pick_zslab(g);
-    const bool stage = tune ? tune->stage != 0 : false;
+    const bool stage = !tune || tune->stage != 2;   // 0 automatic (staged), 1 staged, 2 global
     const dim3 block(BX, BY);
     const dim3 grid((g.nx + BX - 1) / BX, (g.ny + BY - 1) / BY, (g.nz + zs - 1) / zs);
     const NormFinish nf = nfp ? *nfp : NormFinish{nullptr, 0, nullptr};

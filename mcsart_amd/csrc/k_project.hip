@@ -510,22 +510,24 @@ int launch_mode(const Geom &g, const ViewTrig &t, const ViewInfo &vi, const Proj
     return MCSART_OK;
 }
 
-// Rows kernel shape: R rays per lane and S plane chunks per workgroup, enough wavefronts to fill
-// the device (>= 4096) with at least 4 planes per chunk.
+// Rows kernel shape: R rays per lane and S plane chunks per workgroup.  Automatic choice: the
+// most rays per lane for which chunks of at least 16 planes still give >= 4096 wavefronts (the
+// shapes the timing autotuner picks on 128^3 / 256x256x128 scans: R = 1, S = 8 / R = 4, S = 8).
 void pick_rows(const Geom &g, int rows, int split, int &R, int &S) {
     const int np = g.nx > g.ny ? g.nx : g.ny;
     const long long cols = (g.na + RAYS - 1) / RAYS;
     auto split_for = [&](int r) {
         const long long tiles = cols * ((g.nb + r - 1) / r);
         int sp = 1;
-        while (sp < 16 && tiles * sp < 4096 && np / (2 * sp) >= 4) sp *= 2;
+        while (sp < 16 && tiles * sp < 4096 && np / (2 * sp) >= 16) sp *= 2;
         return sp;
     };
-    R = rows > 0 ? rows : 0;
-    if (!R) {   // the most rows per lane that still fills the device
+    auto waves = [&](int r) { return cols * ((g.nb + r - 1) / r) * split_for(r); };
+    R = rows;
+    if (!R) {
         R = 1;
         for (int r = 4; r >= 2; r /= 2)
-            if (cols * ((g.nb + r - 1) / r) * split_for(r) >= 4096) {
+            if (waves(r) >= 4096) {
                 R = r;
                 break;
             }

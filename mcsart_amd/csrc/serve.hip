@@ -229,7 +229,7 @@ extern "C" int mcsart_server_create(int device, const mcsart_server_opts *o,
     const mcsart_tuning t{o->proj_split, o->back_zslab, o->back_stage, o->proj_rows};
     if (t.proj_split < 0 || t.proj_split > 16 || (t.proj_split & (t.proj_split - 1)) ||
         (t.proj_rows != 0 && t.proj_rows != 1 && t.proj_rows != 2 && t.proj_rows != 4) ||
-        t.back_zslab < 0 || t.back_zslab > 64 || (t.back_stage != 0 && t.back_stage != 1))
+        t.back_zslab < 0 || t.back_zslab > 64 || t.back_stage < 0 || t.back_stage > 2)
         return fail(MCSART_EINVAL, "bad fixed launch shapes");
     MCSART_HIP_TRY(hipSetDevice(device));
     auto *sv = new mcsart_server();

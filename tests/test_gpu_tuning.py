@@ -2,7 +2,7 @@
 
 The projector's split along each ray changes only the order of its partial sums, the
 backprojector's z-slab only where its axial bound lines are anchored, and LDS staging only where
-its detector values are read from, so all shapes agree to float rounding (staging on/off: bit for
+its detector values are read from, so all shapes agree to float rounding (staged or not: bit for
 bit), and all of them match the FP64 oracle to the reconstruction bar.
 """
 import numpy as np
@@ -17,8 +17,9 @@ from _util import rel_l2
 
 pytestmark = pytest.mark.gpu
 # (proj_split, back_zslab, back_stage, proj_rows)
-SHAPES = [(1, 1, 0, 1), (1, 1, 1, 1), (8, 16, 0, 2), (8, 16, 1, 2), (2, 4, 1, 4), (4, 2, 0, 4),
-          (16, 64, 1, 4), (16, 8, 0, 1), (0, 0, 0, 1), (0, 0, 0, 2)]
+# back_stage: 0 automatic (staged), 1 staged in LDS, 2 detector read from global memory
+SHAPES = [(1, 1, 2, 1), (1, 1, 1, 1), (8, 16, 2, 2), (8, 16, 1, 2), (2, 4, 1, 4), (4, 2, 2, 4),
+          (16, 64, 1, 4), (16, 8, 2, 1), (0, 0, 0, 1), (0, 0, 0, 2)]
 
 
 @pytest.fixture(scope="module")
@@ -63,8 +64,8 @@ def test_shapes_agree(scan):
             assert e <= 1e-5, f"shape {sh}: rel-L2 {e:.3e} against the automatic shape"
     job.close()
     for sp, zs, rw in {(s[0], s[1], s[3]) for s in SHAPES}:
-        if (sp, zs, 0, rw) in imgs and (sp, zs, 1, rw) in imgs:
-            assert np.array_equal(imgs[(sp, zs, 0, rw)], imgs[(sp, zs, 1, rw)]), (sp, zs, rw)
+        if (sp, zs, 1, rw) in imgs and (sp, zs, 2, rw) in imgs:
+            assert np.array_equal(imgs[(sp, zs, 1, rw)], imgs[(sp, zs, 2, rw)]), (sp, zs, rw)
     zeros = np.zeros(len(ang), np.float32)
     ref, _, _, _ = oracle.aumisart(np.zeros(g.vol_shape), proj, g, ang, zeros, zeros,
                                    np.array([1, 1], np.int32), lam=1.0, nonneg=True)
@@ -78,7 +79,7 @@ def test_autotune_keeps_state(scan):
     img0, nrm0 = job.image(), job.norms()
     chosen = job.autotune()
     assert chosen["proj_split"] in (1, 2, 4, 8, 16) and chosen["proj_rows"] in (1, 2, 4)
-    assert 1 <= chosen["back_zslab"] <= 16 and chosen["back_stage"] in (0, 1)
+    assert 1 <= chosen["back_zslab"] <= 16 and chosen["back_stage"] in (1, 2)
     assert job.tuning() == chosen
     assert np.array_equal(job.image(), img0)
     assert np.array_equal(job.norms(), nrm0)
@@ -90,7 +91,7 @@ def test_autotune_keeps_state(scan):
 
 def test_bad_shapes_rejected(scan):
     job = _job(scan, op_iter=(1,))
-    for bad in [(3, 0, 0, 0), (32, 0, 0, 0), (0, -1, 0, 0), (0, 65, 0, 0), (0, 0, 2, 0),
+    for bad in [(3, 0, 0, 0), (32, 0, 0, 0), (0, -1, 0, 0), (0, 65, 0, 0), (0, 0, 3, 0),
                 (0, 0, 0, 3), (0, 0, 0, 8)]:
         with pytest.raises(McsartError):
             job.set_tuning(*bad)
