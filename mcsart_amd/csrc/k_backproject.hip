@@ -70,7 +70,7 @@ __global__ void __launch_bounds__(BX *BY) k_backproject(GeomArgs g, double c, do
                                                        const float *__restrict__ proj,
                                                        const float2 *__restrict__ rp,
                                                        float *__restrict__ out, int zslab,
-                                                       NormFinish nf) {
+                                                       double ida, double idb, NormFinish nf) {
     const int tid = threadIdx.y * BX + threadIdx.x;
     // the preceding projector's per-view norm (its partials are complete: kernel boundary); the
     // whole first wavefront (threads 0..63) of the first workgroup does it
@@ -95,7 +95,7 @@ __global__ void __launch_bounds__(BX *BY) k_backproject(GeomArgs g, double c, do
     float aminL = 0.f, amaxL = 0.f;
     BLines bl{};
     if (live) {
-        column_setup(g, c, s, ix, iy, C);
+        column_setup_dev(g, c, s, ix, iy, ida, idb, C);
         ia_lo = (int)fmin(fmax(floor(C.Amin), 0.0), (double)g.na);   // :140,:150
         aminL = (float)(C.Amin - ia_lo);
         amaxL = (float)(C.Amax - ia_lo);
@@ -167,19 +167,18 @@ __global__ void __launch_bounds__(BX *BY) k_backproject(GeomArgs g, double c, do
                 int e[8];
 #pragma unroll
                 for (int k = 0; k < 8; ++k) {
-                    e[k] = base + k * BX * BY;
-                    int row = (int)((float)e[k] * inv);   // e < 4096 + 2048: off by at most one
+                    // past the end: element total-1 again (the same value to the same slot)
+                    e[k] = min(base + k * BX * BY, total - 1);
+                    int row = (int)((float)e[k] * inv);   // e < 4096: off by at most one
                     row -= (row * RA > e[k]) ? 1 : 0;
                     row += ((row + 1) * RA <= e[k]) ? 1 : 0;
                     const int cl = e[k] - row * RA;
-                    const unsigned src = (unsigned)((B0 + min(row, RB - 1)) * g.na + A0 + cl);
-                    if (e[k] < total)
-                        v[k] = MODE == BACK_PLAIN ? det_at<MODE>(proj, src * 4u)
-                                                  : det_at<MODE>(rp, src * 8u);
+                    const unsigned src = (unsigned)((B0 + row) * g.na + A0 + cl);
+                    v[k] = MODE == BACK_PLAIN ? det_at<MODE>(proj, src * 4u)
+                                              : det_at<MODE>(rp, src * 8u);
                 }
 #pragma unroll
-                for (int k = 0; k < 8; ++k)
-                    if (e[k] < total) stage[e[k]] = v[k];
+                for (int k = 0; k < 8; ++k) stage[e[k]] = v[k];
             }
         }
         __syncthreads();
@@ -286,12 +285,13 @@ template <int MODE, int WA>
 void launch_wa(dim3 grid, dim3 block, hipStream_t st, const GeomArgs &ga, const ViewTrig &t,
                const float *proj, const float2 *rp, float *out, int zs, const NormFinish &nf,
                bool stage) {
+    const double ida = 1.0 / (double)ga.da, idb = 1.0 / (double)ga.db;
     if (stage)
         hipLaunchKernelGGL((k_backproject<MODE, WA, true>), grid, block, 0, st, ga, t.c, t.s,
-                           proj, rp, out, zs, nf);
+                           proj, rp, out, zs, ida, idb, nf);
     else
         hipLaunchKernelGGL((k_backproject<MODE, WA, false>), grid, block, 0, st, ga, t.c, t.s,
-                           proj, rp, out, zs, nf);
+                           proj, rp, out, zs, ida, idb, nf);
 }
 
 template <int MODE>

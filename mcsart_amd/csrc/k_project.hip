@@ -274,14 +274,20 @@ __device__ inline float masked(float x, bool in) {
     return __uint_as_float(__float_as_uint(x) & (in ? 0xffffffffu : 0u));
 }
 
+// med3(x, lo, hi) = min(max(x, lo), hi) when lo <= hi, in one instruction (v_med3_f32).
+__device__ inline float clampf(float x, float lo, float hi) {
+    return __builtin_amdgcn_fmed3f(x, lo, hi);
+}
+
 template <int W>
 __device__ inline void window(float lo, float hi, float top, float (&w)[W], float &sw, float &f0) {
-    f0 = fminf(fmaxf(floorf(lo), 0.f), top);   // window start, inside [0, n - W]
-    float F = fmaxf(f0, lo);                    // F(f0): f0 <= hi whenever the plane is inside
+    f0 = clampf(floorf(lo), 0.f, top);   // window start, inside [0, n - W]
+    float F = fmaxf(f0, lo);              // F(f0): f0 <= hi whenever the plane is inside
     const float F0 = F;
 #pragma unroll
     for (int k = 0; k < W; ++k) {
-        const float Fn = (k + 1 < W) ? fminf(fmaxf(f0 + (float)(k + 1), lo), hi)
+        // interior boundaries clamp to [lo, hi] (lo <= hi inside; outside the plane is dropped)
+        const float Fn = (k + 1 < W) ? clampf(f0 + (float)(k + 1), lo, hi)
                                      : fminf(f0 + (float)W, hi);   // f0 + W >= lo inside
         w[k] = Fn - F;
         F = Fn;
