@@ -9,9 +9,11 @@ timed region starts.  Synthetic data: seeded ellipsoid phantom, breathing trace 
 v/f fields; the measured projections are made on the GPU with this library's warp and projector
 and Poisson noise at 1e5 photons (the FP64 CPU path is far too slow to synthesise them here).
 
-Output: one JSON line with scans/hour (value), per-scan latency p50/p99, the dominant kernel's
-roofline figures (HIP events on the stream the kernels run on) and the CPU baseline (the FP64
-OpenMP oracle timed on a bounded sample of the same scan on this host).
+Output: one JSON line with scans/hour (value), per-scan latency p50/p99, the roofline figures of
+every kernel of the sweeps and of the dominant one (the largest share of a scan; HIP events on
+the stream the kernels run on), the whole-step figure (the unfused loop's algorithmic bytes over
+the scan time) and the CPU baseline (the FP64 OpenMP oracle timed on a bounded sample of the same
+scan on this host).
 """
 import argparse
 import json
@@ -31,21 +33,48 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def algorithmic_bytes(spec, kind):
+def algorithmic_bytes(spec, kind, layouts=None):
     """HBM bytes one launch must move at minimum (4-byte floats).
     projector, image sweep : read x (N) and p (D), write (r, P1) (2D)      -> 4N + 12D
     projector, motion sweep: read g, q (2N) and p (D), write (r, Pq) (2D)  -> 8N + 12D
     backprojector          : read (r, P1) (2D), write u (N)                -> 4N + 8D
+    warp                   : read 6 fields and x, write x_i in each layout the view's rays
+                             read (1 or 2)                                 -> 28N + 4N per layout
+    warp-add               : read 6 fields, u and x, write x               -> 36N
+    motion step            : read and write 6 fields, read u and x, write (g, q) in each layout
+                             the next view's rays read                     -> 56N + 8N per layout
+    `layouts` (per view, bit 0 natural, bit 1 transposed) averages the per-layout terms over the
+    views the kernel is launched for.
     """
     g = spec.geometry
     N, D = g.nvox, g.ndet
+
+    def mean_layouts(views):
+        if layouts is None or len(views) == 0:
+            return 1.0
+        return float(np.mean([bin(int(layouts[i]) & 3).count("1") for i in views]))
+
+    V = spec.n_views
     if kind == "proj_image":
         return 4 * N + 12 * D
     if kind == "proj_motion":
         return 8 * N + 12 * D
     if kind == "back":
         return 4 * N + 8 * D
+    if kind == "warp":
+        return 28 * N + 4 * N * mean_layouts(range(V))
+    if kind == "warp_add":
+        return 36 * N
+    if kind == "motion":
+        return 56 * N + 8 * N * mean_layouts(range(2, V))   # prep of views 2..V-1
     raise KeyError(kind)
+
+
+def step_bytes(spec):
+    """Algorithmic bytes of one view of each sweep kind, unfused (one HBM pass per operator of
+    the reference's loop): image view 84N + 28D, motion view 128N + 44D."""
+    g = spec.geometry
+    return 84 * g.nvox + 28 * g.ndet, 128 * g.nvox + 44 * g.ndet
 
 
 def profiled_traffic(workload, kind):
@@ -54,7 +83,9 @@ def profiled_traffic(workload, kind):
     path = os.path.join(ROOT, "profiles", f"{workload}_kernels.json")
     if not os.path.exists(path):
         return None
-    prefix = "k_project_rows<1," if kind == "projector" else "k_backproject<1,"
+    prefix = {"projector": "k_project_rows<1,", "backprojector": "k_backproject<1,",
+              "warp": "k_warp_tile", "warp_add": "k_warp_add_tile",
+              "motion": "k_motion_step<true, true>"}[kind]
     with open(path) as f:
         rows = json.load(f).get("kernels", [])
     best = [r for r in rows if r["kernel"].startswith(prefix) and "traffic_bytes" in r]
@@ -241,18 +272,36 @@ def main():
     times = {k: ctx.kernel_time(k) for k in ("projector", "backprojector", "warp", "motion")}
     ctx.enable_timing(False)
     K = args.steps
-    # Roofline of the dominant footprint kernel: every view of the scan launched back to back
-    # between two HIP events on the job's stream (no per-launch event overhead).
-    iso_us = {k: job.kernel_time(k, reps=3) for k in ("projector", "backprojector")}
+    # Rooflines: every view of the scan launched back to back between two HIP events on the
+    # job's stream (no per-launch event overhead), for each kernel of the sweeps.  The dominant
+    # kernel is the one with the largest share of a scan (average launch x launches per scan).
+    layouts = job.view_layouts()
+    n_img_views, n_mot_views = launches_per_scan(spec)
+    n_warps = n_img_views if spec.motion else 0
+    per_scan = {"projector": n_img_views + n_mot_views,
+                "backprojector": n_img_views + n_mot_views,
+                "warp": n_warps, "warp_add": n_warps, "motion": n_mot_views}
+    byte_kind = {"projector": "proj_image", "backprojector": "back", "warp": "warp",
+                 "warp_add": "warp_add", "motion": "motion"}
+    iso_us = {k: job.kernel_time(k, reps=3) for k in per_scan}
     log(f"[bench] isolated launches: {iso_us}")
-    dom = max(iso_us, key=iso_us.get)
-    nbytes = algorithmic_bytes(spec, "proj_image" if dom == "projector" else "back")
-    achieved = nbytes / (iso_us[dom] * 1e-6) / 1e9
-    traffic = profiled_traffic(spec.name, dom)
+    kernels_roofline = {}
+    for k, us in iso_us.items():
+        nbytes = algorithmic_bytes(spec, byte_kind[k], layouts)
+        ach = nbytes / (us * 1e-6) / 1e9
+        kernels_roofline[k] = {
+            "avg_launch_us": round(us, 2), "bytes_per_launch": int(nbytes),
+            "achieved": round(ach, 1), "frac": round(ach / HBM_PEAK_GBS, 4),
+            "launches_per_scan": per_scan[k],
+            "ms_per_scan": round(us * per_scan[k] * 1e-3, 2),
+            "traffic": profiled_traffic(spec.name, k)}
+    dom = max(kernels_roofline, key=lambda k: kernels_roofline[k]["ms_per_scan"])
+    d = kernels_roofline[dom]
+    traffic = d["traffic"]
     roofline = {
-        "bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-        "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-        "avg_launch_us": round(iso_us[dom], 2), "bytes_per_launch": int(nbytes),
+        "bound": "hbm", "kernel": dom, "achieved": d["achieved"], "peak": HBM_PEAK_GBS,
+        "unit": "GB/s", "frac": d["frac"], "traffic": traffic,
+        "avg_launch_us": d["avg_launch_us"], "bytes_per_launch": d["bytes_per_launch"],
         "timing": "HIP events around all views launched back to back, 3 repetitions",
         "traffic_source": (f"profiles/{spec.name}_kernels.json (rocprofv3 FETCH_SIZE + "
                            "WRITE_SIZE, calibrated on a 64 MiB copy)") if traffic else None,
@@ -277,6 +326,13 @@ def main():
 
     lat_ms = np.array(lat) * 1e3
     value = 3600.0 * K / elapsed
+    # Whole-step figure: the unfused loop's algorithmic bytes per scan over the measured scan time
+    img_b, mot_b = step_bytes(spec)
+    scan_bytes = n_img_views * img_b + n_mot_views * mot_b
+    step_ach = scan_bytes / (elapsed / K) / 1e9
+    step_roofline = {"image_view_bytes": int(img_b), "motion_view_bytes": int(mot_b),
+                     "bytes_per_scan": int(scan_bytes), "achieved": round(step_ach, 1),
+                     "unit": "GB/s", "frac": round(step_ach / HBM_PEAK_GBS, 4)}
     out = {
         "metric": "McSART scans/hour (128^3 motion-compensated scan, 200 views, 10 outer iterations)",
         "value": round(value, 2),
@@ -299,6 +355,8 @@ def main():
         "kernels_note": "eager pass with HIP events around every launch (includes event overhead)",
         "eager_ms_per_scan_with_events": round(eager_ms, 2),
         "isolated_launch_us": {k: round(v, 2) for k, v in iso_us.items()},
+        "kernels_roofline": kernels_roofline,
+        "step_roofline": step_roofline,
         "host_boundary_ms_per_scan": round(host_ms, 1),
         "concurrent": conc,
         "launch": "one HIP graph per sweep",

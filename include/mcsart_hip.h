@@ -339,11 +339,18 @@ int mcsart_job_get_tuning(mcsart_job *job, mcsart_tuning *t);
  * current image, backprojector in ratio mode into scratch; the job's image, fields and norms are
  * untouched), keep the fastest and report it in *chosen (may be NULL).  Synchronous. */
 int mcsart_job_autotune(mcsart_job *job, mcsart_tuning *chosen);
-/* Average duration (microseconds) of one launch of the projector (kind 0, residual mode) or the
- * backprojector (kind 1, ratio mode) with the job's launch shapes: every view of the scan is
- * launched back to back `reps` times between two HIP events on the job's stream (scratch outputs
- * only; the job's image, fields and norms are untouched).  Synchronous. */
+/* Average duration (microseconds) of one launch of a kernel of the sweeps, with the job's launch
+ * shapes: 0 projector (residual mode), 1 backprojector (ratio mode), 2 warp of the image into the
+ * layouts the view's rays read, 3 warp-add (x += lambda W(u)), 4 motion step (field update of
+ * view i fused with the prep of view i+1).  Every view of the scan (views 1..V-2 for kind 4) is
+ * launched back to back `reps` times between two HIP events on the job's stream.  Outputs go to
+ * scratch: kind 3 updates a copy of the image, kind 4 a temporary copy of the fields (6N floats,
+ * allocated and freed by the call); the job's image, fields and norms are untouched.
+ * Synchronous. */
 int mcsart_job_kernel_time(mcsart_job *job, int kind, int reps, double *avg_us);
+/* Detector layouts each view's rays read: bit 0 the natural volume layout (y-branch rays), bit 1
+ * the xy-transposed layout (x-branch rays); layouts[n_views]. */
+int mcsart_job_view_layouts(mcsart_job *job, int *layouts);
 
 /* ---------------------------------------------------------------------------------------------
  * 5. Reconstruction server: continuous batching of jobs on one device (no counterpart in the

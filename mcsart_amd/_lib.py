@@ -71,6 +71,7 @@ SIGNATURES = {
     "mcsart_job_get_tuning": ([_P, _P], _I),
     "mcsart_job_autotune": ([_P, _P], _I),
     "mcsart_job_kernel_time": ([_P, _I, _I, ctypes.POINTER(ctypes.c_double)], _I),
+    "mcsart_job_view_layouts": ([_P, _P], _I),
     "mcsart_ctx_enable_timing": ([_P, _I], _I),
     "mcsart_ctx_kernel_time": ([_P, _I, ctypes.POINTER(ctypes.c_double),
                                 ctypes.POINTER(ctypes.c_longlong)], _I),

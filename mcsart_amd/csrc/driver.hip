@@ -530,25 +530,70 @@ extern "C" int mcsart_job_get_tuning(mcsart_job *j, mcsart_tuning *t) {
     return MCSART_OK;
 }
 
+extern "C" int mcsart_job_view_layouts(mcsart_job *j, int *layouts) {
+    int rc = check_job(j);
+    if (rc) return rc;
+    if (!layouts) return fail(MCSART_EINVAL, "null layouts");
+    for (int iv = 0; iv < j->V; ++iv)
+        layouts[iv] = (j->vinfo[iv].has_y ? 1 : 0) | (j->vinfo[iv].has_x ? 2 : 0);
+    return MCSART_OK;
+}
+
 extern "C" int mcsart_job_kernel_time(mcsart_job *j, int kind, int reps, double *avg_us) {
     int rc = check_job(j);
     if (rc) return rc;
-    if (!avg_us || reps < 1 || (kind != 0 && kind != 1))
-        return fail(MCSART_EINVAL, "kind must be 0 (projector) or 1 (backprojector), reps >= 1");
+    if (!avg_us || reps < 1 || kind < 0 || kind > 4)
+        return fail(MCSART_EINVAL, "kind must be 0..4 (projector, backprojector, warp, warp-add, "
+                                   "motion step), reps >= 1");
     mcsart_ctx *ctx = j->ctx;
     if (ctx->timing) return fail(MCSART_EINVAL, "kernel timing while per-kernel events are on");
     MCSART_HIP_TRY(hipSetDevice(ctx->device));
     const Geom &g = j->g;
-    const long long D = g.ndet();
+    const long long D = g.ndet(), N = g.nvox();
     hipStream_t st = ctx->stream;
     MCSART_HIP_TRY(hipStreamSynchronize(st));
     rc = launch_transpose(j->d_w1T, j->d_img, g.nx, g.ny, g.nz, st);   // scratch copy
     if (rc) return rc;
-    // every view of the scan once per rep, back to back, as the image sweep launches them
+    // warp-add updates a scratch copy of the image; the motion step a scratch copy of the fields
+    float *tmp_fields = nullptr;
+    if (kind == 3)
+        MCSART_HIP_TRY(hipMemcpyAsync(j->d_w1, j->d_img, sizeof(float) * N,
+                                      hipMemcpyDeviceToDevice, st));
+    if (kind == 4) {
+        MCSART_HIP_TRY(hipMalloc(&tmp_fields, sizeof(float) * 6 * N));
+        if (hipMemcpyAsync(tmp_fields, j->d_fields, sizeof(float) * 6 * N,
+                           hipMemcpyDeviceToDevice, st) != hipSuccess) {
+            hipFree(tmp_fields);
+            return fail(MCSART_EHIP, "field copy failed");
+        }
+    }
+    const Fields f = fields_of(j);
+    float *fl[6];
+    for (int c = 0; c < 6; ++c) fl[c] = tmp_fields ? tmp_fields + c * N : nullptr;
+    float2 *pair = (float2 *)j->d_w1, *pairT = (float2 *)j->d_w1T;
+    // every view of the scan once per rep, back to back, as the sweeps launch them
     auto all_views = [&]() {
         for (int iv = 0; iv < j->V; ++iv) {
+            const ViewInfo &vi = j->vinfo[iv];
             int r;
-            if (kind == 0) {
+            if (kind == 2) {   // x_i = W(x; v_i - v_0, f_i - f_0) in the layouts P reads
+                r = launch_warp(vi.has_y ? j->d_w1 : nullptr, vi.has_x ? j->d_w1T : nullptr,
+                                j->d_img, g.nx, g.ny, g.nz, j->volumes[iv] - j->volumes[0],
+                                j->flows[iv] - j->flows[0], f, st);
+            } else if (kind == 3) {   // x += lambda W(u; v_0 - v_i, f_0 - f_i), on the copy
+                r = launch_warp_add(j->d_w1, j->d_w2, g.nx, g.ny, g.nz,
+                                    j->volumes[0] - j->volumes[iv], j->flows[0] - j->flows[iv],
+                                    f, j->opts.lambda, j->opts.nonneg, st);
+            } else if (kind == 4) {   // update of view iv, prep of view iv + 1 (views 1..V-2)
+                if (iv == 0 || iv + 1 >= j->V) continue;
+                const ViewInfo &vn = j->vinfo[iv + 1];
+                r = launch_motion_step(fl, j->d_w2, j->d_img, vn.has_y ? pair : nullptr,
+                                       vn.has_x ? pairT : nullptr, g.nx, g.ny, g.nz,
+                                       j->volumes[iv - 1] - j->volumes[iv],
+                                       j->flows[iv - 1] - j->flows[iv],
+                                       j->volumes[iv + 1] - j->volumes[iv],
+                                       j->flows[iv + 1] - j->flows[iv], true, true, st);
+            } else if (kind == 0) {
                 ProjArgs a{};
                 a.vol0 = j->d_img;
                 a.vol0T = j->vinfo[iv].has_x ? j->d_w1T : nullptr;
@@ -567,11 +612,17 @@ extern "C" int mcsart_job_kernel_time(mcsart_job *j, int kind, int reps, double 
         return (int)MCSART_OK;
     };
     rc = all_views();   // warm-up
-    if (rc) return rc;
     float ms = 0.f;
-    rc = time_on_stream(j, reps, all_views, &ms);
+    if (!rc) rc = time_on_stream(j, reps, all_views, &ms);
+    if (tmp_fields) {
+        if (hipStreamSynchronize(st) != hipSuccess && !rc)
+            rc = fail(MCSART_EHIP, "stream synchronisation failed");
+        hipFree(tmp_fields);
+    }
     if (rc) return rc;
-    *avg_us = 1e3 * (double)ms / ((double)reps * j->V);
+    const int launches = kind == 4 ? std::max(j->V - 2, 0) : j->V;
+    if (launches == 0) return fail(MCSART_EINVAL, "the motion step needs at least three views");
+    *avg_us = 1e3 * (double)ms / ((double)reps * launches);
     return MCSART_OK;
 }
 

@@ -163,44 +163,133 @@ __global__ void k_forward_dvf(float *__restrict__ mx, float *__restrict__ my,
 // LDS tile (+1 padding: conflict-free columns) so that global accesses stay coalesced.
 constexpr int TT = 32, TR = 8, TK = TT / TR;
 
+// Index arithmetic of the warps.  The fast form (FAST = true) uses 24-bit multiplies and unsigned
+// 32-bit byte offsets from a volume's base address, so every load is one instruction on a uniform
+// base; it needs n0 < 2^24, n1 * n2 < 2^24 and fewer than 2^30 voxels (fast_index below).  The
+// general form uses 32-bit signed element indices (volumes of < 2^31 voxels).
+inline bool fast_index(int n0, int n1, int n2) {
+    return n0 >= 2 && n0 < (1 << 24) && (long long)n1 * n2 < (1 << 24) &&
+           (long long)n0 * n1 * n2 < (1LL << 30);
+}
+
+__device__ inline float ldb(const float *__restrict__ base, unsigned off) {
+    return *reinterpret_cast<const float *>(reinterpret_cast<const char *>(base) + off);
+}
+// Two consecutive floats at a 4-byte aligned byte offset, as one 8-byte load.
+__device__ inline float2 ldb2(const float *__restrict__ base, unsigned off) {
+    float2 v;
+    __builtin_memcpy(&v, reinterpret_cast<const char *>(base) + off, sizeof(v));
+    return v;
+}
+__device__ inline void stb(float *__restrict__ base, unsigned off, float v) {
+    *reinterpret_cast<float *>(reinterpret_cast<char *>(base) + off) = v;
+}
+
+// Element index i0 + i1*n0 + i2*n0*n1 of a voxel: a byte offset in the fast form.
+template <bool FAST>
+__device__ inline unsigned voxel_off(int i0, int i1, int i2, int n0, int n1) {
+    if constexpr (FAST)
+        return (__umul24(__umul24((unsigned)i2, (unsigned)n1) + (unsigned)i1, (unsigned)n0) +
+                (unsigned)i0) << 2;
+    else
+        return (unsigned)(i0 + i1 * n0 + i2 * n0 * n1);
+}
+template <bool FAST>
+__device__ inline float vload(const float *__restrict__ p, unsigned o) {
+    if constexpr (FAST)
+        return ldb(p, o);
+    else
+        return p[(int)o];
+}
+template <bool FAST>
+__device__ inline void vstore(float *__restrict__ p, unsigned o, float v) {
+    if constexpr (FAST)
+        stb(p, o, v);
+    else
+        p[(int)o] = v;
+}
+
 // Clamp-to-edge trilinear sample of img at (i0 + m0, i1 + m1, i2 + m2), the value the
 // reference's texture fetch returns at unnormalised coordinate (i + 1 + m - 0.5)
 // (src/cu_deform.cu:86-89), with exact FP32 weights.  The integer part of each displacement is
 // split off first, so the fraction keeps full precision for any index size.
+template <bool FAST>
 __device__ inline float trilinear(const float *__restrict__ img, int n0, int n1, int n2, int i0,
                                   float m0, int i1, float m1, int i2, float m2) {
     // A displacement larger than the volume lands outside it on the same side, where clamp-to-edge
     // addressing returns the edge voxel whatever its size: clamping it to +-(n+1) changes no
-    // result and keeps the float->int conversions in range (a NaN displacement becomes -(n+1)).
-    m0 = fminf(fmaxf(m0, -(float)(n0 + 1)), (float)(n0 + 1));
-    m1 = fminf(fmaxf(m1, -(float)(n1 + 1)), (float)(n1 + 1));
-    m2 = fminf(fmaxf(m2, -(float)(n2 + 1)), (float)(n2 + 1));
+    // result and keeps the float->int conversions in range.  v_med3_f32 returns the minimum of
+    // its operands when one is NaN, so a NaN displacement (an arithmetic result, hence a quiet
+    // NaN) becomes -(n+1).
+    m0 = __builtin_amdgcn_fmed3f(m0, -(float)(n0 + 1), (float)(n0 + 1));
+    m1 = __builtin_amdgcn_fmed3f(m1, -(float)(n1 + 1), (float)(n1 + 1));
+    m2 = __builtin_amdgcn_fmed3f(m2, -(float)(n2 + 1), (float)(n2 + 1));
     const float f0 = floorf(m0), f1 = floorf(m1), f2 = floorf(m2);
     const float t0 = m0 - f0, t1 = m1 - f1, t2 = m2 - f2;
     const int b0 = i0 + (int)f0, b1 = i1 + (int)f1, b2 = i2 + (int)f2;
-    const int a0 = clampi_d(b0, 0, n0 - 1), c0 = clampi_d(b0 + 1, 0, n0 - 1);
-    const int a1 = clampi_d(b1, 0, n1 - 1), c1 = clampi_d(b1 + 1, 0, n1 - 1);
-    const int a2 = clampi_d(b2, 0, n2 - 1), c2 = clampi_d(b2 + 1, 0, n2 - 1);
-    const int s2 = n0 * n1;
-    const int r00 = a1 * n0 + a2 * s2, r10 = c1 * n0 + a2 * s2;
-    const int r01 = a1 * n0 + c2 * s2, r11 = c1 * n0 + c2 * s2;
     const float u0 = 1.f - t0, u1 = 1.f - t1, u2 = 1.f - t2;
-    const float c00 = img[r00 + a0] * u0 + img[r00 + c0] * t0;
-    const float c10 = img[r10 + a0] * u0 + img[r10 + c0] * t0;
-    const float c01 = img[r01 + a0] * u0 + img[r01 + c0] * t0;
-    const float c11 = img[r11 + a0] * u0 + img[r11 + c0] * t0;
+    float v[8];
+    if constexpr (FAST) {
+        // lower corner clamped into the volume; the upper corner is one step further along an
+        // axis exactly when 0 <= b < n - 1, and the same voxel otherwise
+        const int a0 = min(max(b0, 0), n0 - 1);
+        const int a1 = min(max(b1, 0), n1 - 1);
+        const int a2 = min(max(b2, 0), n2 - 1);
+        const unsigned s0 = (unsigned)b0 < (unsigned)(n0 - 1) ? 4u : 0u;
+        const unsigned s1 = (unsigned)b1 < (unsigned)(n1 - 1) ? 4u * (unsigned)n0 : 0u;
+        const unsigned s2 =
+            (unsigned)b2 < (unsigned)(n2 - 1) ? 4u * (unsigned)n0 * (unsigned)n1 : 0u;
+        // Each row's pair (a0, c0) comes from one 8-byte load of elements (e, e + 1) of that
+        // row: e = a0, except at the upper edge (c0 == a0 == n0 - 1), where e = a0 - 1 and both
+        // values are the second element.  With n0 >= 2 both elements lie in the row.
+        const bool hi = s0 == 0u && a0 > 0;
+        const bool second = s0 != 0u || hi;   // c0 is element e + 1
+        const unsigned o = voxel_off<true>(a0, a1, a2, n0, n1) - (hi ? 4u : 0u);
+        const unsigned o1 = o + s1, o2 = o + s2, o3 = o2 + s1;
+        const float2 p0 = ldb2(img, o), p1 = ldb2(img, o1), p2 = ldb2(img, o2),
+                     p3 = ldb2(img, o3);
+        v[0] = hi ? p0.y : p0.x;
+        v[1] = second ? p0.y : p0.x;
+        v[2] = hi ? p1.y : p1.x;
+        v[3] = second ? p1.y : p1.x;
+        v[4] = hi ? p2.y : p2.x;
+        v[5] = second ? p2.y : p2.x;
+        v[6] = hi ? p3.y : p3.x;
+        v[7] = second ? p3.y : p3.x;
+    } else {
+        const int a0 = clampi_d(b0, 0, n0 - 1), c0 = clampi_d(b0 + 1, 0, n0 - 1);
+        const int a1 = clampi_d(b1, 0, n1 - 1), c1 = clampi_d(b1 + 1, 0, n1 - 1);
+        const int a2 = clampi_d(b2, 0, n2 - 1), c2 = clampi_d(b2 + 1, 0, n2 - 1);
+        const int s2 = n0 * n1;
+        const int r00 = a1 * n0 + a2 * s2, r10 = c1 * n0 + a2 * s2;
+        const int r01 = a1 * n0 + c2 * s2, r11 = c1 * n0 + c2 * s2;
+        v[0] = img[r00 + a0];
+        v[1] = img[r00 + c0];
+        v[2] = img[r10 + a0];
+        v[3] = img[r10 + c0];
+        v[4] = img[r01 + a0];
+        v[5] = img[r01 + c0];
+        v[6] = img[r11 + a0];
+        v[7] = img[r11 + c0];
+    }
+    const float c00 = v[0] * u0 + v[1] * t0;
+    const float c10 = v[2] * u0 + v[3] * t0;
+    const float c01 = v[4] * u0 + v[5] * t0;
+    const float c11 = v[6] * u0 + v[7] * t0;
     const float e0 = c00 * u1 + c10 * t1;
     const float e1 = c01 * u1 + c11 * t1;
     return e0 * u2 + e1 * t2;
 }
 
-// DVF of voxel id (kernel_forwardDVF, src/cu_deform.cu:74-76): (mx, my, mz).
+// DVF of voxel o (kernel_forwardDVF, src/cu_deform.cu:74-76): (mx, my, mz).
 struct Dvf {
     float x, y, z;
 };
-__device__ inline Dvf dvf_at(const Fields &f, int id, float vol, float flow) {
-    return Dvf{f.ax[id] * vol + f.bx[id] * flow, f.ay[id] * vol + f.by[id] * flow,
-               f.az[id] * vol + f.bz[id] * flow};
+template <bool FAST>
+__device__ inline Dvf dvf_at(const Fields &f, unsigned o, float vol, float flow) {
+    return Dvf{vload<FAST>(f.ax, o) * vol + vload<FAST>(f.bx, o) * flow,
+               vload<FAST>(f.ay, o) * vol + vload<FAST>(f.by, o) * flow,
+               vload<FAST>(f.az, o) * vol + vload<FAST>(f.bz, o) * flow};
 }
 
 // Forward differences of x at voxel id, zero at the last index (src/cu_diff.cu:2-33).
@@ -249,6 +338,7 @@ __device__ inline void store_transposed(T (*t)[TT + 1], T *__restrict__ outT, co
 }
 
 // Forward warp (src/cu_deform.cu:79-90): natural output (out) and/or transposed output (outT).
+template <bool FAST>
 __global__ void __launch_bounds__(TT *TR) k_warp_tile(float *__restrict__ out,
                                                      float *__restrict__ outT,
                                                      const float *__restrict__ src, int nx, int ny,
@@ -257,31 +347,33 @@ __global__ void __launch_bounds__(TT *TR) k_warp_tile(float *__restrict__ out,
     const int X0 = blockIdx.x * TT, Y0 = blockIdx.y * TT, i2 = blockIdx.z;
     const int i0 = X0 + threadIdx.x;
     Dvf m[TK];
-    int id[TK];
+    unsigned id[TK];
     bool ok[TK];
 #pragma unroll
     for (int k = 0; k < TK; ++k) {
         const int i1 = Y0 + threadIdx.y + k * TR;
         ok[k] = i0 < nx && i1 < ny;
-        id[k] = i0 + i1 * nx + i2 * nx * ny;
-        m[k] = ok[k] ? dvf_at(f, id[k], vol, flow) : Dvf{0.f, 0.f, 0.f};
+        id[k] = voxel_off<FAST>(i0, i1, i2, nx, ny);
+        m[k] = ok[k] ? dvf_at<FAST>(f, id[k], vol, flow) : Dvf{0.f, 0.f, 0.f};
     }
     float v[TK];
 #pragma unroll
     for (int k = 0; k < TK; ++k) {
         const int i1 = Y0 + threadIdx.y + k * TR;
         // sample at (i0 + my, i1 + mx, i2 + mz) (:85-89)
-        v[k] = ok[k] ? trilinear(src, nx, ny, nz, i0, m[k].y, i1, m[k].x, i2, m[k].z) : 0.f;
+        v[k] = ok[k] ? trilinear<FAST>(src, nx, ny, nz, i0, m[k].y, i1, m[k].x, i2, m[k].z)
+                     : 0.f;
     }
     if (out) {
 #pragma unroll
         for (int k = 0; k < TK; ++k)
-            if (ok[k]) out[id[k]] = v[k];
+            if (ok[k]) vstore<FAST>(out, id[k], v[k]);
     }
     if (outT) store_transposed(t, outT, v, nx, ny, X0, Y0, i2);
 }
 
 // x += lambda * W(u; vol, flow) (src/AUMISART.cu:222-223), then the optional clamp at zero.
+template <bool FAST>
 __global__ void __launch_bounds__(TT *TR) k_warp_add_tile(float *__restrict__ x,
                                                          const float *__restrict__ u, int nx,
                                                          int ny, int nz, float vol, float flow,
@@ -290,27 +382,27 @@ __global__ void __launch_bounds__(TT *TR) k_warp_add_tile(float *__restrict__ x,
     const int i0 = X0 + threadIdx.x;
     Dvf m[TK];
     float xv[TK];
-    int id[TK];
+    unsigned id[TK];
     bool ok[TK];
 #pragma unroll
     for (int k = 0; k < TK; ++k) {
         const int i1 = Y0 + threadIdx.y + k * TR;
         ok[k] = i0 < nx && i1 < ny;
-        id[k] = i0 + i1 * nx + i2 * nx * ny;
-        m[k] = ok[k] ? dvf_at(f, id[k], vol, flow) : Dvf{0.f, 0.f, 0.f};
-        xv[k] = ok[k] ? x[id[k]] : 0.f;
+        id[k] = voxel_off<FAST>(i0, i1, i2, nx, ny);
+        m[k] = ok[k] ? dvf_at<FAST>(f, id[k], vol, flow) : Dvf{0.f, 0.f, 0.f};
+        xv[k] = ok[k] ? vload<FAST>(x, id[k]) : 0.f;
     }
     float v[TK];
 #pragma unroll
     for (int k = 0; k < TK; ++k) {
         const int i1 = Y0 + threadIdx.y + k * TR;
-        v[k] = ok[k] ? trilinear(u, nx, ny, nz, i0, m[k].y, i1, m[k].x, i2, m[k].z) : 0.f;
+        v[k] = ok[k] ? trilinear<FAST>(u, nx, ny, nz, i0, m[k].y, i1, m[k].x, i2, m[k].z) : 0.f;
     }
 #pragma unroll
     for (int k = 0; k < TK; ++k) {
         float y = xv[k] + v[k] * lambda;
         if (nonneg) y = fmaxf(y, 0.f);
-        if (ok[k]) x[id[k]] = y;
+        if (ok[k]) vstore<FAST>(x, id[k], y);
     }
 }
 
@@ -478,6 +570,7 @@ __global__ void __launch_bounds__(TT *TR) k_transpose(float *__restrict__ dst,
 // Gauss-Seidel sweeps x <- -m_x(p + d), y <- -m_y(p + d), z <- -m_z(p + d), each component
 // sampled (clamp-to-edge trilinear, index space, dim 0 fastest) at the current estimate
 // d = (x, y, z) with the components updated so far.
+template <bool FAST>
 __global__ void k_invert_dvf(float *__restrict__ mx2, float *__restrict__ my2,
                              float *__restrict__ mz2, const float *__restrict__ mx,
                              const float *__restrict__ my, const float *__restrict__ mz, int nx,
@@ -488,14 +581,14 @@ __global__ void k_invert_dvf(float *__restrict__ mx2, float *__restrict__ my2,
     if (i0 >= nx || i1 >= ny) return;
     float x = 0.f, y = 0.f, z = 0.f;
     for (int it = 0; it < niter; ++it) {
-        x = -trilinear(mx, nx, ny, nz, i0, x, i1, y, i2, z);
-        y = -trilinear(my, nx, ny, nz, i0, x, i1, y, i2, z);
-        z = -trilinear(mz, nx, ny, nz, i0, x, i1, y, i2, z);
+        x = -trilinear<FAST>(mx, nx, ny, nz, i0, x, i1, y, i2, z);
+        y = -trilinear<FAST>(my, nx, ny, nz, i0, x, i1, y, i2, z);
+        z = -trilinear<FAST>(mz, nx, ny, nz, i0, x, i1, y, i2, z);
     }
-    const int id = i0 + i1 * nx + i2 * nx * ny;
-    mx2[id] = x;
-    my2[id] = y;
-    mz2[id] = z;
+    const unsigned id = voxel_off<FAST>(i0, i1, i2, nx, ny);
+    vstore<FAST>(mx2, id, x);
+    vstore<FAST>(my2, id, y);
+    vstore<FAST>(mz2, id, z);
 }
 
 inline dim3 vgrid(int nx, int ny, int nz) {
@@ -511,8 +604,12 @@ inline dim3 tile_grid(int nx, int ny, int nz) {
 
 int launch_warp(float *out, float *outT, const float *src, int nx, int ny, int nz, float vol,
                 float flow, const Fields &f, hipStream_t st) {
-    hipLaunchKernelGGL(k_warp_tile, tile_grid(nx, ny, nz), dim3(TT, TR), 0, st, out, outT, src,
-                       nx, ny, nz, vol, flow, f);
+    if (fast_index(nx, ny, nz))
+        hipLaunchKernelGGL(k_warp_tile<true>, tile_grid(nx, ny, nz), dim3(TT, TR), 0, st, out,
+                           outT, src, nx, ny, nz, vol, flow, f);
+    else
+        hipLaunchKernelGGL(k_warp_tile<false>, tile_grid(nx, ny, nz), dim3(TT, TR), 0, st, out,
+                           outT, src, nx, ny, nz, vol, flow, f);
     MCSART_CHECK_LAUNCH();
     return MCSART_OK;
 }
@@ -534,8 +631,12 @@ int launch_add_tile(float *x, float *xT, const float *u, int nx, int ny, int nz,
 
 int launch_warp_add(float *x, const float *u, int nx, int ny, int nz, float vol, float flow,
                     const Fields &f, float lambda, int nonneg, hipStream_t st) {
-    hipLaunchKernelGGL(k_warp_add_tile, tile_grid(nx, ny, nz), dim3(TT, TR), 0, st, x, u, nx, ny,
-                       nz, vol, flow, f, lambda, nonneg);
+    if (fast_index(nx, ny, nz))
+        hipLaunchKernelGGL(k_warp_add_tile<true>, tile_grid(nx, ny, nz), dim3(TT, TR), 0, st, x,
+                           u, nx, ny, nz, vol, flow, f, lambda, nonneg);
+    else
+        hipLaunchKernelGGL(k_warp_add_tile<false>, tile_grid(nx, ny, nz), dim3(TT, TR), 0, st, x,
+                           u, nx, ny, nz, vol, flow, f, lambda, nonneg);
     MCSART_CHECK_LAUNCH();
     return MCSART_OK;
 }
@@ -653,8 +754,12 @@ int launch_forward_dvf(float *mx, float *my, float *mz, const Fields &f, int nx,
 
 int launch_invert_dvf(float *mx2, float *my2, float *mz2, const float *mx, const float *my,
                       const float *mz, int nx, int ny, int nz, int niter, hipStream_t st) {
-    hipLaunchKernelGGL(k_invert_dvf, vgrid(nx, ny, nz), dim3(EX, EY), 0, st, mx2, my2, mz2, mx,
-                       my, mz, nx, ny, nz, niter);
+    if (fast_index(nx, ny, nz))
+        hipLaunchKernelGGL(k_invert_dvf<true>, vgrid(nx, ny, nz), dim3(EX, EY), 0, st, mx2, my2,
+                           mz2, mx, my, mz, nx, ny, nz, niter);
+    else
+        hipLaunchKernelGGL(k_invert_dvf<false>, vgrid(nx, ny, nz), dim3(EX, EY), 0, st, mx2, my2,
+                           mz2, mx, my, mz, nx, ny, nz, niter);
     MCSART_CHECK_LAUNCH();
     return MCSART_OK;
 }

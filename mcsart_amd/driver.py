@@ -278,14 +278,23 @@ class Job:
         check(self._lib.mcsart_job_autotune(self.ptr, ctypes.byref(t)), "mcsart_job_autotune")
         return t.as_dict()
 
+    KERNEL_KINDS = {"projector": 0, "backprojector": 1, "warp": 2, "warp_add": 3, "motion": 4}
+
     def kernel_time(self, kind: str, reps: int = 3) -> float:
-        """Average microseconds per launch of 'projector' or 'backprojector' over all views,
-        launched back to back between two HIP events on the job's stream."""
+        """Average microseconds per launch of one of KERNEL_KINDS over all views of the scan,
+        launched back to back between two HIP events on the job's stream (scratch outputs)."""
         us = ctypes.c_double()
-        check(self._lib.mcsart_job_kernel_time(self.ptr, {"projector": 0, "backprojector": 1}[kind],
-                                               int(reps), ctypes.byref(us)),
+        check(self._lib.mcsart_job_kernel_time(self.ptr, self.KERNEL_KINDS[kind], int(reps),
+                                               ctypes.byref(us)),
               "mcsart_job_kernel_time")
         return us.value
+
+    def view_layouts(self) -> np.ndarray:
+        """Per view: bit 0 = rays read the natural layout, bit 1 = the transposed layout."""
+        out = np.zeros(self.V, np.int32)
+        check(self._lib.mcsart_job_view_layouts(self.ptr, out.ctypes.data),
+              "mcsart_job_view_layouts")
+        return out
 
     def run(self):
         check(self._lib.mcsart_job_run(self.ptr), "mcsart_job_run")

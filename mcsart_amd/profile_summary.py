@@ -52,7 +52,7 @@ def algorithmic_bytes(kernel: str, N: int, D: int):
         "k_transpose": 8 * N,
         "k_add_tile": 12 * N,
     }
-    return table.get(kernel)
+    return table.get(re.sub(r"<(true|false)>$", "", kernel))
 
 
 def load_stats(d):

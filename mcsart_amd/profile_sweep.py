@@ -6,8 +6,10 @@ hardware-counter passes).
 Builds the named workload's scan (synthetic inputs, measured projections made on the GPU),
 autotunes the footprint kernels' launch shapes as the benchmark does, then runs the given sweeps
 (1 = image sweep, 0 = motion sweep) kernel by kernel (no graphs, so every kernel is its own
-dispatch), and finally one device-to-device copy of --copy-mb MiB: the profiler's FETCH_SIZE /
-WRITE_SIZE for that copy calibrate the counters against a known byte count.
+dispatch), optionally the isolated launches the benchmark times (--isolated: every view of the
+scan back to back for each named kernel, as mcsart_job_kernel_time launches them), and finally
+one device-to-device copy of --copy-mb MiB: the profiler's FETCH_SIZE / WRITE_SIZE for that copy
+calibrate the counters against a known byte count.
 """
 import argparse
 import sys
@@ -32,6 +34,9 @@ def main(argv=None):
     ap.add_argument("--sweeps", default="1,0")
     ap.add_argument("--views", type=int, default=0, help="use only the first V views (0: all)")
     ap.add_argument("--copy-mb", type=int, default=64)
+    ap.add_argument("--isolated", default="",
+                    help="comma-separated kernels to launch over all views back to back after "
+                         "the sweeps: projector, backprojector, warp, warp_add, motion")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--tuning", default="",
                     help="fixed launch shapes proj_split,back_zslab,back_stage,proj_rows "
@@ -61,6 +66,9 @@ def main(argv=None):
     ctx.synchronize()
     print(f"[profile_sweep] {spec.name}: sweeps {sweeps} over {spec.n_views} views in "
           f"{(time.perf_counter() - t0) * 1e3:.1f} ms", flush=True)
+    for kind in [k for k in args.isolated.split(",") if k]:
+        us = job.kernel_time(kind, reps=1)
+        print(f"[profile_sweep] isolated {kind}: {us:.2f} us per launch", flush=True)
     if args.copy_mb:
         n = args.copy_mb << 20
         a, b = DeviceArray.empty((n // 4,)), DeviceArray.empty((n // 4,))

@@ -97,3 +97,40 @@ def test_bad_shapes_rejected(scan):
             job.set_tuning(*bad)
     assert job.tuning() == {"proj_split": 0, "back_zslab": 0, "back_stage": 0, "proj_rows": 0}
     job.close()
+
+
+def test_kernel_time_all_kinds_leave_job_untouched():
+    """Isolated timing of every sweep kernel writes only scratch: a motion scan run after the
+    timings gives bit for bit the image, norms and fields of the same scan run before them."""
+    spec = configs.c1_static64()
+    g = spec.geometry
+    V = 12
+    ang = np.linspace(0, 2 * np.pi, V, endpoint=False).astype(np.float32)
+    vol = np.linspace(-1.0, 1.0, V).astype(np.float32)
+    ctx = Context(0)
+    job = Job(ctx, g, ang, vol, 0.2 * vol, np.array([1, 0], np.int32), lam=1.0, motion=True,
+              nonneg=True)
+    rng = np.random.default_rng(7)
+    job.set_projections(rng.random((V,) + g.det_shape, dtype=np.float32))
+    job.set_image(None)
+    job.set_fields([0.3 * rng.standard_normal(g.vol_shape).astype(np.float32)
+                    for _ in range(6)])
+    job.run()
+    before = job.image(), job.norms(), job.fields()
+    layouts = job.view_layouts()
+    assert layouts.shape == (V,) and ((layouts >= 1) & (layouts <= 3)).all()
+    for kind in Job.KERNEL_KINDS:
+        us = job.kernel_time(kind, reps=2)
+        assert 0.0 < us < 1e5, (kind, us)
+    assert np.array_equal(job.image(), before[0])
+    assert np.array_equal(job.norms(), before[1])
+    for a, b in zip(job.fields(), before[2]):
+        assert np.array_equal(a, b)
+    job.reset()
+    job.run()
+    assert np.array_equal(job.image(), before[0])
+    assert np.array_equal(job.norms(), before[1])
+    with pytest.raises(McsartError):
+        job.kernel_time("projector", reps=0)
+    job.close()
+    ctx.close()
