@@ -41,8 +41,9 @@ def algorithmic_bytes(spec, kind, layouts=None):
     warp                   : read 6 fields and x, write x_i in each layout the view's rays
                              read (1 or 2)                                 -> 28N + 4N per layout
     warp-add               : read 6 fields, u and x, write x               -> 36N
-    motion step            : read and write 6 fields, read u and x, write (g, q) in each layout
-                             the next view's rays read                     -> 56N + 8N per layout
+    motion step            : read u, read and write the two accumulators, read the constants
+                             (Ga, Gb, H, Q) and x, write (g, q) in each layout the next
+                             view's rays read                              -> 40N + 8N per layout
     `layouts` (per view, bit 0 natural, bit 1 transposed) averages the per-layout terms over the
     views the kernel is launched for.
     """
@@ -66,7 +67,7 @@ def algorithmic_bytes(spec, kind, layouts=None):
     if kind == "warp_add":
         return 36 * N
     if kind == "motion":
-        return 56 * N + 8 * N * mean_layouts(range(2, V))   # prep of views 2..V-1
+        return 40 * N + 8 * N * mean_layouts(range(2, V))   # prep of views 2..V-1
     raise KeyError(kind)
 
 
@@ -85,7 +86,7 @@ def profiled_traffic(workload, kind):
         return None
     prefix = {"projector": "k_project_rows<1,", "backprojector": "k_backproject<1,",
               "warp": "k_warp_tile", "warp_add": "k_warp_add_tile",
-              "motion": "k_motion_step<true, true>"}[kind]
+              "motion": "k_motion_acc<false, false>"}[kind]
     with open(path) as f:
         rows = json.load(f).get("kernels", [])
     best = [r for r in rows if r["kernel"].startswith(prefix) and "traffic_bytes" in r]

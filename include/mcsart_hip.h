@@ -344,9 +344,8 @@ int mcsart_job_autotune(mcsart_job *job, mcsart_tuning *chosen);
  * layouts the view's rays read, 3 warp-add (x += lambda W(u)), 4 motion step (field update of
  * view i fused with the prep of view i+1).  Every view of the scan (views 1..V-2 for kind 4) is
  * launched back to back `reps` times between two HIP events on the job's stream.  Outputs go to
- * scratch: kind 3 updates a copy of the image, kind 4 a temporary copy of the fields (6N floats,
- * allocated and freed by the call); the job's image, fields and norms are untouched.
- * Synchronous. */
+ * the job's scratch: kind 3 updates a copy of the image, kind 4 the motion sweep's accumulators;
+ * the job's image, fields and norms are untouched.  Synchronous. */
 int mcsart_job_kernel_time(mcsart_job *job, int kind, int reps, double *avg_us);
 /* Detector layouts each view's rays read: bit 0 the natural volume layout (y-branch rays), bit 1
  * the xy-transposed layout (x-branch rays); layouts[n_views]. */

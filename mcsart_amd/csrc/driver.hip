@@ -31,6 +31,8 @@ struct mcsart_job {
     float *d_w1 = nullptr;       // 2N: warped image (image sweep) / (g, q) pairs (motion sweep)
     float *d_w1T = nullptr;      // 2N: the same in the xy-transposed layout / transposed image
     float *d_w2 = nullptr;       // N: u
+    float *d_acc = nullptr;      // 2N: motion-sweep accumulators SV, SF
+    float4 *d_cst = nullptr;     // N: motion-sweep constants (Ga, Gb, H, Q)
     float2 *d_rp = nullptr;      // D
     float *d_norm = nullptr;     // n_iter*V
     float *d_resid = nullptr;    // V*D (keep_residual)
@@ -56,7 +58,7 @@ namespace {
 size_t align_up(size_t v) { return (v + 255) & ~(size_t)255; }
 
 struct Layout {
-    size_t proj, img, fields, init, w1, w1T, w2, rp, norm, resid, partials, total;
+    size_t proj, img, fields, init, w1, w1T, w2, acc, cst, rp, norm, resid, partials, total;
 };
 
 Layout layout(const Geom &g, int V, int n_iter, int keep_residual) {
@@ -75,6 +77,8 @@ Layout layout(const Geom &g, int V, int n_iter, int keep_residual) {
     L.w1 = take(sizeof(float) * N * 2);
     L.w1T = take(sizeof(float) * N * 2);
     L.w2 = take(sizeof(float) * N);
+    L.acc = take(sizeof(float) * N * 2);
+    L.cst = take(sizeof(float4) * N);
     L.rp = take(sizeof(float2) * D);
     L.norm = take(sizeof(float) * (size_t)n_iter * V);
     L.resid = take(keep_residual ? sizeof(float) * D * V : 0);
@@ -86,6 +90,12 @@ Layout layout(const Geom &g, int V, int n_iter, int keep_residual) {
 int check_job(mcsart_job *job) {
     if (!job) return fail(MCSART_EINVAL, "null job");
     return MCSART_OK;
+}
+
+FieldsW fields_w(mcsart_job *j) {
+    const size_t N = (size_t)j->g.nvox();
+    float *f = j->d_fields;
+    return FieldsW{f, f + N, f + 2 * N, f + 3 * N, f + 4 * N, f + 5 * N};
 }
 
 Fields fields_of(const mcsart_job *j) {
@@ -157,25 +167,27 @@ int image_sweep(mcsart_job *j, int it) {
 }
 
 // Motion (udvf) sweep (src/AUMISART.cu:236-288; static overload :66-118 steps views by 10).
-// Per view: prep (g, q) -> projector -> backprojector -> field update.  The update of one view
-// and the prep of the next run as one kernel (launch_motion_step), which reads the fields once.
+// Per view: prep (g, q) -> projector -> backprojector -> field update.  x is constant over the
+// sweep, so the updates are carried as two accumulators (launch_motion_acc): the per-sweep
+// constants and the first view's prep come from launch_motion_begin, each update is fused with
+// the next view's prep, and the sweep's last update writes the fields.
 int motion_sweep(mcsart_job *j, int it) {
     const Geom &g = j->g;
     const long long D = g.ndet(), N = g.nvox();
     hipStream_t st = j->ctx->stream;
-    float *fl[6];
-    for (int c = 0; c < 6; ++c) fl[c] = j->d_fields + c * N;
+    const FieldsW fw = fields_w(j);
+    float *sv = j->d_acc, *sf = j->d_acc + N;
     const int stride = j->opts.motion ? 1 : 10;
     float2 *pair = (float2 *)j->d_w1, *pairT = (float2 *)j->d_w1T;
     auto vdiff = [&](int iv) { return j->volumes[iv] - j->volumes[iv - 1]; };   // :249
     auto fdiff = [&](int iv) { return j->flows[iv] - j->flows[iv - 1]; };       // :250
     int rc, nparts = 0;
-    if (j->V > 1) {   // prep of the first view
+    if (j->V > 1) {   // constants and the prep of the first view
         const ViewInfo &vi = j->vinfo[1];
         TimedRegion tr(j->ctx, TK_MOTION);
-        rc = launch_motion_step(fl, nullptr, j->d_img, vi.has_y ? pair : nullptr,
-                                vi.has_x ? pairT : nullptr, g.nx, g.ny, g.nz, 0.f, 0.f, vdiff(1),
-                                fdiff(1), false, true, st);
+        rc = launch_motion_begin(fields_of(j), j->d_img, j->d_cst, vi.has_y ? pair : nullptr,
+                                 vi.has_x ? pairT : nullptr, g.nx, g.ny, g.nz, vdiff(1),
+                                 fdiff(1), st);
         if (rc) return rc;
     }
     for (int iv = 1; iv < j->V; iv += stride) {
@@ -204,11 +216,12 @@ int motion_sweep(mcsart_job *j, int it) {
             const bool more = nv < j->V;
             const ViewInfo &vn = j->vinfo[more ? nv : iv];
             TimedRegion tr(j->ctx, TK_MOTION);
-            rc = launch_motion_step(fl, j->d_w2, j->d_img, more && vn.has_y ? pair : nullptr,
-                                    more && vn.has_x ? pairT : nullptr, g.nx, g.ny, g.nz,
-                                    j->volumes[iv - 1] - j->volumes[iv],
-                                    j->flows[iv - 1] - j->flows[iv], more ? vdiff(nv) : 0.f,
-                                    more ? fdiff(nv) : 0.f, true, more, st);
+            rc = launch_motion_acc(sv, sf, j->d_w2, j->d_img, j->d_cst,
+                                   more && vn.has_y ? pair : nullptr,
+                                   more && vn.has_x ? pairT : nullptr, fw, g.nx, g.ny, g.nz,
+                                   j->volumes[iv - 1] - j->volumes[iv],
+                                   j->flows[iv - 1] - j->flows[iv], more ? vdiff(nv) : 0.f,
+                                   more ? fdiff(nv) : 0.f, iv == 1, !more, st);
             if (rc) return rc;
         }
     }
@@ -274,6 +287,8 @@ extern "C" int mcsart_job_create(mcsart_ctx *ctx, const mcsart_geom *gp, int n_v
     j->d_w1 = (float *)(b + L.w1);
     j->d_w1T = (float *)(b + L.w1T);
     j->d_w2 = (float *)(b + L.w2);
+    j->d_acc = (float *)(b + L.acc);
+    j->d_cst = (float4 *)(b + L.cst);
     j->d_rp = (float2 *)(b + L.rp);
     j->d_norm = (float *)(b + L.norm);
     j->d_resid = opts->keep_residual ? (float *)(b + L.resid) : nullptr;
@@ -554,23 +569,21 @@ extern "C" int mcsart_job_kernel_time(mcsart_job *j, int kind, int reps, double 
     MCSART_HIP_TRY(hipStreamSynchronize(st));
     rc = launch_transpose(j->d_w1T, j->d_img, g.nx, g.ny, g.nz, st);   // scratch copy
     if (rc) return rc;
-    // warp-add updates a scratch copy of the image; the motion step a scratch copy of the fields
-    float *tmp_fields = nullptr;
+    // warp-add updates a scratch copy of the image; the motion step works on the sweep's scratch
+    // accumulators and constants (made here as a motion sweep makes them)
     if (kind == 3)
         MCSART_HIP_TRY(hipMemcpyAsync(j->d_w1, j->d_img, sizeof(float) * N,
                                       hipMemcpyDeviceToDevice, st));
-    if (kind == 4) {
-        MCSART_HIP_TRY(hipMalloc(&tmp_fields, sizeof(float) * 6 * N));
-        if (hipMemcpyAsync(tmp_fields, j->d_fields, sizeof(float) * 6 * N,
-                           hipMemcpyDeviceToDevice, st) != hipSuccess) {
-            hipFree(tmp_fields);
-            return fail(MCSART_EHIP, "field copy failed");
-        }
-    }
     const Fields f = fields_of(j);
-    float *fl[6];
-    for (int c = 0; c < 6; ++c) fl[c] = tmp_fields ? tmp_fields + c * N : nullptr;
     float2 *pair = (float2 *)j->d_w1, *pairT = (float2 *)j->d_w1T;
+    if (kind == 4) {
+        if (j->V < 3) return fail(MCSART_EINVAL, "the motion step needs at least three views");
+        rc = launch_motion_begin(f, j->d_img, j->d_cst, pair, pairT, g.nx, g.ny, g.nz, 0.f, 0.f,
+                                 st);
+        if (rc) return rc;
+        MCSART_HIP_TRY(hipMemsetAsync(j->d_acc, 0, sizeof(float) * 2 * N, st));
+    }
+    const FieldsW fw = fields_w(j);
     // every view of the scan once per rep, back to back, as the sweeps launch them
     auto all_views = [&]() {
         for (int iv = 0; iv < j->V; ++iv) {
@@ -587,12 +600,12 @@ extern "C" int mcsart_job_kernel_time(mcsart_job *j, int kind, int reps, double 
             } else if (kind == 4) {   // update of view iv, prep of view iv + 1 (views 1..V-2)
                 if (iv == 0 || iv + 1 >= j->V) continue;
                 const ViewInfo &vn = j->vinfo[iv + 1];
-                r = launch_motion_step(fl, j->d_w2, j->d_img, vn.has_y ? pair : nullptr,
-                                       vn.has_x ? pairT : nullptr, g.nx, g.ny, g.nz,
-                                       j->volumes[iv - 1] - j->volumes[iv],
-                                       j->flows[iv - 1] - j->flows[iv],
-                                       j->volumes[iv + 1] - j->volumes[iv],
-                                       j->flows[iv + 1] - j->flows[iv], true, true, st);
+                r = launch_motion_acc(j->d_acc, j->d_acc + N, j->d_w2, j->d_img, j->d_cst,
+                                      vn.has_y ? pair : nullptr, vn.has_x ? pairT : nullptr, fw,
+                                      g.nx, g.ny, g.nz, j->volumes[iv - 1] - j->volumes[iv],
+                                      j->flows[iv - 1] - j->flows[iv],
+                                      j->volumes[iv + 1] - j->volumes[iv],
+                                      j->flows[iv + 1] - j->flows[iv], false, false, st);
             } else if (kind == 0) {
                 ProjArgs a{};
                 a.vol0 = j->d_img;
@@ -614,14 +627,8 @@ extern "C" int mcsart_job_kernel_time(mcsart_job *j, int kind, int reps, double 
     rc = all_views();   // warm-up
     float ms = 0.f;
     if (!rc) rc = time_on_stream(j, reps, all_views, &ms);
-    if (tmp_fields) {
-        if (hipStreamSynchronize(st) != hipSuccess && !rc)
-            rc = fail(MCSART_EHIP, "stream synchronisation failed");
-        hipFree(tmp_fields);
-    }
     if (rc) return rc;
-    const int launches = kind == 4 ? std::max(j->V - 2, 0) : j->V;
-    if (launches == 0) return fail(MCSART_EINVAL, "the motion step needs at least three views");
+    const int launches = kind == 4 ? j->V - 2 : j->V;
     *avg_us = 1e3 * (double)ms / ((double)reps * launches);
     return MCSART_OK;
 }

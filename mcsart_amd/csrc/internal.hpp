@@ -140,6 +140,9 @@ int launch_backprojector(const Geom &g, const ViewTrig &t, const ViewInfo &vi, B
 struct Fields {
     const float *ax, *ay, *az, *bx, *by, *bz;
 };
+struct FieldsW {   // the same six fields, writable
+    float *ax, *ay, *az, *bx, *by, *bz;
+};
 // out = W(src; vol, flow) in the natural layout and/or (outT) the xy-transposed layout; either
 // output may be null; neither may alias src.
 int launch_warp(float *out, float *outT, const float *src, int nx, int ny, int nz, float vol,
@@ -182,6 +185,19 @@ int launch_motion_update(float *const fields[6], const float *u, const float *x,
 int launch_motion_step(float *const fields[6], const float *u, const float *x, float2 *pair,
                        float2 *pairT, int nx, int ny, int nz, float wv, float wf, float vd,
                        float fd, bool update, bool prep, hipStream_t st);
+
+// The motion sweep in accumulator form (x constant over the sweep): per-sweep constants
+// cst = (Ga, Gb, H, Q) from the starting fields plus the prep of the first view ...
+int launch_motion_begin(const Fields &f, const float *x, float4 *cst, float2 *pair,
+                        float2 *pairT, int nx, int ny, int nz, float vd, float fd,
+                        hipStream_t st);
+// ... then per updated view SV += wv u^T, SF += wf u^T (from zero when `first`), followed by the
+// prep of the next view (vd, fd) or, when `final_step`, by writing the fields
+// alpha_c = alpha_c0 + D_c x SV, beta_c = beta_c0 + D_c x SF in place (f).
+int launch_motion_acc(float *sv, float *sf, const float *u, const float *x, const float4 *cst,
+                      float2 *pair, float2 *pairT, const FieldsW &f, int nx, int ny, int nz,
+                      float wv, float wf, float vd, float fd, bool first, bool final_step,
+                      hipStream_t st);
 
 // img1 *= w * D_ind(img0) (src/cu_multiple.cu:11-43)
 int launch_multiple(float *a, const float *b, int nx, int ny, int nz, float w, int ind,

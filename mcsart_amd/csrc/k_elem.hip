@@ -523,6 +523,182 @@ __global__ void __launch_bounds__(TT *TR) k_motion_step(float *__restrict__ ax,
     }
 }
 
+// The motion sweep in accumulator form.  x is constant over a motion sweep, so every update of
+// the sweep adds a multiple of the same D_c x to a field (src/AUMISART.cu:272-277):
+//   alpha_c(P) = alpha_c0(P) + D_c x(P) * SV(P),  SV(P) = sum over updated views of wv_j u_j(P^T)
+//   beta_c(P)  = beta_c0(P)  + D_c x(P) * SF(P),  SF(P) = sum of wf_j u_j(P^T)
+// with the fields' pairing (alpha_y, D1), (alpha_x, D2), (alpha_z, D3).  The prep of a view
+// (src/AUMISART.cu:252-258,267) then needs, at voxel p, only
+//   img0(p) = vd Ga(p) + fd Gb(p) + H(p) (vd SV(p^T) + fd SF(p^T)),
+//   Ga(p) = sum_c alpha_c0(p^T) D_c x(p),  Gb(p) = sum_c beta_c0(p^T) D_c x(p),
+//   H(p) = sum_c D_c x(p^T) D_c x(p),  Q(p) = (D1 + D2 + D3) x(p),
+// so a view step reads and writes two accumulators instead of six fields.  The per-sweep
+// constants (Ga, Gb, H, Q) are made once by k_motion_begin, and the last update writes the fields.
+
+// Store pair values (g, q) of the A^T lanes: natural layout at jd[k] and/or, through LDS, the
+// transposed layout at the A lanes' indices id[k].
+__device__ inline void store_pairs(float (*t0)[TT + 1], float (*t1)[TT + 1],
+                                   float2 *__restrict__ pair, float2 *__restrict__ pairT,
+                                   const float *gv, const float *qv, const int *id, const bool *ok,
+                                   const int *jd, const bool *jok) {
+    const int tx = threadIdx.x, ty = threadIdx.y;
+    if (pair) {
+#pragma unroll
+        for (int k = 0; k < TK; ++k)
+            if (jok[k]) pair[jd[k]] = make_float2(gv[k], qv[k]);
+    }
+    if (pairT) {
+        __syncthreads();   // every earlier read of t0/t1 is done
+#pragma unroll
+        for (int k = 0; k < TK; ++k) {
+            t0[ty + k * TR][tx] = gv[k];
+            t1[ty + k * TR][tx] = qv[k];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < TK; ++k) {
+            const int r = ty + k * TR;
+            if (ok[k]) pairT[id[k]] = make_float2(t0[tx][r], t1[tx][r]);
+        }
+    }
+}
+
+// Per-sweep constants C = (Ga, Gb, H, Q) at every voxel from the sweep's starting fields, and the
+// prep (g, q) of the sweep's first view (both accumulators still zero).
+__global__ void __launch_bounds__(TT *TR) k_motion_begin(Fields f, const float *__restrict__ x,
+                                                        float4 *__restrict__ cst,
+                                                        float2 *__restrict__ pair,
+                                                        float2 *__restrict__ pairT, int nx,
+                                                        int ny, int nz, float vd, float fd) {
+    __shared__ float t[9][TT][TT + 1];
+    const int X0 = blockIdx.x * TT, Y0 = blockIdx.y * TT, i2 = blockIdx.z;
+    const int tx = threadIdx.x, ty = threadIdx.y;
+    const int plane = nx * ny;
+    const float *const F[6] = {f.ax, f.ay, f.az, f.bx, f.by, f.bz};
+    int id[TK], jd[TK];
+    bool ok[TK], jok[TK];
+    // A side: starting fields and D x at P, staged so that the A^T lane of p finds them at p^T
+#pragma unroll
+    for (int k = 0; k < TK; ++k) {
+        const int i1 = Y0 + ty + k * TR;
+        ok[k] = X0 + tx < nx && i1 < ny;
+        id[k] = X0 + tx + i1 * nx + i2 * plane;
+        const Diffs d =
+            ok[k] ? diffs_at(x, id[k], X0 + tx, i1, i2, nx, ny, nz) : Diffs{0.f, 0.f, 0.f, 0.f};
+        const int r = ty + k * TR;
+#pragma unroll
+        for (int c = 0; c < 6; ++c) t[c][r][tx] = ok[k] ? F[c][id[k]] : 0.f;
+        t[6][r][tx] = d.d1;
+        t[7][r][tx] = d.d2;
+        t[8][r][tx] = d.d3;
+    }
+    __syncthreads();
+    float gv[TK], qv[TK];
+#pragma unroll
+    for (int k = 0; k < TK; ++k) {
+        const int r = ty + k * TR;
+        const int j0 = Y0 + tx, j1 = X0 + r;
+        jok[k] = j0 < nx && j1 < ny;
+        jd[k] = j0 + j1 * nx + i2 * plane;
+        const Diffs d =
+            jok[k] ? diffs_at(x, jd[k], j0, j1, i2, nx, ny, nz) : Diffs{0.f, 0.f, 0.f, 0.f};
+        // fields index order: ax ay az bx by bz; pairing (y, D1), (x, D2), (z, D3)
+        const float ga = t[1][tx][r] * d.d1 + t[0][tx][r] * d.d2 + t[2][tx][r] * d.d3;
+        const float gb = t[4][tx][r] * d.d1 + t[3][tx][r] * d.d2 + t[5][tx][r] * d.d3;
+        const float h = t[6][tx][r] * d.d1 + t[7][tx][r] * d.d2 + t[8][tx][r] * d.d3;
+        const float q = d.d1 + d.d2 + d.d3;
+        if (jok[k]) cst[jd[k]] = make_float4(ga, gb, h, q);
+        gv[k] = d.x - (vd * ga + fd * gb);
+        qv[k] = q * (-vd + -fd);
+    }
+    store_pairs(t[0], t[1], pair, pairT, gv, qv, id, ok, jd, jok);
+}
+
+// One view step of the accumulator form: SV += wv u^T, SF += wf u^T on A (FIRST: from zero),
+// then either the prep of the next view on A^T (FINAL = false) or, for the sweep's last update,
+// the fields alpha_c0 + D_c x SV, beta_c0 + D_c x SF written on A (FINAL = true).
+template <bool FIRST, bool FINAL>
+__global__ void __launch_bounds__(TT *TR) k_motion_acc(float *__restrict__ sv_acc,
+                                                      float *__restrict__ sf_acc,
+                                                      const float *__restrict__ u,
+                                                      const float *__restrict__ x,
+                                                      const float4 *__restrict__ cst,
+                                                      float2 *__restrict__ pair,
+                                                      float2 *__restrict__ pairT, FieldsW f,
+                                                      int nx, int ny, int nz, float wv, float wf,
+                                                      float vd, float fd) {
+    __shared__ float t[2][TT][TT + 1];
+    const int X0 = blockIdx.x * TT, Y0 = blockIdx.y * TT, i2 = blockIdx.z;
+    const int tx = threadIdx.x, ty = threadIdx.y;
+    const int plane = nx * ny;
+    int id[TK];
+    bool ok[TK];
+    float sv[TK], sf[TK];
+    load_transposed(t[0], u, nx, ny, X0, Y0, i2);
+#pragma unroll
+    for (int k = 0; k < TK; ++k) {
+        const int i1 = Y0 + ty + k * TR;
+        ok[k] = X0 + tx < nx && i1 < ny;
+        id[k] = X0 + tx + i1 * nx + i2 * plane;
+        sv[k] = (!FIRST && ok[k]) ? sv_acc[id[k]] : 0.f;
+        sf[k] = (!FIRST && ok[k]) ? sf_acc[id[k]] : 0.f;
+    }
+    __syncthreads();   // the u tile is complete
+#pragma unroll
+    for (int k = 0; k < TK; ++k) {
+        const float ut = t[0][tx][ty + k * TR];
+        sv[k] += wv * ut;
+        sf[k] += wf * ut;
+    }
+    if (FINAL) {
+        float *const F[6] = {f.ax, f.ay, f.az, f.bx, f.by, f.bz};
+#pragma unroll
+        for (int k = 0; k < TK; ++k) {
+            if (!ok[k]) continue;
+            const Diffs d = diffs_at(x, id[k], X0 + tx, Y0 + ty + k * TR, i2, nx, ny, nz);
+            const float a0 = F[0][id[k]], a1 = F[1][id[k]], a2 = F[2][id[k]];
+            const float b0 = F[3][id[k]], b1 = F[4][id[k]], b2 = F[5][id[k]];
+            F[1][id[k]] = a1 + d.d1 * sv[k];   // alpha_y with D1
+            F[0][id[k]] = a0 + d.d2 * sv[k];   // alpha_x with D2
+            F[2][id[k]] = a2 + d.d3 * sv[k];   // alpha_z with D3
+            F[4][id[k]] = b1 + d.d1 * sf[k];   // beta_y
+            F[3][id[k]] = b0 + d.d2 * sf[k];   // beta_x
+            F[5][id[k]] = b2 + d.d3 * sf[k];   // beta_z
+        }
+        return;
+    }
+#pragma unroll
+    for (int k = 0; k < TK; ++k)
+        if (ok[k]) {
+            sv_acc[id[k]] = sv[k];
+            sf_acc[id[k]] = sf[k];
+        }
+    __syncthreads();   // every lane is done with the u tile
+#pragma unroll
+    for (int k = 0; k < TK; ++k) {
+        t[0][ty + k * TR][tx] = sv[k];
+        t[1][ty + k * TR][tx] = sf[k];
+    }
+    __syncthreads();
+    // lane (tx, r) owns p = (Y0 + tx, X0 + r) of A^T; SV(p^T), SF(p^T) are at t[.][tx][r]
+    float gv[TK], qv[TK];
+    int jd[TK];
+    bool jok[TK];
+#pragma unroll
+    for (int k = 0; k < TK; ++k) {
+        const int r = ty + k * TR;
+        const int j0 = Y0 + tx, j1 = X0 + r;
+        jok[k] = j0 < nx && j1 < ny;
+        jd[k] = j0 + j1 * nx + i2 * plane;
+        const float4 c = jok[k] ? cst[jd[k]] : make_float4(0.f, 0.f, 0.f, 0.f);
+        const float xv = jok[k] ? x[jd[k]] : 0.f;
+        const float img0 = vd * c.x + fd * c.y + c.z * (vd * t[0][tx][r] + fd * t[1][tx][r]);
+        gv[k] = xv - img0;
+        qv[k] = c.w * (-vd + -fd);
+    }
+    store_pairs(t[0], t[1], pair, pairT, gv, qv, id, ok, jd, jok);
+}
+
 __global__ void __launch_bounds__(TT *TR) k_add_tile(float *__restrict__ x, float *__restrict__ xT,
                                                     const float *__restrict__ u, int nx, int ny,
                                                     int nz, float lambda, int nonneg) {
@@ -718,6 +894,32 @@ int launch_motion_update(float *const fields[6], const float *u, const float *x,
                          int nz, float wv, float wf, hipStream_t st) {
     return launch_motion_step(fields, u, x, nullptr, nullptr, nx, ny, nz, wv, wf, 0.f, 0.f, true,
                               false, st);
+}
+
+int launch_motion_begin(const Fields &f, const float *x, float4 *cst, float2 *pair,
+                        float2 *pairT, int nx, int ny, int nz, float vd, float fd,
+                        hipStream_t st) {
+    hipLaunchKernelGGL(k_motion_begin, tile_grid(nx, ny, nz), dim3(TT, TR), 0, st, f, x, cst,
+                       pair, pairT, nx, ny, nz, vd, fd);
+    MCSART_CHECK_LAUNCH();
+    return MCSART_OK;
+}
+
+int launch_motion_acc(float *sv, float *sf, const float *u, const float *x, const float4 *cst,
+                      float2 *pair, float2 *pairT, const FieldsW &f, int nx, int ny, int nz,
+                      float wv, float wf, float vd, float fd, bool first, bool final_step,
+                      hipStream_t st) {
+    const dim3 grid = tile_grid(nx, ny, nz), block(TT, TR);
+#define MCSART_MOTION_ACC(A, B)                                                                  \
+    hipLaunchKernelGGL((k_motion_acc<A, B>), grid, block, 0, st, sv, sf, u, x, cst, pair, pairT, \
+                       f, nx, ny, nz, wv, wf, vd, fd)
+    if (first && final_step) MCSART_MOTION_ACC(true, true);
+    else if (first) MCSART_MOTION_ACC(true, false);
+    else if (final_step) MCSART_MOTION_ACC(false, true);
+    else MCSART_MOTION_ACC(false, false);
+#undef MCSART_MOTION_ACC
+    MCSART_CHECK_LAUNCH();
+    return MCSART_OK;
 }
 
 int launch_motion_step(float *const fields[6], const float *u, const float *x, float2 *pair,

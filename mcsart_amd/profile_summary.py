@@ -49,6 +49,12 @@ def algorithmic_bytes(kernel: str, N: int, D: int):
         "k_motion_step<true, true>": 72 * N,    # 6 fields read+written, u, x, pairs
         "k_motion_step<true, false>": 56 * N,
         "k_motion_step<false, true>": 40 * N,
+        # accumulator form, one pair layout: u, SV/SF read+written, (Ga, Gb, H, Q), x, pairs
+        "k_motion_acc<false, false>": 48 * N,
+        "k_motion_acc<true, false>": 40 * N,    # first update: accumulators not read
+        "k_motion_acc<false, true>": 64 * N,    # last update: u, SV/SF, x, fields read+written
+        "k_motion_acc<true, true>": 56 * N,
+        "k_motion_begin": 52 * N,               # fields, x read; constants, pairs written
         "k_transpose": 8 * N,
         "k_add_tile": 12 * N,
     }
