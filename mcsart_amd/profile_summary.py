@@ -133,10 +133,31 @@ def to_markdown(s, command_lines):
     return "\n".join(out) + "\n"
 
 
+def stats_markdown(trace, title, command_lines, notes=()):
+    """Plain rocprofv3 --stats table (every kernel of a run, by total time)."""
+    rows = list(csv.DictReader(open(os.path.join(trace, "run_kernel_stats.csv"))))
+    rows.sort(key=lambda r: -float(r["TotalDurationNs"]))
+    out = [f"# {title}", "", "Commands (on the MI355X box, from /tmp, PYTHONPATH = repository root):",
+           ""]
+    out += ["    " + c for c in command_lines]
+    out += [""] + list(notes) + ["" if notes else None]
+    out = [o for o in out if o is not None]
+    out += ["| kernel | calls | avg us | min us | max us | total ms | % of kernel time |",
+            "|---|---:|---:|---:|---:|---:|---:|"]
+    for r in rows:
+        out.append(f"| `{short(r['Name'])}` | {r['Calls']} | {float(r['AverageNs']) / 1e3:.2f} | "
+                   f"{float(r['MinNs']) / 1e3:.2f} | {float(r['MaxNs']) / 1e3:.2f} | "
+                   f"{float(r['TotalDurationNs']) / 1e6:.1f} | {float(r['Percentage']):.2f} |")
+    return "\n".join(out) + "\n"
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
-    ap.add_argument("--workload", required=True, choices=sorted(WORKLOADS))
-    ap.add_argument("--trace", required=True)
+    ap.add_argument("--stats", help="write only the --stats table of this trace directory")
+    ap.add_argument("--title", default="rocprofv3 kernel statistics")
+    ap.add_argument("--note", action="append", default=[])
+    ap.add_argument("--workload", choices=sorted(WORKLOADS))
+    ap.add_argument("--trace")
     ap.add_argument("--fetch")
     ap.add_argument("--write")
     ap.add_argument("--views", type=int, default=0)
@@ -145,6 +166,16 @@ def main(argv=None):
     ap.add_argument("--command", action="append", default=[],
                     help="command line that produced the inputs (repeatable)")
     args = ap.parse_args(argv)
+    if args.stats:
+        md = stats_markdown(args.stats, args.title, args.command, args.note)
+        if args.md:
+            with open(args.md, "w") as f:
+                f.write(md)
+        else:
+            print(md)
+        return
+    if not args.workload or not args.trace:
+        ap.error("--workload and --trace are required without --stats")
     s = summarise(args.workload, args.trace, args.fetch, args.write, views=args.views)
     if args.json:
         with open(args.json, "w") as f:
