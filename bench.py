@@ -78,9 +78,10 @@ def step_bytes(spec):
     return 84 * g.nvox + 28 * g.ndet, 128 * g.nvox + 44 * g.ndet
 
 
-def profiled_traffic(workload, kind):
-    """HBM bytes per launch of the kernel, from the committed rocprofv3 counter summary of this
-    workload (None when there is none)."""
+def profiled(workload, kind, field="traffic_bytes"):
+    """A per-launch figure of the kernel from the committed rocprofv3 counter summary of this
+    workload: HBM bytes (FETCH_SIZE + WRITE_SIZE, calibrated) or the VALU-busy fraction (SQ
+    counters).  None when the summary or the figure is absent."""
     path = os.path.join(ROOT, "profiles", f"{workload}_kernels.json")
     if not os.path.exists(path):
         return None
@@ -89,11 +90,12 @@ def profiled_traffic(workload, kind):
               "motion": "k_motion_acc<false, false>"}[kind]
     with open(path) as f:
         rows = json.load(f).get("kernels", [])
-    best = [r for r in rows if r["kernel"].startswith(prefix) and "traffic_bytes" in r]
+    best = [r for r in rows if r["kernel"].startswith(prefix) and field in r]
     if not best:
         return None
     best.sort(key=lambda r: -r["launches"])
-    return int(best[0]["traffic_bytes"])
+    v = best[0][field]
+    return int(v) if field == "traffic_bytes" else v
 
 
 def concurrent_scans(ctx, spec, g, data, proj, tuning, jobs, rounds):
@@ -295,7 +297,8 @@ def main():
             "achieved": round(ach, 1), "frac": round(ach / HBM_PEAK_GBS, 4),
             "launches_per_scan": per_scan[k],
             "ms_per_scan": round(us * per_scan[k] * 1e-3, 2),
-            "traffic": profiled_traffic(spec.name, k)}
+            "traffic": profiled(spec.name, k),
+            "valu_busy": profiled(spec.name, k, "valu_busy")}
     dom = max(kernels_roofline, key=lambda k: kernels_roofline[k]["ms_per_scan"])
     d = kernels_roofline[dom]
     traffic = d["traffic"]
@@ -306,6 +309,10 @@ def main():
         "timing": "HIP events around all views launched back to back, 3 repetitions",
         "traffic_source": (f"profiles/{spec.name}_kernels.json (rocprofv3 FETCH_SIZE + "
                            "WRITE_SIZE, calibrated on a 64 MiB copy)") if traffic else None,
+        "valu_busy": d["valu_busy"],
+        "valu_busy_source": (f"profiles/{spec.name}_kernels.json (rocprofv3 SQ_INSTS_VALU, "
+                             "SQ_BUSY_CYCLES: share of SIMD cycles issuing vector "
+                             "instructions)") if d["valu_busy"] is not None else None,
     }
     kernel_ms = {k: {"total_ms": round(v[0], 2), "launches": v[1],
                      "avg_us": round(v[0] * 1e3 / max(v[1], 1), 2)} for k, v in times.items()}

@@ -233,17 +233,22 @@ __global__ void __launch_bounds__(BX *BY) k_backproject(GeomArgs g, double c, do
             float acc = 0.f, acc1 = 0.f;
             if (!wide) {
                 // two rows per step, all their loads in flight; a second row past the end reads
-                // the first again with weight F(lb+2) - F(lb+1) = 0
+                // the first again with weight F(lb+2) - F(lb+1) = 0.  F at the step's upper
+                // boundary is the next step's lower one (lbf counts rows in float: exact).
+                float Fa = F(0), lbf = 0.f;
                 for (int lb = 0; lb < n; lb += 2) {
                     const int r0 = ib_lo + lb - rb0;
                     const int r1 = ib_lo + min(lb + 1, n - 1) - rb0;
-                    const float Fa = F(lb), Fb = F(lb + 1), Fc = F(lb + 2);
+                    const float Fb = fminf(fmaxf(lbf + 1.f, bminL), bhi);
+                    const float Fc = fminf(fmaxf(lbf + 2.f, bminL), bhi);
                     const float2 s0 = row_sum(r0), s1 = row_sum(r1);
                     const float w0 = Fb - Fa, w1 = Fc - Fb;
                     acc = fmaf(w0, s0.x, acc);
                     acc1 = fmaf(w0, s0.y, acc1);
                     acc = fmaf(w1, s1.x, acc);
                     acc1 = fmaf(w1, s1.y, acc1);
+                    Fa = Fc;
+                    lbf += 2.f;
                 }
             } else {
                 for (int lb = 0; lb < n; ++lb) {
@@ -261,7 +266,9 @@ __global__ void __launch_bounds__(BX *BY) k_backproject(GeomArgs g, double c, do
             if (MODE == BACK_PLAIN) {
                 out[id] = acc;
             } else {
-                float u = acc / acc1;   // src/cu_division.cu:20-24
+                // src/cu_division.cu:20-24, with the hardware reciprocal (<= 2 ulp from the
+                // IEEE quotient); 0/0 and x/0 still give NaN / Inf, mapped to 0
+                float u = acc * __builtin_amdgcn_rcpf(acc1);
                 if (isnan(u) || isinf(u)) u = 0.f;
                 out[id] = u;
             }

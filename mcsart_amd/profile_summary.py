@@ -81,13 +81,34 @@ def load_pmc(d):
     return {k: sum(v) / len(v) for k, v in per.items()}, calib
 
 
-def summarise(workload, trace, fetch, write, copy_mb=64, views=0):
+SIMDS = 1024          # 256 CUs x 4 SIMDs
+SQ_INSTANCES = 32     # SQ_BUSY_CYCLES is summed over the 32 shader engines
+VALU_CYCLES = 4       # a wave64 VALU instruction occupies a SIMD16 for 4 cycles (no packed FP32)
+
+
+def load_sq(d):
+    """Per kernel, the average per dispatch of every SQ counter in a --pmc run."""
+    per = collections.defaultdict(lambda: collections.defaultdict(list))
+    for r in csv.DictReader(open(os.path.join(d, "run_counter_collection.csv"))):
+        per[short(r["Kernel_Name"])][r["Counter_Name"]].append(float(r["Counter_Value"]))
+    return {k: {c: sum(v) / len(v) for c, v in cs.items()} for k, cs in per.items()}
+
+
+def valu_busy(c):
+    """Fraction of the SIMDs' cycles spent issuing VALU instructions while the kernel ran."""
+    if "SQ_INSTS_VALU" not in c or not c.get("SQ_BUSY_CYCLES"):
+        return None
+    return VALU_CYCLES * c["SQ_INSTS_VALU"] / (SIMDS * c["SQ_BUSY_CYCLES"] / SQ_INSTANCES)
+
+
+def summarise(workload, trace, fetch, write, copy_mb=64, views=0, sq=None):
     spec = WORKLOADS[workload]()
     g = spec.geometry
     N, D = g.nvox, g.ndet
     stats = load_stats(trace)
     fe, fcal = load_pmc(fetch) if fetch else ({}, None)
     wr, wcal = load_pmc(write) if write else ({}, None)
+    sqc = load_sq(sq) if sq else {}
     true_bytes = copy_mb << 20
     ffac = true_bytes / fcal if fcal else None
     wfac = true_bytes / wcal if wcal else None
@@ -105,6 +126,13 @@ def summarise(workload, trace, fetch, write, copy_mb=64, views=0):
             row["fetch_bytes"] = int(fe[k] * ffac)
         if k in wr and wfac:
             row["write_bytes"] = int(wr[k] * wfac)
+        if k in sqc:
+            vb = valu_busy(sqc[k])
+            if vb is not None:
+                row["valu_busy"] = round(vb, 3)
+            waves = sqc[k].get("SQ_WAVES")
+            if waves and "SQ_INSTS_VALU" in sqc[k]:
+                row["valu_per_wave"] = round(sqc[k]["SQ_INSTS_VALU"] / waves, 1)
         if "fetch_bytes" in row and "write_bytes" in row:
             row["traffic_bytes"] = row["fetch_bytes"] + row["write_bytes"]
             if alg:
@@ -120,16 +148,19 @@ def to_markdown(s, command_lines):
            "", "Commands (on the MI355X box, from /tmp, PYTHONPATH = repository root):", ""]
     out += ["    " + c for c in command_lines]
     out += ["", f"Counter calibration against the run's final 64 MiB device copy: FETCH_SIZE x "
-            f"{s['fetch_factor']}, WRITE_SIZE x {s['write_factor']}.", "",
+            f"{s['fetch_factor']}, WRITE_SIZE x {s['write_factor']}.  VALU busy = "
+            "4 x SQ_INSTS_VALU / (1024 SIMDs x SQ_BUSY_CYCLES / 32): the share of the SIMDs' "
+            "cycles spent issuing vector instructions.", "",
             "| kernel | launches | avg us | algorithmic MB | achieved GB/s | frac of 8 TB/s | "
-            "fetch MB | write MB | traffic / algorithmic |",
-            "|---|---:|---:|---:|---:|---:|---:|---:|---:|"]
+            "fetch MB | write MB | traffic / algorithmic | VALU busy |",
+            "|---|---:|---:|---:|---:|---:|---:|---:|---:|---:|"]
     for r in s["kernels"]:
         mb = lambda b: "" if b is None else f"{b / 1e6:.2f}"
         out.append(f"| `{r['kernel']}` | {r['launches']} | {r['avg_us']} | "
                    f"{mb(r.get('algorithmic_bytes'))} | {r.get('achieved_gbs', '')} | "
                    f"{r.get('frac_hbm_peak', '')} | {mb(r.get('fetch_bytes'))} | "
-                   f"{mb(r.get('write_bytes'))} | {r.get('traffic_over_algorithmic', '')} |")
+                   f"{mb(r.get('write_bytes'))} | {r.get('traffic_over_algorithmic', '')} | "
+                   f"{r.get('valu_busy', '')} |")
     return "\n".join(out) + "\n"
 
 
@@ -160,6 +191,7 @@ def main(argv=None):
     ap.add_argument("--trace")
     ap.add_argument("--fetch")
     ap.add_argument("--write")
+    ap.add_argument("--sq", help="rocprofv3 --pmc run with SQ_INSTS_VALU, SQ_BUSY_CYCLES, SQ_WAVES")
     ap.add_argument("--views", type=int, default=0)
     ap.add_argument("--md")
     ap.add_argument("--json")
@@ -176,7 +208,7 @@ def main(argv=None):
         return
     if not args.workload or not args.trace:
         ap.error("--workload and --trace are required without --stats")
-    s = summarise(args.workload, args.trace, args.fetch, args.write, views=args.views)
+    s = summarise(args.workload, args.trace, args.fetch, args.write, views=args.views, sq=args.sq)
     if args.json:
         with open(args.json, "w") as f:
             json.dump(s, f, indent=1)

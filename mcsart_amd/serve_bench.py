@@ -42,8 +42,11 @@ class Scans:
         key = (spec.name, spec.n_views, tuple(spec.op_iter))
         if key not in self.cache:
             t = time.perf_counter()
+            log(f"[serve_bench] making inputs {spec.name} {spec.nx}x{spec.ny}x{spec.nz} "
+                f"V={spec.n_views} ...")
             d = make_scan_inputs(spec)
-            p = measure(spec, d, spec.seed + 404)
+            log(f"[serve_bench] {spec.name}: phantom and fields in {time.perf_counter() - t:.1f} s")
+            p = measure(spec, d, spec.seed + 404, log=log)
             self.cache[key] = (spec, d, p)
             log(f"[serve_bench] inputs {spec.name} {spec.nx}x{spec.ny}x{spec.nz} det "
                 f"{spec.na}x{spec.nb} V={spec.n_views}: {time.perf_counter() - t:.1f} s")
@@ -129,11 +132,17 @@ def run_serve(args):
             with Server(slots=slots, autotune=args.autotune) as srv:
                 t0 = time.perf_counter()
                 tickets = []
-                for a, sc in zip(arrivals, inputs):
+                for k, (a, sc) in enumerate(zip(arrivals, inputs)):
                     dt = a - (time.perf_counter() - t0)
                     if dt > 0:
                         time.sleep(dt)
                     tickets.append(submit(srv, sc))
+                    log(f"[serve_bench] rate {rate}/s {label}: request {k} ({sc[0].name}) "
+                        f"submitted at {time.perf_counter() - t0:.1f} s")
+                for k, t in enumerate(tickets):   # in submission order, reporting progress
+                    srv.wait(t)
+                    log(f"[serve_bench] rate {rate}/s {label}: request {k} done by "
+                        f"{time.perf_counter() - t0:.1f} s")
                 srv.drain()
                 t1 = time.perf_counter()
                 infos = [srv.info(t) for t in tickets]

@@ -13,9 +13,10 @@ from .ops import DeviceArray, host_deform, host_projection
 from .synth import poisson_noise
 
 
-def measure(spec: ScanSpec, data: ScanData, photons_seed: int) -> np.ndarray:
+def measure(spec: ScanSpec, data: ScanData, photons_seed: int, log=None) -> np.ndarray:
     """(V, nb, na) float32 projections of ``data.phantom`` warped per view, plus Poisson noise
-    at ``spec.noise_photons`` photons per ray when it is set."""
+    at ``spec.noise_photons`` photons per ray when it is set.  ``log(msg)``, when given, is
+    called every 100 views and around the noise step (large scans take minutes)."""
     g = spec.geometry
     d_ph = DeviceArray.from_host(data.phantom)
     d_w = DeviceArray.empty(g.vol_shape)
@@ -32,9 +33,15 @@ def measure(spec: ScanSpec, data: ScanData, photons_seed: int) -> np.ndarray:
             host_projection(d_p, src, float(a), g.SO, g.SD, g.da, g.na, g.ai, g.db, g.nb, g.bi,
                             g.nx, g.ny, g.nz)
             out[i] = d_p.to_host()
+            if log and (i + 1) % 100 == 0:
+                log(f"[measure] {spec.name}: {i + 1}/{spec.n_views} views projected")
     finally:
         for x in [d_ph, d_w, d_p] + (d_f or []):
             x.close()
     if spec.noise_photons:
+        if log:
+            log(f"[measure] {spec.name}: Poisson noise on {out.size} rays")
         out = poisson_noise(out, photons_seed, photons=spec.noise_photons, dx_mm=spec.vx_mm)
+        if log:
+            log(f"[measure] {spec.name}: noise done")
     return out
