@@ -41,7 +41,8 @@ def measure(spec: ScanSpec, data: ScanData, photons_seed: int, log=None) -> np.n
     if spec.noise_photons:
         if log:
             log(f"[measure] {spec.name}: Poisson noise on {out.size} rays")
-        out = poisson_noise(out, photons_seed, photons=spec.noise_photons, dx_mm=spec.vx_mm)
+        out = poisson_noise(out, photons_seed, photons=spec.noise_photons, dx_mm=spec.vx_mm,
+                            log=log)
         if log:
             log(f"[measure] {spec.name}: noise done")
     return out

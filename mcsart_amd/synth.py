@@ -58,6 +58,36 @@ class SplitMix64:
         z = np.concatenate([r * np.cos(2 * np.pi * u2), r * np.sin(2 * np.pi * u2)])
         return z[:n]
 
+    def _uniform_at(self, k0: int, n: int) -> np.ndarray:
+        """Uniforms of counters k0+1 .. k0+n, without advancing the stream."""
+        k = np.arange(k0 + 1, k0 + 1 + n, dtype=np.uint64)
+        with np.errstate(over="ignore"):
+            z = self.state + k * _GOLDEN
+            z = (z ^ (z >> np.uint64(30))) * _M1
+            z = (z ^ (z >> np.uint64(27))) * _M2
+            z = z ^ (z >> np.uint64(31))
+        return (z >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+
+    def normal_range(self, n: int, s: int, e: int) -> np.ndarray:
+        """Entries s .. e-1 of what normal(n) would return now, without advancing the stream
+        (normal(n) takes m = ceil(n/2) pairs: entry i < m is r_i cos, entry i >= m is
+        r_(i-m) sin)."""
+        m = (n + 1) // 2
+        out = np.empty(e - s, dtype=np.float64)
+        for lo, hi, first in ((s, min(e, m), True), (max(s, m), e, False)):
+            if lo >= hi:
+                continue
+            j0, j1 = (lo, hi) if first else (lo - m, hi - m)
+            u1 = self._uniform_at(self.counter + j0, j1 - j0)
+            u2 = self._uniform_at(self.counter + m + j0, j1 - j0)
+            r = np.sqrt(-2.0 * np.log1p(-u1))
+            out[lo - s:hi - s] = r * (np.cos if first else np.sin)(2 * np.pi * u2)
+        return out
+
+    def skip_normal(self, n: int) -> None:
+        """Advance the stream past what normal(n) consumes."""
+        self.counter += 2 * ((n + 1) // 2)
+
     def integers(self, n: int, lo: int, hi: int) -> np.ndarray:
         """Uniform integers in [lo, hi]."""
         return (lo + np.floor(self.uniform(n) * (hi - lo + 1))).astype(np.int64)
@@ -134,15 +164,24 @@ def motion_fields(nx, ny, nz, vox: Voxel, seed: int, si_mm=12.0, ap_mm=4.0, scal
     return fx, fy, fzz
 
 
-def poisson_noise(proj: np.ndarray, seed: int, photons=1e5, dx_mm=1.0) -> np.ndarray:
+def poisson_noise(proj: np.ndarray, seed: int, photons=1e5, dx_mm=1.0, log=None,
+                  block=1 << 24) -> np.ndarray:
     """counts ~ Poisson(I0 exp(-p dx)), p~ = -ln(max(counts, 1)/I0)/dx.  The Poisson draw uses the
     normal approximation N(lam, lam) rounded to an integer (lam >= ~600 for the configs' data);
-    exact inversion is used where lam < 50."""
+    exact inversion is used where lam < 50.  Rays are drawn in blocks of `block` (the same
+    numbers as one draw over all of them); `log(msg)`, when given, reports each block."""
     rng = SplitMix64(seed)
     p = np.asarray(proj, dtype=np.float64)
     lam = photons * np.exp(-p * dx_mm)
     flat = lam.ravel()
-    counts = np.rint(flat + np.sqrt(flat) * rng.normal(flat.size))
+    n = flat.size
+    counts = np.empty(n, dtype=np.float64)
+    for s in range(0, n, block):
+        e = min(n, s + block)
+        counts[s:e] = np.rint(flat[s:e] + np.sqrt(flat[s:e]) * rng.normal_range(n, s, e))
+        if log and e < n:
+            log(f"[poisson_noise] {e}/{n} rays")
+    rng.skip_normal(n)
     small = np.nonzero(flat < 50.0)[0]
     if small.size:
         u = rng.uniform(small.size)

@@ -216,3 +216,21 @@ def test_gateway_parser_rules():
     del it["n_views"]   # op_iter is checked first (src/entry_AUMISART.cu:119-128)
     with pytest.raises(MexBadInput, match="op_iter"):
         parse_iter(it)
+
+
+def test_poisson_noise_blocks_draw_the_same_numbers():
+    """Blocked Poisson noise (progress on large scans) equals the one-draw form: the normal
+    sequence is rebuilt from the counter-based stream, so the synthetic data do not depend on
+    the block size."""
+    from mcsart_amd.synth import SplitMix64, poisson_noise
+    rng = SplitMix64(123)
+    for n in (1, 2, 7, 10, 11):
+        ref = SplitMix64(99).normal(n)
+        r = SplitMix64(99)
+        for b in (1, 2, 3, 5, n):
+            got = np.concatenate([r.normal_range(n, s, min(n, s + b)) for s in range(0, n, b)])
+            assert np.array_equal(got, ref), (n, b)
+    proj = rng.uniform(3 * 5 * 7).reshape(3, 5, 7) * 8.0   # includes lam < 50 rays
+    a = poisson_noise(proj, 5, photons=1e4, dx_mm=1.0, block=1 << 24)
+    for b in (1, 4, 13, 105):
+        assert np.array_equal(poisson_noise(proj, 5, photons=1e4, dx_mm=1.0, block=b), a), b
