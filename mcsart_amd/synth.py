@@ -164,6 +164,21 @@ def motion_fields(nx, ny, nz, vox: Voxel, seed: int, si_mm=12.0, ap_mm=4.0, scal
     return fx, fy, fzz
 
 
+def poisson_inversion(lam: np.ndarray, u: np.ndarray) -> np.ndarray:
+    """Poisson draws by inversion of the CDF, all rays at once: the smallest n with
+    u <= sum_{k<=n} e^-lam lam^k / k! (at most 1000), each term formed as term *= lam / n."""
+    term = np.array([math.exp(-x) for x in lam], dtype=np.float64)
+    cdf = term.copy()
+    n = np.zeros(lam.size, dtype=np.float64)
+    active = np.nonzero(u > cdf)[0]
+    while active.size:
+        n[active] += 1.0
+        term[active] *= lam[active] / n[active]
+        cdf[active] += term[active]
+        active = active[(u[active] > cdf[active]) & (n[active] < 1000.0)]
+    return n
+
+
 def poisson_noise(proj: np.ndarray, seed: int, photons=1e5, dx_mm=1.0, log=None,
                   block=1 << 24) -> np.ndarray:
     """counts ~ Poisson(I0 exp(-p dx)), p~ = -ln(max(counts, 1)/I0)/dx.  The Poisson draw uses the
@@ -185,15 +200,11 @@ def poisson_noise(proj: np.ndarray, seed: int, photons=1e5, dx_mm=1.0, log=None,
     small = np.nonzero(flat < 50.0)[0]
     if small.size:
         u = rng.uniform(small.size)
-        for j, k in enumerate(small):   # inversion of the Poisson CDF
-            lk = flat[k]
-            term = math.exp(-lk)
-            cdf = term
-            n = 0
-            while u[j] > cdf and n < 1000:
-                n += 1
-                term *= lk / n
-                cdf += term
-            counts[k] = n
+        sb = 1 << 22   # each draw depends only on its own ray: blocks change nothing
+        for s in range(0, small.size, sb):
+            e = min(small.size, s + sb)
+            counts[small[s:e]] = poisson_inversion(flat[small[s:e]], u[s:e])
+            if log:
+                log(f"[poisson_noise] low-count rays: {e}/{small.size}")
     counts = np.maximum(counts.reshape(p.shape), 1.0)
     return (-np.log(counts / photons) / dx_mm).astype(np.float32)

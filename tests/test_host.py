@@ -234,3 +234,23 @@ def test_poisson_noise_blocks_draw_the_same_numbers():
     a = poisson_noise(proj, 5, photons=1e4, dx_mm=1.0, block=1 << 24)
     for b in (1, 4, 13, 105):
         assert np.array_equal(poisson_noise(proj, 5, photons=1e4, dx_mm=1.0, block=b), a), b
+
+
+def test_poisson_inversion_matches_scalar_loop():
+    """The vectorised Poisson inversion for low-count rays performs, ray by ray, the scalar
+    loop's float64 operations (term *= lam / n; cdf += term), so its draws are identical."""
+    import math
+    from mcsart_amd.synth import SplitMix64, poisson_inversion
+    rng = SplitMix64(7)
+    lam = np.concatenate([rng.uniform(200, 0.0, 50.0), [0.0, 1e-9, 49.999]])
+    u = rng.uniform(lam.size)
+    ref = np.empty(lam.size)
+    for j, lk in enumerate(lam):
+        term = math.exp(-lk)
+        cdf, n = term, 0
+        while u[j] > cdf and n < 1000:
+            n += 1
+            term *= lk / n
+            cdf += term
+        ref[j] = n
+    assert np.array_equal(poisson_inversion(lam, u), ref)
