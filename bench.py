@@ -18,6 +18,7 @@ scan on this host).
 import argparse
 import json
 import os
+import re
 import sys
 import time
 
@@ -41,9 +42,12 @@ def algorithmic_bytes(spec, kind, layouts=None):
     warp                   : read 6 fields and x, write x_i in each layout the view's rays
                              read (1 or 2)                                 -> 28N + 4N per layout
     warp-add               : read 6 fields, u and x, write x               -> 36N
-    motion step            : read u, read and write the two accumulators, read the constants
-                             (Ga, Gb, H, Q) and x, write (g, q) in each layout the next
-                             view's rays read                              -> 40N + 8N per layout
+    motion step            : the motion sweep's ratio backprojection with the view's update and
+                             the next view's prep fused: read (r, Pq) (2D), read and write
+                             the two accumulators, read the constants (Ga, Gb, H, Q) and x,
+                             write (g, q) in each layout the next view's rays read
+                                                                           -> 36N + 8N per layout
+                                                                              + 8D
     `layouts` (per view, bit 0 natural, bit 1 transposed) averages the per-layout terms over the
     views the kernel is launched for.
     """
@@ -67,7 +71,7 @@ def algorithmic_bytes(spec, kind, layouts=None):
     if kind == "warp_add":
         return 36 * N
     if kind == "motion":
-        return 40 * N + 8 * N * mean_layouts(range(2, V))   # prep of views 2..V-1
+        return 36 * N + 8 * N * mean_layouts(range(2, V)) + 8 * D   # prep of views 2..V-1
     raise KeyError(kind)
 
 
@@ -85,12 +89,12 @@ def profiled(workload, kind, field="traffic_bytes"):
     path = os.path.join(ROOT, "profiles", f"{workload}_kernels.json")
     if not os.path.exists(path):
         return None
-    prefix = {"projector": "k_project_rows<1,", "backprojector": "k_backproject<1,",
-              "warp": "k_warp_tile", "warp_add": "k_warp_add_tile",
-              "motion": "k_motion_acc<false, false>"}[kind]
+    pattern = {"projector": r"k_project_rows<1,", "backprojector": r"k_backproject<1, \d+, \w+, false>",
+               "warp": r"k_warp_tile", "warp_add": r"k_warp_add_tile",
+               "motion": r"k_backproject<1, \d+, \w+, true>"}[kind]
     with open(path) as f:
         rows = json.load(f).get("kernels", [])
-    best = [r for r in rows if r["kernel"].startswith(prefix) and field in r]
+    best = [r for r in rows if re.match(pattern, r["kernel"]) and field in r]
     if not best:
         return None
     best.sort(key=lambda r: -r["launches"])
@@ -281,8 +285,7 @@ def main():
     layouts = job.view_layouts()
     n_img_views, n_mot_views = launches_per_scan(spec)
     n_warps = n_img_views if spec.motion else 0
-    per_scan = {"projector": n_img_views + n_mot_views,
-                "backprojector": n_img_views + n_mot_views,
+    per_scan = {"projector": n_img_views + n_mot_views, "backprojector": n_img_views,
                 "warp": n_warps, "warp_add": n_warps, "motion": n_mot_views}
     byte_kind = {"projector": "proj_image", "backprojector": "back", "warp": "warp",
                  "warp_add": "warp_add", "motion": "motion"}

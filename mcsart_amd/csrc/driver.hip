@@ -31,7 +31,7 @@ struct mcsart_job {
     float *d_w1 = nullptr;       // 2N: warped image (image sweep) / (g, q) pairs (motion sweep)
     float *d_w1T = nullptr;      // 2N: the same in the xy-transposed layout / transposed image
     float *d_w2 = nullptr;       // N: u
-    float *d_acc = nullptr;      // 2N: motion-sweep accumulators SV, SF
+    float *d_acc = nullptr;      // 2N: motion-sweep accumulators, transposed (SVt, SFt)
     float4 *d_cst = nullptr;     // N: motion-sweep constants (Ga, Gb, H, Q)
     float2 *d_rp = nullptr;      // D
     float *d_norm = nullptr;     // n_iter*V
@@ -168,21 +168,20 @@ int image_sweep(mcsart_job *j, int it) {
 
 // Motion (udvf) sweep (src/AUMISART.cu:236-288; static overload :66-118 steps views by 10).
 // Per view: prep (g, q) -> projector -> backprojector -> field update.  x is constant over the
-// sweep, so the updates are carried as two accumulators (launch_motion_acc): the per-sweep
-// constants and the first view's prep come from launch_motion_begin, each update is fused with
-// the next view's prep, and the sweep's last update writes the fields.
+// sweep, so the updates are carried as two accumulators: launch_motion_begin makes the per-sweep
+// constants and the first view's prep; each view's update and the next view's prep run in the
+// ratio backprojector's epilogue (MotionStep); launch_motion_finish writes the fields.
 int motion_sweep(mcsart_job *j, int it) {
     const Geom &g = j->g;
     const long long D = g.ndet(), N = g.nvox();
     hipStream_t st = j->ctx->stream;
-    const FieldsW fw = fields_w(j);
-    float *sv = j->d_acc, *sf = j->d_acc + N;
     const int stride = j->opts.motion ? 1 : 10;
     float2 *pair = (float2 *)j->d_w1, *pairT = (float2 *)j->d_w1T;
     auto vdiff = [&](int iv) { return j->volumes[iv] - j->volumes[iv - 1]; };   // :249
     auto fdiff = [&](int iv) { return j->flows[iv] - j->flows[iv - 1]; };       // :250
     int rc, nparts = 0;
-    if (j->V > 1) {   // constants and the prep of the first view
+    if (j->V < 2) return MCSART_OK;   // no view to update
+    {   // constants and the prep of the first view
         const ViewInfo &vi = j->vinfo[1];
         TimedRegion tr(j->ctx, TK_MOTION);
         rc = launch_motion_begin(fields_of(j), j->d_img, j->d_cst, vi.has_y ? pair : nullptr,
@@ -204,26 +203,29 @@ int motion_sweep(mcsart_job *j, int it) {
                                   j->proj_rows);
             if (rc) return rc;
         }
-        {   // u = B(r) / B(P q)  (:265-270); also rho_i
-            const NormFinish nf{j->d_partials, nparts, j->d_norm + (long long)it * j->V + iv};
-            TimedRegion tr(j->ctx, TK_BACK);
-            rc = launch_backprojector(g, j->trig[iv], vi, BACK_RATIO, nullptr, j->d_rp, j->d_w2,
-                                      st, &nf, &j->back_tune);
-            if (rc) return rc;
-        }
-        {   // fields += u^T * (-vd | -fd) * D_c x (:272-277), then the next view's prep
+        {   // u = B(r) / B(P q) (:265-270), rho_i; fields += u^T * (-vd | -fd) * D_c x
+            // (:272-277) as accumulators; then the next view's prep
             const int nv = iv + stride;
             const bool more = nv < j->V;
             const ViewInfo &vn = j->vinfo[more ? nv : iv];
-            TimedRegion tr(j->ctx, TK_MOTION);
-            rc = launch_motion_acc(sv, sf, j->d_w2, j->d_img, j->d_cst,
-                                   more && vn.has_y ? pair : nullptr,
-                                   more && vn.has_x ? pairT : nullptr, fw, g.nx, g.ny, g.nz,
-                                   j->volumes[iv - 1] - j->volumes[iv],
-                                   j->flows[iv - 1] - j->flows[iv], more ? vdiff(nv) : 0.f,
-                                   more ? fdiff(nv) : 0.f, iv == 1, !more, st);
+            const MotionStep m{j->d_acc, j->d_acc + N, j->d_cst, j->d_img,
+                               more && vn.has_y ? pair : nullptr,
+                               more && vn.has_x ? pairT : nullptr,
+                               j->volumes[iv - 1] - j->volumes[iv], j->flows[iv - 1] - j->flows[iv],
+                               more ? vdiff(nv) : 0.f, more ? fdiff(nv) : 0.f, iv == 1 ? 1 : 0,
+                               more ? 0 : 1};
+            const NormFinish nf{j->d_partials, nparts, j->d_norm + (long long)it * j->V + iv};
+            TimedRegion tr(j->ctx, TK_BACK);
+            rc = launch_backprojector(g, j->trig[iv], vi, BACK_RATIO, nullptr, j->d_rp, nullptr,
+                                      st, &nf, &j->back_tune, &m);
             if (rc) return rc;
         }
+    }
+    {   // fields = fields_0 + D_c x * accumulators
+        TimedRegion tr(j->ctx, TK_MOTION);
+        rc = launch_motion_finish(j->d_acc, j->d_acc + N, j->d_img, fields_w(j), g.nx, g.ny,
+                                  g.nz, st);
+        if (rc) return rc;
     }
     return MCSART_OK;
 }
@@ -583,7 +585,6 @@ extern "C" int mcsart_job_kernel_time(mcsart_job *j, int kind, int reps, double 
         if (rc) return rc;
         MCSART_HIP_TRY(hipMemsetAsync(j->d_acc, 0, sizeof(float) * 2 * N, st));
     }
-    const FieldsW fw = fields_w(j);
     // every view of the scan once per rep, back to back, as the sweeps launch them
     auto all_views = [&]() {
         for (int iv = 0; iv < j->V; ++iv) {
@@ -597,15 +598,18 @@ extern "C" int mcsart_job_kernel_time(mcsart_job *j, int kind, int reps, double 
                 r = launch_warp_add(j->d_w1, j->d_w2, g.nx, g.ny, g.nz,
                                     j->volumes[0] - j->volumes[iv], j->flows[0] - j->flows[iv],
                                     f, j->opts.lambda, j->opts.nonneg, st);
-            } else if (kind == 4) {   // update of view iv, prep of view iv + 1 (views 1..V-2)
+            } else if (kind == 4) {   // ratio backprojection of view iv with its update and
+                                      // the prep of view iv + 1 (views 1..V-2)
                 if (iv == 0 || iv + 1 >= j->V) continue;
                 const ViewInfo &vn = j->vinfo[iv + 1];
-                r = launch_motion_acc(j->d_acc, j->d_acc + N, j->d_w2, j->d_img, j->d_cst,
-                                      vn.has_y ? pair : nullptr, vn.has_x ? pairT : nullptr, fw,
-                                      g.nx, g.ny, g.nz, j->volumes[iv - 1] - j->volumes[iv],
-                                      j->flows[iv - 1] - j->flows[iv],
-                                      j->volumes[iv + 1] - j->volumes[iv],
-                                      j->flows[iv + 1] - j->flows[iv], false, false, st);
+                const MotionStep m{j->d_acc, j->d_acc + N, j->d_cst, j->d_img,
+                                   vn.has_y ? pair : nullptr, vn.has_x ? pairT : nullptr,
+                                   j->volumes[iv - 1] - j->volumes[iv],
+                                   j->flows[iv - 1] - j->flows[iv],
+                                   j->volumes[iv + 1] - j->volumes[iv],
+                                   j->flows[iv + 1] - j->flows[iv], 0, 0};
+                r = launch_backprojector(g, j->trig[iv], vi, BACK_RATIO, nullptr, j->d_rp,
+                                         nullptr, st, nullptr, &j->back_tune, &m);
             } else if (kind == 0) {
                 ProjArgs a{};
                 a.vol0 = j->d_img;

@@ -132,9 +132,24 @@ struct BackTune {
 };
 // vi: the view's transverse window (view_info).  nf (optional): the first wavefront of the first
 // workgroup also finishes the preceding projector's norm reduction.
+// A motion-sweep step fused into the ratio backprojector (x is constant over the sweep; the
+// accumulators are kept transposed, SVt(p) = SV(p^T), so every access is at the voxel itself):
+//   SVt(p) += wv u(p), SFt(p) += wf u(p)                (update of view i, from zero if `first`)
+//   img0(p) = vd Ga(p) + fd Gb(p) + H(p) (vd SVt(p) + fd SFt(p))
+//   (g, q)(p) = (x(p) - img0(p), Q(p) (-vd - fd))      (prep of view i+1, unless `last`)
+// with cst = (Ga, Gb, H, Q) from launch_motion_begin; u itself is not stored.
+struct MotionStep {
+    float *svt, *sft;
+    const float4 *cst;
+    const float *x;
+    float2 *pair, *pairT;   // prep outputs in the layouts the next view reads (either may be null)
+    float wv, wf, vd, fd;
+    int first, last;
+};
 int launch_backprojector(const Geom &g, const ViewTrig &t, const ViewInfo &vi, BackMode mode,
                          const float *proj, const float2 *rp, float *out, hipStream_t st,
-                         const NormFinish *nf = nullptr, const BackTune *tune = nullptr);
+                         const NormFinish *nf = nullptr, const BackTune *tune = nullptr,
+                         const MotionStep *mot = nullptr);
 
 // ---- warps and elementwise ------------------------------------------------------------------
 struct Fields {
@@ -191,13 +206,11 @@ int launch_motion_step(float *const fields[6], const float *u, const float *x, f
 int launch_motion_begin(const Fields &f, const float *x, float4 *cst, float2 *pair,
                         float2 *pairT, int nx, int ny, int nz, float vd, float fd,
                         hipStream_t st);
-// ... then per updated view SV += wv u^T, SF += wf u^T (from zero when `first`), followed by the
-// prep of the next view (vd, fd) or, when `final_step`, by writing the fields
-// alpha_c = alpha_c0 + D_c x SV, beta_c = beta_c0 + D_c x SF in place (f).
-int launch_motion_acc(float *sv, float *sf, const float *u, const float *x, const float4 *cst,
-                      float2 *pair, float2 *pairT, const FieldsW &f, int nx, int ny, int nz,
-                      float wv, float wf, float vd, float fd, bool first, bool final_step,
-                      hipStream_t st);
+// ... then per updated view the ratio backprojector with a MotionStep, and at the end of the
+// sweep the fields alpha_c = alpha_c0 + D_c x SV, beta_c = beta_c0 + D_c x SF in place (f),
+// with SV(P) = svt(P^T), SF(P) = sft(P^T).
+int launch_motion_finish(const float *svt, const float *sft, const float *x, const FieldsW &f,
+                         int nx, int ny, int nz, hipStream_t st);
 
 // img1 *= w * D_ind(img0) (src/cu_multiple.cu:11-43)
 int launch_multiple(float *a, const float *b, int nx, int ny, int nz, float w, int ind,

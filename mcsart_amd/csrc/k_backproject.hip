@@ -65,12 +65,28 @@ __device__ inline float2 det_at(const void *base, unsigned off) {
     return *(const float2 *)((const char *)base + off);
 }
 
-template <int MODE, int WA, bool STAGE>
+// The motion-sweep step fused into the ratio mode (MotionStep, internal.hpp): the update of the
+// transposed accumulators and the next view's prep at the voxel itself, instead of storing u.
+__device__ inline void motion_step_at(const MotionStep &m, int id, int idT, float u) {
+    const float sv = (m.first ? 0.f : m.svt[id]) + m.wv * u;
+    const float sf = (m.first ? 0.f : m.sft[id]) + m.wf * u;
+    m.svt[id] = sv;
+    m.sft[id] = sf;
+    if (m.last) return;
+    const float4 c = m.cst[id];   // (Ga, Gb, H, Q)
+    const float img0 = m.vd * c.x + m.fd * c.y + c.z * (m.vd * sv + m.fd * sf);
+    const float2 gq = make_float2(m.x[id] - img0, c.w * (-m.vd + -m.fd));
+    if (m.pair) m.pair[id] = gq;
+    if (m.pairT) m.pairT[idT] = gq;
+}
+
+template <int MODE, int WA, bool STAGE, bool MOT>
 __global__ void __launch_bounds__(BX *BY) k_backproject(GeomArgs g, double c, double s,
                                                        const float *__restrict__ proj,
                                                        const float2 *__restrict__ rp,
                                                        float *__restrict__ out, int zslab,
-                                                       double ida, double idb, NormFinish nf) {
+                                                       double ida, double idb, NormFinish nf,
+                                                       MotionStep mot) {
     const int tid = threadIdx.y * BX + threadIdx.x;
     // the preceding projector's per-view norm (its partials are complete: kernel boundary); the
     // whole first wavefront (threads 0..63) of the first workgroup does it
@@ -270,7 +286,10 @@ __global__ void __launch_bounds__(BX *BY) k_backproject(GeomArgs g, double c, do
                 // IEEE quotient); 0/0 and x/0 still give NaN / Inf, mapped to 0
                 float u = acc * __builtin_amdgcn_rcpf(acc1);
                 if (isnan(u) || isinf(u)) u = 0.f;
-                out[id] = u;
+                if (MOT)   // square slices: the transposed voxel is (iy, ix, iz)
+                    motion_step_at(mot, id, iy + ix * g.nx + iz * plane, u);
+                else
+                    out[id] = u;
             }
         }
     };
@@ -291,47 +310,58 @@ int pick_zslab(const Geom &g) {
 template <int MODE, int WA>
 void launch_wa(dim3 grid, dim3 block, hipStream_t st, const GeomArgs &ga, const ViewTrig &t,
                const float *proj, const float2 *rp, float *out, int zs, const NormFinish &nf,
-               bool stage) {
+               bool stage, const MotionStep *mot) {
     const double ida = 1.0 / (double)ga.da, idb = 1.0 / (double)ga.db;
-    if (stage)
-        hipLaunchKernelGGL((k_backproject<MODE, WA, true>), grid, block, 0, st, ga, t.c, t.s,
-                           proj, rp, out, zs, ida, idb, nf);
-    else
-        hipLaunchKernelGGL((k_backproject<MODE, WA, false>), grid, block, 0, st, ga, t.c, t.s,
-                           proj, rp, out, zs, ida, idb, nf);
+    const MotionStep m = mot ? *mot : MotionStep{};
+#define MCSART_BACK(ST, MT)                                                                      \
+    hipLaunchKernelGGL((k_backproject<MODE, WA, ST, MT>), grid, block, 0, st, ga, t.c, t.s, proj, \
+                       rp, out, zs, ida, idb, nf, m)
+    if (MODE == BACK_RATIO && mot) {
+        if (stage) MCSART_BACK(true, MODE == BACK_RATIO);
+        else MCSART_BACK(false, MODE == BACK_RATIO);
+    } else {
+        if (stage) MCSART_BACK(true, false);
+        else MCSART_BACK(false, false);
+    }
+#undef MCSART_BACK
 }
 
 template <int MODE>
 void launch_mode(dim3 grid, dim3 block, hipStream_t st, const GeomArgs &ga, const ViewTrig &t,
                  int wa, const float *proj, const float2 *rp, float *out, int zs,
-                 const NormFinish &nf, bool stage) {
+                 const NormFinish &nf, bool stage, const MotionStep *mot) {
+#define MCSART_WA(W) launch_wa<MODE, W>(grid, block, st, ga, t, proj, rp, out, zs, nf, stage, mot)
     switch (wa <= 3 ? 3 : wa >= 8 ? 8 : wa) {
-        case 3: launch_wa<MODE, 3>(grid, block, st, ga, t, proj, rp, out, zs, nf, stage); break;
-        case 4: launch_wa<MODE, 4>(grid, block, st, ga, t, proj, rp, out, zs, nf, stage); break;
-        case 5: launch_wa<MODE, 5>(grid, block, st, ga, t, proj, rp, out, zs, nf, stage); break;
-        case 6: launch_wa<MODE, 6>(grid, block, st, ga, t, proj, rp, out, zs, nf, stage); break;
-        case 7: launch_wa<MODE, 7>(grid, block, st, ga, t, proj, rp, out, zs, nf, stage); break;
-        default: launch_wa<MODE, 8>(grid, block, st, ga, t, proj, rp, out, zs, nf, stage); break;
+        case 3: MCSART_WA(3); break;
+        case 4: MCSART_WA(4); break;
+        case 5: MCSART_WA(5); break;
+        case 6: MCSART_WA(6); break;
+        case 7: MCSART_WA(7); break;
+        default: MCSART_WA(8); break;
     }
+#undef MCSART_WA
 }
 
 }  // namespace
 
 int launch_backprojector(const Geom &g, const ViewTrig &t, const ViewInfo &vi, BackMode mode,
                          const float *proj, const float2 *rp, float *out, hipStream_t st,
-                         const NormFinish *nfp, const BackTune *tune) {
+                         const NormFinish *nfp, const BackTune *tune, const MotionStep *mot) {
     const GeomArgs ga{g.nx, g.ny, g.nz, g.na, g.nb, g.da, g.db, g.ai, g.bi, g.SO, g.SD};
     if (g.ndet() >= (1LL << 29))
         return fail(MCSART_ENOTSUP, "backprojector: detectors of 2^29 cells or more");
+    if (mot && (mode != BACK_RATIO || g.nx != g.ny))
+        return fail(MCSART_EINVAL, "backprojector: the motion step needs the ratio mode and nx == ny");
     const int zs = (tune && tune->zslab > 0) ? tune->zslab : pick_zslab(g);
     const bool stage = !tune || tune->stage != 2;   // 0 automatic (staged), 1 staged, 2 global
     const dim3 block(BX, BY);
     const dim3 grid((g.nx + BX - 1) / BX, (g.ny + BY - 1) / BY, (g.nz + zs - 1) / zs);
     const NormFinish nf = nfp ? *nfp : NormFinish{nullptr, 0, nullptr};
     if (mode == BACK_PLAIN)
-        launch_mode<BACK_PLAIN>(grid, block, st, ga, t, vi.wa, proj, rp, out, zs, nf, stage);
+        launch_mode<BACK_PLAIN>(grid, block, st, ga, t, vi.wa, proj, rp, out, zs, nf, stage,
+                                nullptr);
     else
-        launch_mode<BACK_RATIO>(grid, block, st, ga, t, vi.wa, proj, rp, out, zs, nf, stage);
+        launch_mode<BACK_RATIO>(grid, block, st, ga, t, vi.wa, proj, rp, out, zs, nf, stage, mot);
     MCSART_CHECK_LAUNCH();
     return MCSART_OK;
 }

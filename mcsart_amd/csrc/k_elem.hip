@@ -533,7 +533,9 @@ __global__ void __launch_bounds__(TT *TR) k_motion_step(float *__restrict__ ax,
 //   Ga(p) = sum_c alpha_c0(p^T) D_c x(p),  Gb(p) = sum_c beta_c0(p^T) D_c x(p),
 //   H(p) = sum_c D_c x(p^T) D_c x(p),  Q(p) = (D1 + D2 + D3) x(p),
 // so a view step reads and writes two accumulators instead of six fields.  The per-sweep
-// constants (Ga, Gb, H, Q) are made once by k_motion_begin, and the last update writes the fields.
+// constants (Ga, Gb, H, Q) are made once by k_motion_begin; each view's update and the next
+// view's prep run in the ratio backprojector's epilogue (k_backproject.hip), which keeps the
+// accumulators transposed (svt(p) = SV(p^T)); k_motion_finish writes the fields at the end.
 
 // Store pair values (g, q) of the A^T lanes: natural layout at jd[k] and/or, through LDS, the
 // transposed layout at the A lanes' indices id[k].
@@ -614,89 +616,45 @@ __global__ void __launch_bounds__(TT *TR) k_motion_begin(Fields f, const float *
     store_pairs(t[0], t[1], pair, pairT, gv, qv, id, ok, jd, jok);
 }
 
-// One view step of the accumulator form: SV += wv u^T, SF += wf u^T on A (FIRST: from zero),
-// then either the prep of the next view on A^T (FINAL = false) or, for the sweep's last update,
-// the fields alpha_c0 + D_c x SV, beta_c0 + D_c x SF written on A (FINAL = true).
-template <bool FIRST, bool FINAL>
-__global__ void __launch_bounds__(TT *TR) k_motion_acc(float *__restrict__ sv_acc,
-                                                      float *__restrict__ sf_acc,
-                                                      const float *__restrict__ u,
-                                                      const float *__restrict__ x,
-                                                      const float4 *__restrict__ cst,
-                                                      float2 *__restrict__ pair,
-                                                      float2 *__restrict__ pairT, FieldsW f,
-                                                      int nx, int ny, int nz, float wv, float wf,
-                                                      float vd, float fd) {
+// End of a motion sweep in accumulator form: alpha_c = alpha_c0 + D_c x SV and
+// beta_c = beta_c0 + D_c x SF in place, with SV(P) = svt(P^T), SF(P) = sft(P^T) (the ratio
+// backprojector keeps the accumulators transposed).
+__global__ void __launch_bounds__(TT *TR) k_motion_finish(const float *__restrict__ svt,
+                                                         const float *__restrict__ sft,
+                                                         const float *__restrict__ x, FieldsW f,
+                                                         int nx, int ny, int nz) {
     __shared__ float t[2][TT][TT + 1];
     const int X0 = blockIdx.x * TT, Y0 = blockIdx.y * TT, i2 = blockIdx.z;
     const int tx = threadIdx.x, ty = threadIdx.y;
     const int plane = nx * ny;
+    load_transposed(t[0], svt, nx, ny, X0, Y0, i2);
+    load_transposed(t[1], sft, nx, ny, X0, Y0, i2);
+    float *const F[6] = {f.ax, f.ay, f.az, f.bx, f.by, f.bz};
+    float a[TK][6];
+    Diffs d[TK];
     int id[TK];
     bool ok[TK];
-    float sv[TK], sf[TK];
-    load_transposed(t[0], u, nx, ny, X0, Y0, i2);
 #pragma unroll
     for (int k = 0; k < TK; ++k) {
         const int i1 = Y0 + ty + k * TR;
         ok[k] = X0 + tx < nx && i1 < ny;
         id[k] = X0 + tx + i1 * nx + i2 * plane;
-        sv[k] = (!FIRST && ok[k]) ? sv_acc[id[k]] : 0.f;
-        sf[k] = (!FIRST && ok[k]) ? sf_acc[id[k]] : 0.f;
+        d[k] = ok[k] ? diffs_at(x, id[k], X0 + tx, i1, i2, nx, ny, nz) : Diffs{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int c = 0; c < 6; ++c) a[k][c] = ok[k] ? F[c][id[k]] : 0.f;
     }
-    __syncthreads();   // the u tile is complete
+    __syncthreads();   // both accumulator tiles are complete
 #pragma unroll
     for (int k = 0; k < TK; ++k) {
-        const float ut = t[0][tx][ty + k * TR];
-        sv[k] += wv * ut;
-        sf[k] += wf * ut;
+        if (!ok[k]) continue;
+        const float sv = t[0][tx][ty + k * TR], sf = t[1][tx][ty + k * TR];
+        F[1][id[k]] = a[k][1] + d[k].d1 * sv;   // alpha_y with D1
+        F[0][id[k]] = a[k][0] + d[k].d2 * sv;   // alpha_x with D2
+        F[2][id[k]] = a[k][2] + d[k].d3 * sv;   // alpha_z with D3
+        F[4][id[k]] = a[k][4] + d[k].d1 * sf;   // beta_y
+        F[3][id[k]] = a[k][3] + d[k].d2 * sf;   // beta_x
+        F[5][id[k]] = a[k][5] + d[k].d3 * sf;   // beta_z
     }
-    if (FINAL) {
-        float *const F[6] = {f.ax, f.ay, f.az, f.bx, f.by, f.bz};
-#pragma unroll
-        for (int k = 0; k < TK; ++k) {
-            if (!ok[k]) continue;
-            const Diffs d = diffs_at(x, id[k], X0 + tx, Y0 + ty + k * TR, i2, nx, ny, nz);
-            const float a0 = F[0][id[k]], a1 = F[1][id[k]], a2 = F[2][id[k]];
-            const float b0 = F[3][id[k]], b1 = F[4][id[k]], b2 = F[5][id[k]];
-            F[1][id[k]] = a1 + d.d1 * sv[k];   // alpha_y with D1
-            F[0][id[k]] = a0 + d.d2 * sv[k];   // alpha_x with D2
-            F[2][id[k]] = a2 + d.d3 * sv[k];   // alpha_z with D3
-            F[4][id[k]] = b1 + d.d1 * sf[k];   // beta_y
-            F[3][id[k]] = b0 + d.d2 * sf[k];   // beta_x
-            F[5][id[k]] = b2 + d.d3 * sf[k];   // beta_z
-        }
-        return;
-    }
-#pragma unroll
-    for (int k = 0; k < TK; ++k)
-        if (ok[k]) {
-            sv_acc[id[k]] = sv[k];
-            sf_acc[id[k]] = sf[k];
-        }
-    __syncthreads();   // every lane is done with the u tile
-#pragma unroll
-    for (int k = 0; k < TK; ++k) {
-        t[0][ty + k * TR][tx] = sv[k];
-        t[1][ty + k * TR][tx] = sf[k];
-    }
-    __syncthreads();
-    // lane (tx, r) owns p = (Y0 + tx, X0 + r) of A^T; SV(p^T), SF(p^T) are at t[.][tx][r]
-    float gv[TK], qv[TK];
-    int jd[TK];
-    bool jok[TK];
-#pragma unroll
-    for (int k = 0; k < TK; ++k) {
-        const int r = ty + k * TR;
-        const int j0 = Y0 + tx, j1 = X0 + r;
-        jok[k] = j0 < nx && j1 < ny;
-        jd[k] = j0 + j1 * nx + i2 * plane;
-        const float4 c = jok[k] ? cst[jd[k]] : make_float4(0.f, 0.f, 0.f, 0.f);
-        const float xv = jok[k] ? x[jd[k]] : 0.f;
-        const float img0 = vd * c.x + fd * c.y + c.z * (vd * t[0][tx][r] + fd * t[1][tx][r]);
-        gv[k] = xv - img0;
-        qv[k] = c.w * (-vd + -fd);
-    }
-    store_pairs(t[0], t[1], pair, pairT, gv, qv, id, ok, jd, jok);
 }
 
 __global__ void __launch_bounds__(TT *TR) k_add_tile(float *__restrict__ x, float *__restrict__ xT,
@@ -905,19 +863,10 @@ int launch_motion_begin(const Fields &f, const float *x, float4 *cst, float2 *pa
     return MCSART_OK;
 }
 
-int launch_motion_acc(float *sv, float *sf, const float *u, const float *x, const float4 *cst,
-                      float2 *pair, float2 *pairT, const FieldsW &f, int nx, int ny, int nz,
-                      float wv, float wf, float vd, float fd, bool first, bool final_step,
-                      hipStream_t st) {
-    const dim3 grid = tile_grid(nx, ny, nz), block(TT, TR);
-#define MCSART_MOTION_ACC(A, B)                                                                  \
-    hipLaunchKernelGGL((k_motion_acc<A, B>), grid, block, 0, st, sv, sf, u, x, cst, pair, pairT, \
-                       f, nx, ny, nz, wv, wf, vd, fd)
-    if (first && final_step) MCSART_MOTION_ACC(true, true);
-    else if (first) MCSART_MOTION_ACC(true, false);
-    else if (final_step) MCSART_MOTION_ACC(false, true);
-    else MCSART_MOTION_ACC(false, false);
-#undef MCSART_MOTION_ACC
+int launch_motion_finish(const float *svt, const float *sft, const float *x, const FieldsW &f,
+                         int nx, int ny, int nz, hipStream_t st) {
+    hipLaunchKernelGGL(k_motion_finish, tile_grid(nx, ny, nz), dim3(TT, TR), 0, st, svt, sft, x,
+                       f, nx, ny, nz);
     MCSART_CHECK_LAUNCH();
     return MCSART_OK;
 }

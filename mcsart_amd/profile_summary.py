@@ -40,8 +40,10 @@ def algorithmic_bytes(kernel: str, N: int, D: int):
     if m:
         mode = int(m.group(1))
         return {0: 4 * N + 4 * D, 1: 4 * N + 12 * D, 2: 8 * N + 12 * D}[mode]
-    m = re.match(r"k_backproject<(\d)", kernel)
+    m = re.match(r"k_backproject<(\d), \d+, \w+, (\w+)>", kernel)
     if m:
+        if m.group(2) == "true":   # ratio mode with the motion step: one pair layout
+            return 36 * N + 8 * N + 8 * D
         return 4 * N + (4 * D if m.group(1) == "0" else 8 * D)
     table = {
         "k_warp_tile": 32 * N,        # 6 fields + x read, warped image written
@@ -49,12 +51,8 @@ def algorithmic_bytes(kernel: str, N: int, D: int):
         "k_motion_step<true, true>": 72 * N,    # 6 fields read+written, u, x, pairs
         "k_motion_step<true, false>": 56 * N,
         "k_motion_step<false, true>": 40 * N,
-        # accumulator form, one pair layout: u, SV/SF read+written, (Ga, Gb, H, Q), x, pairs
-        "k_motion_acc<false, false>": 48 * N,
-        "k_motion_acc<true, false>": 40 * N,    # first update: accumulators not read
-        "k_motion_acc<false, true>": 64 * N,    # last update: u, SV/SF, x, fields read+written
-        "k_motion_acc<true, true>": 56 * N,
         "k_motion_begin": 52 * N,               # fields, x read; constants, pairs written
+        "k_motion_finish": 60 * N,              # fields read+written, accumulators, x
         "k_transpose": 8 * N,
         "k_add_tile": 12 * N,
     }
