@@ -89,7 +89,8 @@ def profiled(workload, kind, field="traffic_bytes"):
     path = os.path.join(ROOT, "profiles", f"{workload}_kernels.json")
     if not os.path.exists(path):
         return None
-    pattern = {"projector": r"k_project_rows<1,", "backprojector": r"k_backproject<1, \d+, \w+, false>",
+    pattern = {"projector": r"k_project_rows<1,", "projector_motion": r"k_project_rows<2,",
+               "backprojector": r"k_backproject<1, \d+, \w+, false>",
                "warp": r"k_warp_tile", "warp_add": r"k_warp_add_tile",
                "motion": r"k_backproject<1, \d+, \w+, true>"}[kind]
     with open(path) as f:
@@ -285,10 +286,12 @@ def main():
     layouts = job.view_layouts()
     n_img_views, n_mot_views = launches_per_scan(spec)
     n_warps = n_img_views if spec.motion else 0
-    per_scan = {"projector": n_img_views + n_mot_views, "backprojector": n_img_views,
-                "warp": n_warps, "warp_add": n_warps, "motion": n_mot_views}
-    byte_kind = {"projector": "proj_image", "backprojector": "back", "warp": "warp",
-                 "warp_add": "warp_add", "motion": "motion"}
+    per_scan = {"projector": n_img_views, "projector_motion": n_mot_views,
+                "backprojector": n_img_views, "warp": n_warps, "warp_add": n_warps,
+                "motion": n_mot_views}
+    byte_kind = {"projector": "proj_image", "projector_motion": "proj_motion",
+                 "backprojector": "back", "warp": "warp", "warp_add": "warp_add",
+                 "motion": "motion"}
     iso_us = {k: job.kernel_time(k, reps=3) for k in per_scan}
     log(f"[bench] isolated launches: {iso_us}")
     kernels_roofline = {}

@@ -341,11 +341,13 @@ int mcsart_job_get_tuning(mcsart_job *job, mcsart_tuning *t);
 int mcsart_job_autotune(mcsart_job *job, mcsart_tuning *chosen);
 /* Average duration (microseconds) of one launch of a kernel of the sweeps, with the job's launch
  * shapes: 0 projector (residual mode), 1 backprojector (ratio mode), 2 warp of the image into the
- * layouts the view's rays read, 3 warp-add (x += lambda W(u)), 4 motion step (field update of
- * view i fused with the prep of view i+1).  Every view of the scan (views 1..V-2 for kind 4) is
- * launched back to back `reps` times between two HIP events on the job's stream.  Outputs go to
- * the job's scratch: kind 3 updates a copy of the image, kind 4 the motion sweep's accumulators;
- * the job's image, fields and norms are untouched.  Synchronous. */
+ * layouts the view's rays read, 3 warp-add (x += lambda W(u)), 4 motion step (the motion
+ * sweep's ratio backprojection with the update of view i, as accumulators, and the prep of view
+ * i+1 fused), 5 the motion sweep's projector (reads the (g, q) pairs).  Every view of the scan
+ * (views 1..V-2 for kind 4) is launched back to back `reps` times between two HIP events on the
+ * job's stream.  Outputs go to the job's scratch: kind 3 updates a copy of the image, kinds 4
+ * and 5 the motion sweep's accumulators and pairs; the job's image, fields and norms are
+ * untouched.  Synchronous. */
 int mcsart_job_kernel_time(mcsart_job *job, int kind, int reps, double *avg_us);
 /* Detector layouts each view's rays read: bit 0 the natural volume layout (y-branch rays), bit 1
  * the xy-transposed layout (x-branch rays); layouts[n_views]. */

@@ -559,9 +559,9 @@ extern "C" int mcsart_job_view_layouts(mcsart_job *j, int *layouts) {
 extern "C" int mcsart_job_kernel_time(mcsart_job *j, int kind, int reps, double *avg_us) {
     int rc = check_job(j);
     if (rc) return rc;
-    if (!avg_us || reps < 1 || kind < 0 || kind > 4)
-        return fail(MCSART_EINVAL, "kind must be 0..4 (projector, backprojector, warp, warp-add, "
-                                   "motion step), reps >= 1");
+    if (!avg_us || reps < 1 || kind < 0 || kind > 5)
+        return fail(MCSART_EINVAL, "kind must be 0..5 (projector, backprojector, warp, warp-add, "
+                                   "motion step, motion-sweep projector), reps >= 1");
     mcsart_ctx *ctx = j->ctx;
     if (ctx->timing) return fail(MCSART_EINVAL, "kernel timing while per-kernel events are on");
     MCSART_HIP_TRY(hipSetDevice(ctx->device));
@@ -610,6 +610,16 @@ extern "C" int mcsart_job_kernel_time(mcsart_job *j, int kind, int reps, double 
                                    j->flows[iv + 1] - j->flows[iv], 0, 0};
                 r = launch_backprojector(g, j->trig[iv], vi, BACK_RATIO, nullptr, j->d_rp,
                                          nullptr, st, nullptr, &j->back_tune, &m);
+            } else if (kind == 5) {   // projector of the motion sweep: reads (g, q) pairs
+                ProjArgs a{};
+                a.pair = pair;
+                a.pairT = vi.has_x ? pairT : nullptr;
+                a.meas = j->d_proj + D * iv;
+                a.rp = j->d_rp;
+                a.partials = j->d_partials;
+                int np = 0;
+                r = launch_projector(g, j->trig[iv], vi, PROJ_MOTION, a, st, &np, j->proj_split,
+                                     j->proj_rows);
             } else if (kind == 0) {
                 ProjArgs a{};
                 a.vol0 = j->d_img;

@@ -67,15 +67,30 @@ __device__ inline float2 det_at(const void *base, unsigned off) {
 
 // The motion-sweep step fused into the ratio mode (MotionStep, internal.hpp): the update of the
 // transposed accumulators and the next view's prep at the voxel itself, instead of storing u.
-__device__ inline void motion_step_at(const MotionStep &m, int id, int idT, float u) {
-    const float sv = (m.first ? 0.f : m.svt[id]) + m.wv * u;
-    const float sf = (m.first ? 0.f : m.sft[id]) + m.wf * u;
+// The voxel's inputs are loaded before its backprojection (MotionIn), so their latency hides
+// behind the row loop.
+struct MotionIn {
+    float sv, sf, x;
+    float4 c;   // (Ga, Gb, H, Q)
+};
+__device__ inline MotionIn motion_load(const MotionStep &m, int id) {
+    MotionIn in;
+    in.sv = m.first ? 0.f : m.svt[id];
+    in.sf = m.first ? 0.f : m.sft[id];
+    in.c = m.last ? make_float4(0.f, 0.f, 0.f, 0.f) : m.cst[id];
+    in.x = m.last ? 0.f : m.x[id];
+    return in;
+}
+__device__ inline void motion_step_at(const MotionStep &m, const MotionIn &in, int id, int idT,
+                                      float u) {
+    const float sv = in.sv + m.wv * u;
+    const float sf = in.sf + m.wf * u;
     m.svt[id] = sv;
     m.sft[id] = sf;
     if (m.last) return;
-    const float4 c = m.cst[id];   // (Ga, Gb, H, Q)
+    const float4 c = in.c;
     const float img0 = m.vd * c.x + m.fd * c.y + c.z * (m.vd * sv + m.fd * sf);
-    const float2 gq = make_float2(m.x[id] - img0, c.w * (-m.vd + -m.fd));
+    const float2 gq = make_float2(in.x - img0, c.w * (-m.vd + -m.fd));
     if (m.pair) m.pair[id] = gq;
     if (m.pairT) m.pairT[idT] = gq;
 }
@@ -231,6 +246,9 @@ __global__ void __launch_bounds__(BX *BY) k_backproject(GeomArgs g, double c, do
             return make_float2(s0, s1);
         };
         for (int iz = iz0; iz < iz1; ++iz) {
+            const int id = ix + iy * g.nx + iz * plane;
+            MotionIn min_{};
+            if (MOT) min_ = motion_load(mot, id);
             // axial bounds (:131-132,:137-138) in anchor-relative FP32
             const float d = (float)(iz - iz0);
             const float L0 = fmaf(bl.q0, d, bl.l0), L1 = fmaf(bl.q1, d, bl.l1);
@@ -278,7 +296,6 @@ __global__ void __launch_bounds__(BX *BY) k_backproject(GeomArgs g, double c, do
                     }
                 }
             }
-            const int id = ix + iy * g.nx + iz * plane;
             if (MODE == BACK_PLAIN) {
                 out[id] = acc;
             } else {
@@ -287,7 +304,7 @@ __global__ void __launch_bounds__(BX *BY) k_backproject(GeomArgs g, double c, do
                 float u = acc * __builtin_amdgcn_rcpf(acc1);
                 if (isnan(u) || isinf(u)) u = 0.f;
                 if (MOT)   // square slices: the transposed voxel is (iy, ix, iz)
-                    motion_step_at(mot, id, iy + ix * g.nx + iz * plane, u);
+                    motion_step_at(mot, min_, id, iy + ix * g.nx + iz * plane, u);
                 else
                     out[id] = u;
             }

@@ -278,7 +278,8 @@ class Job:
         check(self._lib.mcsart_job_autotune(self.ptr, ctypes.byref(t)), "mcsart_job_autotune")
         return t.as_dict()
 
-    KERNEL_KINDS = {"projector": 0, "backprojector": 1, "warp": 2, "warp_add": 3, "motion": 4}
+    KERNEL_KINDS = {"projector": 0, "backprojector": 1, "warp": 2, "warp_add": 3, "motion": 4,
+                    "projector_motion": 5}
 
     def kernel_time(self, kind: str, reps: int = 3) -> float:
         """Average microseconds per launch of one of KERNEL_KINDS over all views of the scan,
