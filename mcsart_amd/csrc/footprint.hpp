@@ -334,10 +334,10 @@ inline ViewInfo view_info(const GeomArgs &g, double c, double s) {
     // a small margin absorbs the FP32 rounding of the per-plane bounds
     v.wt = (int)floor(wt_max * (1.0 + 1e-5) + 1e-5) + 2;
     v.wz = (int)floor(wz_max * (1.0 + 1e-5) + 1e-5) + 2;
-    // backprojector: the widest transverse footprint of a voxel column, in cells.  The width is
-    // largest where magnification and |y'| are largest, i.e. on the boundary of the volume's
-    // footprint in the plane, so the boundary columns are scanned (plus a one-cell margin); a
-    // column that still exceeds it is handled by a generic loop in the kernel.
+    // backprojector: the widest transverse footprint of a voxel column, in cells.  The width
+    // grows toward the source and with |y'|, so over the square of columns it is largest on the
+    // boundary, and the boundary columns are scanned; a column that still exceeds the window is
+    // handled by a generic loop in the kernel.
     double wa_max = 0.0;
     auto scan = [&](int ix, int iy) {
         Column C;
@@ -352,7 +352,9 @@ inline ViewInfo view_info(const GeomArgs &g, double c, double s) {
         scan(0, iy);
         scan(g.nx - 1, iy);
     }
-    v.wa = (int)floor(wa_max + 1.0) + 2;
+    // a column covers cells floor(Amin) .. ceil(Amax) - 1: at most floor(width) + 2 of them (the
+    // guard absorbs the device's own FP64 rounding of the same bounds)
+    v.wa = (int)floor(wa_max * (1.0 + 1e-9) + 1e-9) + 2;
     return v;
 }
 

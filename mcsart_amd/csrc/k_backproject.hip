@@ -270,6 +270,7 @@ __global__ void __launch_bounds__(BX *BY) k_backproject(GeomArgs g, double c, do
                 // the first again with weight F(lb+2) - F(lb+1) = 0.  F at the step's upper
                 // boundary is the next step's lower one (lbf counts rows in float: exact).
                 float Fa = F(0), lbf = 0.f;
+#pragma unroll 1   // one or two steps per voxel: unrolling only adds remainder code
                 for (int lb = 0; lb < n; lb += 2) {
                     const int r0 = ib_lo + lb - rb0;
                     const int r1 = ib_lo + min(lb + 1, n - 1) - rb0;
