@@ -3,7 +3,9 @@
 
 Workload: the 128^3 motion-compensated McSART scan (volume 128^3 float32 at 2 mm, detector
 256x192 at 1.5 mm, 200 views over 360 deg, 10 outer iterations of one image sweep and one motion
-sweep, lambda 1, nonnegativity clamp).  One step is one complete reconstruction of that scan,
+sweep, lambda 1, nonnegativity clamp); `--workload clinical256` runs the clinical-size scan
+instead (256x256x128 at 1.5x1.5x2 mm, detector 512x384 at 0.78 mm, 656 views, 20 outer
+iterations).  One step is one complete reconstruction of that scan,
 with the scan's measured projections, initial image and 5DCT fields resident in HBM before the
 timed region starts.  Synthetic data: seeded ellipsoid phantom, breathing trace and Gaussian-blob
 v/f fields; the measured projections are made on the GPU with this library's warp and projector
@@ -212,11 +214,14 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--concurrent", type=int, default=4,
                     help="also measure this many scans in flight at once (0: skip)")
+    ap.add_argument("--workload", default="mcsart128", choices=["mcsart128", "clinical256"],
+                    help="mcsart128: 128^3, 200 views, 10 outer iterations (the headline scan); "
+                         "clinical256: 256x256x128, 656 views, 20 outer iterations")
     args = ap.parse_args()
 
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
-              "(scans are independent: more GPUs would run more scans, see DESIGN.md)",
+              "(scans are independent: more GPUs would run more scans side by side)",
               file=sys.stderr)
         sys.exit(2)
 
@@ -225,7 +230,7 @@ def main():
     from mcsart_amd.driver import Context, Job
     from mcsart_amd.simulate import measure
 
-    spec = configs.c2_mcsart128()
+    spec = {"mcsart128": configs.c2_mcsart128, "clinical256": configs.c3_clinical}[args.workload]()
     g = spec.geometry
     log(f"[bench] workload {spec.name}: {g.nx}x{g.ny}x{g.nz}, det {g.na}x{g.nb}, V={spec.n_views}, "
         f"op_iter={spec.op_iter}")
@@ -338,6 +343,7 @@ def main():
         log("[bench] timing the FP64 CPU oracle on a bounded sample ...")
         cpu = cpu_baseline(spec, data, proj, args.cpu_sample_views)
 
+    vol_name = f"{g.nx}^3" if g.nx == g.ny == g.nz else f"{g.nx}x{g.ny}x{g.nz}"
     lat_ms = np.array(lat) * 1e3
     value = 3600.0 * K / elapsed
     # Whole-step figure: the unfused loop's algorithmic bytes per scan over the measured scan time
@@ -348,7 +354,8 @@ def main():
                      "bytes_per_scan": int(scan_bytes), "achieved": round(step_ach, 1),
                      "unit": "GB/s", "frac": round(step_ach / HBM_PEAK_GBS, 4)}
     out = {
-        "metric": "McSART scans/hour (128^3 motion-compensated scan, 200 views, 10 outer iterations)",
+        "metric": (f"McSART scans/hour ({vol_name} motion-compensated scan, {spec.n_views} views, "
+                   f"{len(spec.op_iter) // 2} outer iterations)"),
         "value": round(value, 2),
         "unit": "scans/hour",
         "n_gpus": 1,
@@ -361,7 +368,8 @@ def main():
         "dtype": "f32",
         "data": "synthetic (seeded phantom, trace, fields; projections made on GPU + Poisson noise)",
         "config": {"workload": spec.name, "volume": [g.nx, g.ny, g.nz], "detector": [g.na, g.nb],
-                   "views": spec.n_views, "op_iter": "[1,0]x10", "lambda": spec.lam,
+                   "views": spec.n_views, "op_iter": f"[1,0]x{len(spec.op_iter) // 2}",
+                   "lambda": spec.lam,
                    "nonneg": spec.nonneg, "jobs_in_flight": 1},
         "latency_ms": {"p50": round(float(np.percentile(lat_ms, 50)), 2),
                        "p99": round(float(np.percentile(lat_ms, 99)), 2)},

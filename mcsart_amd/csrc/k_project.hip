@@ -552,8 +552,9 @@ void launch_rows(const GeomArgs &ga, const ViewTrig &t, const ProjArgs &a, hipSt
 template <int MODE, int WT, int WZ>
 void launch_rows_r(const GeomArgs &ga, const ViewTrig &t, const ProjArgs &a, hipStream_t st,
                    int R, int S) {
-    if (R == 4) launch_rows<MODE, WT, WZ, 4>(ga, t, a, st, S);
-    else if (R == 2) launch_rows<MODE, WT, WZ, 2>(ga, t, a, st, S);
+    if (R == 4 && MODE != PROJ_MOTION)   // the motion mode runs at most two rays per lane
+        launch_rows<MODE, WT, WZ, (MODE == PROJ_MOTION ? 2 : 4)>(ga, t, a, st, S);
+    else if (R >= 2) launch_rows<MODE, WT, WZ, 2>(ga, t, a, st, S);
     else launch_rows<MODE, WT, WZ, 1>(ga, t, a, st, S);
 }
 
@@ -707,7 +708,10 @@ int launch_projector(const Geom &g, const ViewTrig &t, const ViewInfo &vi, ProjM
     if (rows_ok) {
         int R, S;
         pick_rows(g, rows, split, R, S);
-        if (mode == PROJ_MOTION && vi.wt == 3 && vi.wz == 3 && R == 4) R = 2;   // registers
+        // The motion mode gathers (g, q) pairs: with four rays per lane its footprints' values
+        // hold twice the registers and the lanes wait on them (105 us at 256x256x128 against
+        // 74 us with two rays per lane, while the residual mode is fastest with four)
+        if (mode == PROJ_MOTION && R == 4) R = 2;
         switch (mode) {
             case PROJ_PLAIN: rc = launch_mode_rows<PROJ_PLAIN>(g, t, vi, a, st, R, S); break;
             case PROJ_IMAGE: rc = launch_mode_rows<PROJ_IMAGE>(g, t, vi, a, st, R, S); break;
