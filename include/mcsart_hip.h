@@ -332,6 +332,8 @@ typedef struct {
     int back_stage;   /* backprojector detector reads: 0 automatic (LDS-staged rectangle),
                          1 staged in LDS, 2 straight from global memory */
     int proj_rows;    /* detector rows (rays of one column) per projector lane: 0, 1, 2 or 4 */
+    int back_zslab_motion; /* voxels per lane along z of the motion sweep's backprojector (with
+                              the fused field update): 0 (automatic) or 1..64 */
 } mcsart_tuning;
 int mcsart_job_set_tuning(mcsart_job *job, const mcsart_tuning *t);
 int mcsart_job_get_tuning(mcsart_job *job, mcsart_tuning *t);
@@ -369,6 +371,7 @@ typedef struct {
     size_t hbm_budget;   /* device bytes admitted jobs may hold together; 0: 90% of free memory */
     int autotune;        /* 1: autotune each job's launch shapes after admission */
     int proj_split, back_zslab, back_stage, proj_rows;   /* fixed shapes when autotune == 0 */
+    int back_zslab_motion;                               /* (mcsart_tuning's fields) */
 } mcsart_server_opts;
 
 /* All host buffers must stay valid until the request has completed (mcsart_server_wait). */
@@ -398,6 +401,8 @@ typedef struct {
     double run_ms;      /* admission -> results in the output buffers */
     double total_ms;    /* submit -> results in the output buffers */
     mcsart_tuning tuning;   /* launch shapes it ran with */
+    double admit_at_ms; /* admission and completion on one clock for all requests: milliseconds */
+    double done_at_ms;  /* since the server was created (0 until the event happened) */
 } mcsart_request_info;
 
 int mcsart_server_create(int device, const mcsart_server_opts *opts, mcsart_server **server);

@@ -51,6 +51,7 @@ struct mcsart_job {
     bool use_graphs = true;
     int proj_split = 0, proj_rows = 0;   // projector launch shape (0 = automatic)
     BackTune back_tune{0, 0};   // backprojector launch shape (0 = automatic)
+    int back_zslab_motion = 0;  // z-slab of the motion sweep's backprojector (0 = automatic)
 };
 
 namespace {
@@ -215,9 +216,10 @@ int motion_sweep(mcsart_job *j, int it) {
                                more ? vdiff(nv) : 0.f, more ? fdiff(nv) : 0.f, iv == 1 ? 1 : 0,
                                more ? 0 : 1};
             const NormFinish nf{j->d_partials, nparts, j->d_norm + (long long)it * j->V + iv};
+            const BackTune bt{j->back_zslab_motion, j->back_tune.stage};
             TimedRegion tr(j->ctx, TK_BACK);
             rc = launch_backprojector(g, j->trig[iv], vi, BACK_RATIO, nullptr, j->d_rp, nullptr,
-                                      st, &nf, &j->back_tune, &m);
+                                      st, &nf, &bt, &m);
             if (rc) return rc;
         }
     }
@@ -494,16 +496,20 @@ int check_tuning(const mcsart_tuning &t) {
         return fail(MCSART_EINVAL, "back_zslab must be in 0..64");
     if (t.back_stage < 0 || t.back_stage > 2)
         return fail(MCSART_EINVAL, "back_stage must be 0, 1 or 2");
+    if (t.back_zslab_motion < 0 || t.back_zslab_motion > 64)
+        return fail(MCSART_EINVAL, "back_zslab_motion must be in 0..64");
     return MCSART_OK;
 }
 
 void apply_tuning(mcsart_job *j, const mcsart_tuning &t) {
     if (j->proj_split != t.proj_split || j->proj_rows != t.proj_rows ||
-        j->back_tune.zslab != t.back_zslab || j->back_tune.stage != t.back_stage)
+        j->back_tune.zslab != t.back_zslab || j->back_tune.stage != t.back_stage ||
+        j->back_zslab_motion != t.back_zslab_motion)
         for (auto &sg : j->graphs) drop_graph(sg);   // captured launches carry the old shapes
     j->proj_split = t.proj_split;
     j->proj_rows = t.proj_rows;
     j->back_tune = BackTune{t.back_zslab, t.back_stage};
+    j->back_zslab_motion = t.back_zslab_motion;
 }
 
 // Elapsed milliseconds of `fn` enqueued `reps` times on the job's stream.
@@ -543,7 +549,8 @@ extern "C" int mcsart_job_get_tuning(mcsart_job *j, mcsart_tuning *t) {
     int rc = check_job(j);
     if (rc) return rc;
     if (!t) return fail(MCSART_EINVAL, "null tuning");
-    *t = mcsart_tuning{j->proj_split, j->back_tune.zslab, j->back_tune.stage, j->proj_rows};
+    *t = mcsart_tuning{j->proj_split, j->back_tune.zslab, j->back_tune.stage, j->proj_rows,
+                       j->back_zslab_motion};
     return MCSART_OK;
 }
 
@@ -608,8 +615,9 @@ extern "C" int mcsart_job_kernel_time(mcsart_job *j, int kind, int reps, double 
                                    j->flows[iv - 1] - j->flows[iv],
                                    j->volumes[iv + 1] - j->volumes[iv],
                                    j->flows[iv + 1] - j->flows[iv], 0, 0};
+                const BackTune bt{j->back_zslab_motion, j->back_tune.stage};
                 r = launch_backprojector(g, j->trig[iv], vi, BACK_RATIO, nullptr, j->d_rp,
-                                         nullptr, st, nullptr, &j->back_tune, &m);
+                                         nullptr, st, nullptr, &bt, &m);
             } else if (kind == 5) {   // projector of the motion sweep: reads (g, q) pairs
                 ProjArgs a{};
                 a.pair = pair;
@@ -688,7 +696,7 @@ extern "C" int mcsart_job_autotune(mcsart_job *j, mcsart_tuning *chosen) {
         return (int)MCSART_OK;
     };
     const int reps = 3;
-    mcsart_tuning best{0, 0, 0, 0};
+    mcsart_tuning best{0, 0, 0, 0, 0};
     float best_ms = 3.4e38f;
     const int np = std::max(g.nx, g.ny);
     for (int rows = 1; rows <= 4; rows *= 2)
@@ -720,6 +728,45 @@ extern "C" int mcsart_job_autotune(mcsart_job *j, mcsart_tuning *chosen) {
                 best.back_stage = stage;
             }
         }
+    // the motion sweep's backprojector (fused update and next prep) on the same views (view 1
+    // at the least: it needs a previous view), into the sweep's scratch accumulators and pairs
+    if (j->V >= 2) {
+        const long long N = g.nvox();
+        float2 *pair = (float2 *)j->d_w1, *pairT = (float2 *)j->d_w1T;
+        rc = launch_motion_begin(fields_of(j), j->d_img, j->d_cst, pair, pairT, g.nx, g.ny, g.nz,
+                                 0.f, 0.f, st);
+        if (rc) return rc;
+        MCSART_HIP_TRY(hipMemsetAsync(j->d_acc, 0, sizeof(float) * 2 * N, st));
+        auto motion = [&](const BackTune &bt) {
+            for (int iv0 : views) {
+                const int iv = std::max(iv0, 1), nv = std::min(iv + 1, j->V - 1);
+                const ViewInfo &vn = j->vinfo[nv];
+                const MotionStep m{j->d_acc, j->d_acc + N, j->d_cst, j->d_img,
+                                   vn.has_y ? pair : nullptr, vn.has_x ? pairT : nullptr,
+                                   j->volumes[iv - 1] - j->volumes[iv],
+                                   j->flows[iv - 1] - j->flows[iv], j->volumes[nv] - j->volumes[iv],
+                                   j->flows[nv] - j->flows[iv], 0, 0};
+                const int r = launch_backprojector(g, j->trig[iv], j->vinfo[iv], BACK_RATIO,
+                                                   nullptr, j->d_rp, nullptr, st, nullptr, &bt,
+                                                   &m);
+                if (r) return r;
+            }
+            return (int)MCSART_OK;
+        };
+        best_ms = 3.4e38f;
+        for (int zs = 1; zs <= 16 && zs <= g.nz; zs *= 2) {
+            const BackTune bt{zs, best.back_stage};
+            rc = motion(bt);
+            if (rc) return rc;
+            float ms = 0.f;
+            rc = time_on_stream(j, reps, [&] { return motion(bt); }, &ms);
+            if (rc) return rc;
+            if (ms < best_ms) {
+                best_ms = ms;
+                best.back_zslab_motion = zs;
+            }
+        }
+    }
     apply_tuning(j, best);
     if (chosen) *chosen = best;
     return MCSART_OK;

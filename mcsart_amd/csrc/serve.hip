@@ -69,6 +69,7 @@ struct mcsart_server {
     bool stop = false;
     std::thread worker;
     std::string worker_error;
+    Clock::time_point t0 = Clock::now();   // origin of the requests' admit/done times
 };
 
 namespace {
@@ -88,7 +89,7 @@ int start_request(mcsart_server *sv, Slot &s, Request *r) {
         if ((rc = mcsart_job_autotune(j, &r->tuning))) return rc;
     } else {
         mcsart_tuning t{sv->opts.proj_split, sv->opts.back_zslab, sv->opts.back_stage,
-                        sv->opts.proj_rows};
+                        sv->opts.proj_rows, sv->opts.back_zslab_motion};
         if ((rc = mcsart_job_set_tuning(j, &t))) return rc;
         r->tuning = t;
     }
@@ -226,10 +227,12 @@ extern "C" int mcsart_server_create(int device, const mcsart_server_opts *o,
     *out = nullptr;
     if (!o) return fail(MCSART_EINVAL, "null options");
     if (o->slots < 1 || o->slots > 16) return fail(MCSART_EINVAL, "slots must be in 1..16");
-    const mcsart_tuning t{o->proj_split, o->back_zslab, o->back_stage, o->proj_rows};
+    const mcsart_tuning t{o->proj_split, o->back_zslab, o->back_stage, o->proj_rows,
+                          o->back_zslab_motion};
     if (t.proj_split < 0 || t.proj_split > 16 || (t.proj_split & (t.proj_split - 1)) ||
         (t.proj_rows != 0 && t.proj_rows != 1 && t.proj_rows != 2 && t.proj_rows != 4) ||
-        t.back_zslab < 0 || t.back_zslab > 64 || t.back_stage < 0 || t.back_stage > 2)
+        t.back_zslab < 0 || t.back_zslab > 64 || t.back_stage < 0 || t.back_stage > 2 ||
+        t.back_zslab_motion < 0 || t.back_zslab_motion > 64)
         return fail(MCSART_EINVAL, "bad fixed launch shapes");
     MCSART_HIP_TRY(hipSetDevice(device));
     auto *sv = new mcsart_server();
@@ -346,6 +349,8 @@ extern "C" int mcsart_server_info(mcsart_server *sv, long long id, mcsart_reques
     info->queue_ms = ms_between(r->t_submit, admitted ? r->t_admit : now);
     info->run_ms = admitted ? ms_between(r->t_admit, ended ? r->t_done : now) : 0.0;
     info->total_ms = ms_between(r->t_submit, ended ? r->t_done : now);
+    info->admit_at_ms = admitted ? ms_between(sv->t0, r->t_admit) : 0.0;
+    info->done_at_ms = ended ? ms_between(sv->t0, r->t_done) : 0.0;
     return MCSART_OK;
 }
 

@@ -148,7 +148,8 @@ class COpts(ctypes.Structure):
 class CTuning(ctypes.Structure):
     """Launch shapes of the footprint kernels (0 = automatic)."""
     _fields_ = [("proj_split", ctypes.c_int), ("back_zslab", ctypes.c_int),
-                ("back_stage", ctypes.c_int), ("proj_rows", ctypes.c_int)]
+                ("back_stage", ctypes.c_int), ("proj_rows", ctypes.c_int),
+                ("back_zslab_motion", ctypes.c_int)]
 
     def as_dict(self) -> dict:
         return {n: getattr(self, n) for n, _ in self._fields_}
@@ -263,8 +264,8 @@ class Job:
         check(self._lib.mcsart_job_set_graphs(self.ptr, int(enable)), "mcsart_job_set_graphs")
 
     def set_tuning(self, proj_split: int = 0, back_zslab: int = 0, back_stage: int = 0,
-                   proj_rows: int = 0):
-        t = CTuning(proj_split, back_zslab, back_stage, proj_rows)
+                   proj_rows: int = 0, back_zslab_motion: int = 0):
+        t = CTuning(proj_split, back_zslab, back_stage, proj_rows, back_zslab_motion)
         check(self._lib.mcsart_job_set_tuning(self.ptr, ctypes.byref(t)), "mcsart_job_set_tuning")
 
     def tuning(self) -> dict:

@@ -39,7 +39,8 @@ def main(argv=None):
                          "the sweeps: projector, backprojector, warp, warp_add, motion")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--tuning", default="",
-                    help="fixed launch shapes proj_split,back_zslab,back_stage,proj_rows "
+                    help="fixed launch shapes proj_split,back_zslab,back_stage,proj_rows"
+                         "[,back_zslab_motion] "
                          "(no autotune: the profile then holds the sweep's kernels only)")
     args = ap.parse_args(argv)
     spec = WORKLOADS[args.workload]()

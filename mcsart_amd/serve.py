@@ -25,7 +25,7 @@ class CServerOpts(ctypes.Structure):
     _fields_ = [("slots", ctypes.c_int), ("hbm_budget", ctypes.c_size_t),
                 ("autotune", ctypes.c_int), ("proj_split", ctypes.c_int),
                 ("back_zslab", ctypes.c_int), ("back_stage", ctypes.c_int),
-                ("proj_rows", ctypes.c_int)]
+                ("proj_rows", ctypes.c_int), ("back_zslab_motion", ctypes.c_int)]
 
 
 class CRequest(ctypes.Structure):
@@ -40,7 +40,8 @@ class CRequestInfo(ctypes.Structure):
     _fields_ = [("state", ctypes.c_int), ("status", ctypes.c_int), ("slot", ctypes.c_int),
                 ("bytes", ctypes.c_size_t), ("queue_ms", ctypes.c_double),
                 ("run_ms", ctypes.c_double), ("total_ms", ctypes.c_double),
-                ("tuning", CTuning)]
+                ("tuning", CTuning), ("admit_at_ms", ctypes.c_double),
+                ("done_at_ms", ctypes.c_double)]
 
 
 STATES = {0: "queued", 1: "running", 2: "done", 3: "failed"}
@@ -62,7 +63,7 @@ class Ticket:
 
 class Server:
     def __init__(self, slots: int = 4, hbm_budget: int = 0, autotune: bool = False,
-                 tuning: Sequence[int] = (0, 0, 0, 0), device: int = 0):
+                 tuning: Sequence[int] = (0, 0, 0, 0, 0), device: int = 0):
         self._lib = load()
         o = CServerOpts(int(slots), int(hbm_budget), int(autotune), *[int(t) for t in tuning])
         p = ctypes.c_void_p()
@@ -116,7 +117,8 @@ class Server:
         check(self._lib.mcsart_server_info(self.ptr, t.id, ctypes.byref(i)), "mcsart_server_info")
         return {"state": STATES.get(i.state, i.state), "status": i.status, "slot": i.slot,
                 "bytes": i.bytes, "queue_ms": i.queue_ms, "run_ms": i.run_ms,
-                "total_ms": i.total_ms, "tuning": i.tuning.as_dict()}
+                "total_ms": i.total_ms, "tuning": i.tuning.as_dict(),
+                "admit_at_ms": i.admit_at_ms, "done_at_ms": i.done_at_ms}
 
     def release(self, t: Ticket):
         check(self._lib.mcsart_server_release(self.ptr, t.id), "mcsart_server_release")
