@@ -741,9 +741,11 @@ inline dim3 tile_grid(int nx, int ny, int nz) {
     return dim3((nx + TT - 1) / TT, (ny + TT - 1) / TT, nz);
 }
 
-// Lanes per tile for the warps: 32 x 16, two voxels per lane.  Against four per lane (32 x 8),
-// measured in one session: 128^3 warps 17.0 -> 16.0 us, 256x256x128 66 -> 62 us.
-constexpr int WARP_ROWS = 16;
+// Lanes per tile for the warps: 32 x 32, one voxel per lane (more wavefronts in flight to hide
+// the field-load -> gather -> store chain).  Measured in one session on warps after a motion
+// sweep: four voxels per lane 66 us, two 62 us, one 59.6 us at 256x256x128; two 28.4 us, one
+// 26.1 us at 128^3.
+constexpr int WARP_ROWS = 32;
 
 int launch_warp(float *out, float *outT, const float *src, int nx, int ny, int nz, float vol,
                 float flow, const Fields &f, hipStream_t st) {
