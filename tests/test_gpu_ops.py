@@ -111,7 +111,9 @@ def test_backprojection_constant(gpu_lib):
     assert np.all(got >= 0)
 
 
-@pytest.mark.parametrize("shape", [(16, 24, 24), (32, 64, 64)])
+# (nz, ny, nx): partial tiles, full tiles, odd sizes with a one-voxel edge tile, and a 1 x 1
+# column (the warps' general 32-bit index form: the fast form needs nx >= 2)
+@pytest.mark.parametrize("shape", [(16, 24, 24), (32, 64, 64), (19, 33, 33), (7, 1, 1)])
 def test_deform_parity(gpu_lib, shape):
     img = smooth_random(shape, 11)
     fields = random_fields(shape, 23)
