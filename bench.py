@@ -92,9 +92,9 @@ def profiled(workload, kind, field="traffic_bytes"):
     if not os.path.exists(path):
         return None
     pattern = {"projector": r"k_project_rows<1,", "projector_motion": r"k_project_rows<2,",
-               "backprojector": r"k_backproject<1, \d+, \w+, false>",
+               "backprojector": r"k_backproject<1, \d+, \w+, 0>",
                "warp": r"k_warp_tile", "warp_add": r"k_warp_add_tile",
-               "motion": r"k_backproject<1, \d+, \w+, true>"}[kind]
+               "motion": r"k_backproject<1, \d+, \w+, 1>"}[kind]
     with open(path) as f:
         rows = json.load(f).get("kernels", [])
     best = [r for r in rows if re.match(pattern, r["kernel"]) and field in r]

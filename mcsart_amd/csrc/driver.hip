@@ -144,23 +144,24 @@ int image_sweep(mcsart_job *j, int it) {
                                   j->proj_rows);
             if (rc) return rc;
         }
-        {   // u = B(r) / B(P1)  (:212-218); also rho_i from the projector's partials
+        {   // u = B(r) / B(P1)  (:212-218); also rho_i from the projector's partials.  Static
+            // sweeps add lambda u to x in the same pass (:62), with the transposed copy the next
+            // view's x-branch rays read.
             const NormFinish nf{j->d_partials, nparts, j->d_norm + (long long)it * j->V + iv};
+            const AddStep add{j->d_img,
+                              (iv + 1 < j->V && j->vinfo[iv + 1].has_x) ? j->d_w1T : nullptr,
+                              j->opts.lambda, j->opts.nonneg};
             TimedRegion tr(j->ctx, TK_BACK);
-            rc = launch_backprojector(g, j->trig[iv], vi, BACK_RATIO, nullptr, j->d_rp, j->d_w2,
-                                      st, &nf, &j->back_tune);
+            rc = launch_backprojector(g, j->trig[iv], vi, BACK_RATIO, nullptr, j->d_rp,
+                                      motion ? j->d_w2 : nullptr, st, &nf, &j->back_tune, nullptr,
+                                      motion ? nullptr : &add);
             if (rc) return rc;
         }
-        {   // x += lambda * W(u; v_0 - v_i, f_0 - f_i)  (:222-223)
+        if (motion) {   // x += lambda * W(u; v_0 - v_i, f_0 - f_i)  (:222-223)
             TimedRegion tr(j->ctx, TK_WARP);
-            if (motion)
-                rc = launch_warp_add(j->d_img, j->d_w2, g.nx, g.ny, g.nz,
-                                     j->volumes[0] - j->volumes[iv], j->flows[0] - j->flows[iv],
-                                     f, j->opts.lambda, j->opts.nonneg, st);
-            else
-                rc = launch_add_tile(j->d_img,
-                                     (iv + 1 < j->V && j->vinfo[iv + 1].has_x) ? j->d_w1T : nullptr,
-                                     j->d_w2, g.nx, g.ny, g.nz, j->opts.lambda, j->opts.nonneg, st);
+            rc = launch_warp_add(j->d_img, j->d_w2, g.nx, g.ny, g.nz,
+                                 j->volumes[0] - j->volumes[iv], j->flows[0] - j->flows[iv], f,
+                                 j->opts.lambda, j->opts.nonneg, st);
             if (rc) return rc;
         }
     }

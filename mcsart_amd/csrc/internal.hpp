@@ -146,10 +146,19 @@ struct MotionStep {
     float wv, wf, vd, fd;
     int first, last;
 };
+// The static image sweep's update fused into the ratio backprojector (src/AUMISART.cu:62):
+//   x(p) = max(x(p) + lambda u(p), 0 if nonneg), and the same value at p^T in xT when given (the
+//   xy-transposed copy the next view's x-branch rays read); u itself is not stored.
+struct AddStep {
+    float *x, *xT;
+    float lambda;
+    int nonneg;
+};
+// At most one of mot and add: the ratio mode's u then goes to that step instead of out.
 int launch_backprojector(const Geom &g, const ViewTrig &t, const ViewInfo &vi, BackMode mode,
                          const float *proj, const float2 *rp, float *out, hipStream_t st,
                          const NormFinish *nf = nullptr, const BackTune *tune = nullptr,
-                         const MotionStep *mot = nullptr);
+                         const MotionStep *mot = nullptr, const AddStep *add = nullptr);
 
 // ---- warps and elementwise ------------------------------------------------------------------
 struct Fields {
@@ -164,9 +173,6 @@ int launch_warp(float *out, float *outT, const float *src, int nx, int ny, int n
                 float flow, const Fields &f, hipStream_t st);
 // dst = xy-transpose of src (dst[iy + ix*ny + iz*nx*ny] = src[ix + iy*nx + iz*nx*ny])
 int launch_transpose(float *dst, const float *src, int nx, int ny, int nz, hipStream_t st);
-// x += lambda * u, optional clamp, and (xT non-null) the transposed copy of the result
-int launch_add_tile(float *x, float *xT, const float *u, int nx, int ny, int nz, float lambda,
-                    int nonneg, hipStream_t st);
 // x += lambda * W(u; vol, flow), then optional clamp at zero.
 int launch_warp_add(float *x, const float *u, int nx, int ny, int nz, float vol, float flow,
                     const Fields &f, float lambda, int nonneg, hipStream_t st);

@@ -95,13 +95,16 @@ __device__ inline void motion_step_at(const MotionStep &m, const MotionIn &in, i
     if (m.pairT) m.pairT[idT] = gq;
 }
 
-template <int MODE, int WA, bool STAGE, bool MOT>
+// Epilogue of the ratio mode: store u, or the motion-sweep step, or the static sweep's add.
+enum BackEpi { EPI_STORE = 0, EPI_MOTION = 1, EPI_ADD = 2 };
+
+template <int MODE, int WA, bool STAGE, int EPI>
 __global__ void __launch_bounds__(BX *BY) k_backproject(GeomArgs g, double c, double s,
                                                        const float *__restrict__ proj,
                                                        const float2 *__restrict__ rp,
                                                        float *__restrict__ out, int zslab,
                                                        double ida, double idb, NormFinish nf,
-                                                       MotionStep mot) {
+                                                       MotionStep mot, AddStep add) {
     const int tid = threadIdx.y * BX + threadIdx.x;
     // the preceding projector's per-view norm (its partials are complete: kernel boundary); the
     // whole first wavefront (threads 0..63) of the first workgroup does it
@@ -248,7 +251,9 @@ __global__ void __launch_bounds__(BX *BY) k_backproject(GeomArgs g, double c, do
         for (int iz = iz0; iz < iz1; ++iz) {
             const int id = ix + iy * g.nx + iz * plane;
             MotionIn min_{};
-            if (MOT) min_ = motion_load(mot, id);
+            float x_old = 0.f;
+            if (EPI == EPI_MOTION) min_ = motion_load(mot, id);
+            if (EPI == EPI_ADD) x_old = add.x[id];
             // axial bounds (:131-132,:137-138) in anchor-relative FP32
             const float d = (float)(iz - iz0);
             const float L0 = fmaf(bl.q0, d, bl.l0), L1 = fmaf(bl.q1, d, bl.l1);
@@ -304,10 +309,16 @@ __global__ void __launch_bounds__(BX *BY) k_backproject(GeomArgs g, double c, do
                 // IEEE quotient); 0/0 and x/0 still give NaN / Inf, mapped to 0
                 float u = acc * __builtin_amdgcn_rcpf(acc1);
                 if (isnan(u) || isinf(u)) u = 0.f;
-                if (MOT)   // square slices: the transposed voxel is (iy, ix, iz)
+                if (EPI == EPI_MOTION) {   // square slices: the transposed voxel is (iy, ix, iz)
                     motion_step_at(mot, min_, id, iy + ix * g.nx + iz * plane, u);
-                else
+                } else if (EPI == EPI_ADD) {   // src/cu_add.cu:17, then the optional clamp
+                    float y = x_old + u * add.lambda;
+                    if (add.nonneg) y = fmaxf(y, 0.f);
+                    add.x[id] = y;
+                    if (add.xT) add.xT[iy + ix * g.ny + iz * plane] = y;
+                } else {
                     out[id] = u;
+                }
             }
         }
     };
@@ -328,18 +339,24 @@ int pick_zslab(const Geom &g) {
 template <int MODE, int WA>
 void launch_wa(dim3 grid, dim3 block, hipStream_t st, const GeomArgs &ga, const ViewTrig &t,
                const float *proj, const float2 *rp, float *out, int zs, const NormFinish &nf,
-               bool stage, const MotionStep *mot) {
+               bool stage, const MotionStep *mot, const AddStep *add) {
     const double ida = 1.0 / (double)ga.da, idb = 1.0 / (double)ga.db;
     const MotionStep m = mot ? *mot : MotionStep{};
-#define MCSART_BACK(ST, MT)                                                                      \
-    hipLaunchKernelGGL((k_backproject<MODE, WA, ST, MT>), grid, block, 0, st, ga, t.c, t.s, proj, \
-                       rp, out, zs, ida, idb, nf, m)
+    const AddStep ad = add ? *add : AddStep{};
+#define MCSART_BACK(ST, EP)                                                                      \
+    hipLaunchKernelGGL((k_backproject<MODE, WA, ST, EP>), grid, block, 0, st, ga, t.c, t.s, proj, \
+                       rp, out, zs, ida, idb, nf, m, ad)
+    constexpr int MOTION = MODE == BACK_RATIO ? EPI_MOTION : EPI_STORE;
+    constexpr int ADD = MODE == BACK_RATIO ? EPI_ADD : EPI_STORE;
     if (MODE == BACK_RATIO && mot) {
-        if (stage) MCSART_BACK(true, MODE == BACK_RATIO);
-        else MCSART_BACK(false, MODE == BACK_RATIO);
+        if (stage) MCSART_BACK(true, MOTION);
+        else MCSART_BACK(false, MOTION);
+    } else if (MODE == BACK_RATIO && add) {
+        if (stage) MCSART_BACK(true, ADD);
+        else MCSART_BACK(false, ADD);
     } else {
-        if (stage) MCSART_BACK(true, false);
-        else MCSART_BACK(false, false);
+        if (stage) MCSART_BACK(true, EPI_STORE);
+        else MCSART_BACK(false, EPI_STORE);
     }
 #undef MCSART_BACK
 }
@@ -347,8 +364,9 @@ void launch_wa(dim3 grid, dim3 block, hipStream_t st, const GeomArgs &ga, const 
 template <int MODE>
 void launch_mode(dim3 grid, dim3 block, hipStream_t st, const GeomArgs &ga, const ViewTrig &t,
                  int wa, const float *proj, const float2 *rp, float *out, int zs,
-                 const NormFinish &nf, bool stage, const MotionStep *mot) {
-#define MCSART_WA(W) launch_wa<MODE, W>(grid, block, st, ga, t, proj, rp, out, zs, nf, stage, mot)
+                 const NormFinish &nf, bool stage, const MotionStep *mot, const AddStep *add) {
+#define MCSART_WA(W)                                                                             \
+    launch_wa<MODE, W>(grid, block, st, ga, t, proj, rp, out, zs, nf, stage, mot, add)
     switch (wa <= 3 ? 3 : wa >= 8 ? 8 : wa) {
         case 3: MCSART_WA(3); break;
         case 4: MCSART_WA(4); break;
@@ -364,12 +382,15 @@ void launch_mode(dim3 grid, dim3 block, hipStream_t st, const GeomArgs &ga, cons
 
 int launch_backprojector(const Geom &g, const ViewTrig &t, const ViewInfo &vi, BackMode mode,
                          const float *proj, const float2 *rp, float *out, hipStream_t st,
-                         const NormFinish *nfp, const BackTune *tune, const MotionStep *mot) {
+                         const NormFinish *nfp, const BackTune *tune, const MotionStep *mot,
+                         const AddStep *add) {
     const GeomArgs ga{g.nx, g.ny, g.nz, g.na, g.nb, g.da, g.db, g.ai, g.bi, g.SO, g.SD};
     if (g.ndet() >= (1LL << 29))
         return fail(MCSART_ENOTSUP, "backprojector: detectors of 2^29 cells or more");
     if (mot && (mode != BACK_RATIO || g.nx != g.ny))
         return fail(MCSART_EINVAL, "backprojector: the motion step needs the ratio mode and nx == ny");
+    if (add && (mode != BACK_RATIO || mot))
+        return fail(MCSART_EINVAL, "backprojector: the add step needs the ratio mode alone");
     const int zs = (tune && tune->zslab > 0) ? tune->zslab : pick_zslab(g);
     const bool stage = !tune || tune->stage != 2;   // 0 automatic (staged), 1 staged, 2 global
     const dim3 block(BX, BY);
@@ -377,9 +398,10 @@ int launch_backprojector(const Geom &g, const ViewTrig &t, const ViewInfo &vi, B
     const NormFinish nf = nfp ? *nfp : NormFinish{nullptr, 0, nullptr};
     if (mode == BACK_PLAIN)
         launch_mode<BACK_PLAIN>(grid, block, st, ga, t, vi.wa, proj, rp, out, zs, nf, stage,
-                                nullptr);
+                                nullptr, nullptr);
     else
-        launch_mode<BACK_RATIO>(grid, block, st, ga, t, vi.wa, proj, rp, out, zs, nf, stage, mot);
+        launch_mode<BACK_RATIO>(grid, block, st, ga, t, vi.wa, proj, rp, out, zs, nf, stage, mot,
+                                add);
     MCSART_CHECK_LAUNCH();
     return MCSART_OK;
 }

@@ -662,34 +662,6 @@ __global__ void __launch_bounds__(TT *TR) k_motion_finish(const float *__restric
     }
 }
 
-__global__ void __launch_bounds__(TT *TR) k_add_tile(float *__restrict__ x, float *__restrict__ xT,
-                                                    const float *__restrict__ u, int nx, int ny,
-                                                    int nz, float lambda, int nonneg) {
-    __shared__ float t[TT][TT + 1];
-    const int X0 = blockIdx.x * TT, Y0 = blockIdx.y * TT, i2 = blockIdx.z;
-    const int i0 = X0 + threadIdx.x;
-    float xv[TK], uv[TK];
-    int id[TK];
-    bool ok[TK];
-#pragma unroll
-    for (int k = 0; k < TK; ++k) {
-        const int i1 = Y0 + threadIdx.y + k * TR;
-        ok[k] = i0 < nx && i1 < ny;
-        id[k] = i0 + i1 * nx + i2 * nx * ny;
-        xv[k] = ok[k] ? x[id[k]] : 0.f;
-        uv[k] = ok[k] ? u[id[k]] : 0.f;
-    }
-    float v[TK];
-#pragma unroll
-    for (int k = 0; k < TK; ++k) {
-        float y = xv[k] + uv[k] * lambda;   // src/cu_add.cu:17
-        if (nonneg) y = fmaxf(y, 0.f);
-        v[k] = y;
-        if (ok[k]) x[id[k]] = y;
-    }
-    if (xT) store_transposed(t, xT, v, nx, ny, X0, Y0, i2);
-}
-
 __global__ void __launch_bounds__(TT *TR) k_transpose(float *__restrict__ dst,
                                                      const float *__restrict__ src, int nx, int ny,
                                                      int nz) {
@@ -763,14 +735,6 @@ int launch_warp(float *out, float *outT, const float *src, int nx, int ny, int n
 int launch_transpose(float *dst, const float *src, int nx, int ny, int nz, hipStream_t st) {
     hipLaunchKernelGGL(k_transpose, tile_grid(nx, ny, nz), dim3(TT, TR), 0, st, dst, src, nx, ny,
                        nz);
-    MCSART_CHECK_LAUNCH();
-    return MCSART_OK;
-}
-
-int launch_add_tile(float *x, float *xT, const float *u, int nx, int ny, int nz, float lambda,
-                    int nonneg, hipStream_t st) {
-    hipLaunchKernelGGL(k_add_tile, tile_grid(nx, ny, nz), dim3(TT, TR), 0, st, x, xT, u, nx, ny,
-                       nz, lambda, nonneg);
     MCSART_CHECK_LAUNCH();
     return MCSART_OK;
 }

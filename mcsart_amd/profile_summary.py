@@ -40,10 +40,12 @@ def algorithmic_bytes(kernel: str, N: int, D: int):
     if m:
         mode = int(m.group(1))
         return {0: 4 * N + 4 * D, 1: 4 * N + 12 * D, 2: 8 * N + 12 * D}[mode]
-    m = re.match(r"k_backproject<(\d), \d+, \w+, (\w+)>", kernel)
+    m = re.match(r"k_backproject<(\d), \d+, \w+, (\d)>", kernel)
     if m:
-        if m.group(2) == "true":   # ratio mode with the motion step: one pair layout
+        if m.group(2) == "1":   # ratio mode with the motion step: one pair layout
             return 36 * N + 8 * N + 8 * D
+        if m.group(2) == "2":   # ratio mode with the static add: x read and written, one xT
+            return 12 * N + 8 * D
         return 4 * N + (4 * D if m.group(1) == "0" else 8 * D)
     table = {
         "k_warp_tile": 32 * N,        # 6 fields + x read, warped image written
@@ -54,7 +56,6 @@ def algorithmic_bytes(kernel: str, N: int, D: int):
         "k_motion_begin": 52 * N,               # fields, x read; constants, pairs written
         "k_motion_finish": 60 * N,              # fields read+written, accumulators, x
         "k_transpose": 8 * N,
-        "k_add_tile": 12 * N,
     }
     return table.get(re.sub(r"<(true|false)>$", "", kernel))
 
