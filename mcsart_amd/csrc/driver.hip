@@ -72,9 +72,11 @@ Layout layout(const Geom &g, int V, int n_iter, int keep_residual) {
         return o;
     };
     L.proj = take(sizeof(float) * D * V);
-    L.img = take(sizeof(float) * N);
     L.fields = take(sizeof(float) * N * 6);
     L.init = take(sizeof(float) * N * 7);
+    // the volumes a projector reads next to each other: its gathers address both layouts from
+    // one base with 32-bit byte offsets
+    L.img = take(sizeof(float) * N);
     L.w1 = take(sizeof(float) * N * 2);
     L.w1T = take(sizeof(float) * N * 2);
     L.w2 = take(sizeof(float) * N);

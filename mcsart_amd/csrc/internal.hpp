@@ -83,6 +83,10 @@ struct ProjArgs {
     float2 *rp;            // IMAGE/MOTION: D float2 (residual, normaliser projection)
     float *resid_copy;     // optional: residual copy (reference h_outproj slice)
     double *partials;      // per-block (sum p^2, sum r^2), plain stores
+    // set by launch_projector for the rows kernel: the lower of the two layouts' addresses and
+    // each layout's byte offset from it (both layouts within 4 GiB of it)
+    const char *rbase;
+    unsigned roffN, roffT;
 };
 // vi: the view's branches and footprint spans (footprint.hpp view_info).  SART modes leave one
 // (sum p^2, sum r^2) pair per workgroup in a.partials and report their count in *nparts; a later

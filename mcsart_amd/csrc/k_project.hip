@@ -258,41 +258,137 @@ __global__ void __launch_bounds__(RAYS *MAXSPLIT) k_project(GeomArgs g, double c
 // formula itself.  Overlaps are differences of the clamped boundary F(b) = min(max(b, lo), hi):
 // w_k = F(T0+k+1) - F(T0+k) = min(T0+k+1, hi) - max(T0+k, lo) whenever the cell meets the
 // footprint (src/cu_projection.cu:83,86), and 0 otherwise.  Plane offsets use 24-bit multiplies
-// (launch requires nx*ny < 2^24).  Requires the xy-transposed layout for x-branch rays.
-template <int WT>
-struct TPlane {
-    float w[WT];
-    float sw, width;
-    bool in;
-    unsigned base;   // T0 + ip * plane stride
-};
-
-// x where `in`, +0 elsewhere, as a bit mask rather than a select: the contribution of a plane
-// outside the volume is dropped even when its (in-volume, clamped) window holds non-finite
-// values, and the compiler keeps every gather unconditional instead of branching around them.
-__device__ inline float masked(float x, bool in) {
-    return __uint_as_float(__float_as_uint(x) & (in ? 0xffffffffu : 0u));
-}
-
+// and gathers 32-bit byte offsets from one base (launch requires nx*ny < 2^24 and both layouts
+// within 4 GiB of the lower one).  Requires the xy-transposed layout for x-branch rays.
 // med3(x, lo, hi) = min(max(x, lo), hi) when lo <= hi, in one instruction (v_med3_f32).
 __device__ inline float clampf(float x, float lo, float hi) {
     return __builtin_amdgcn_fmed3f(x, lo, hi);
 }
 
+// Weights of the W-voxel window over [lo, hi], 0 <= lo <= hi <= n (bounds already clamped into
+// the volume; lo == hi, all weights 0, for a plane outside it).
 template <int W>
 __device__ inline void window(float lo, float hi, float top, float (&w)[W], float &sw, float &f0) {
     f0 = clampf(floorf(lo), 0.f, top);   // window start, inside [0, n - W]
-    float F = fmaxf(f0, lo);              // F(f0): f0 <= hi whenever the plane is inside
+    float F = fmaxf(f0, lo);              // F(f0), = lo: f0 <= lo
     const float F0 = F;
 #pragma unroll
     for (int k = 0; k < W; ++k) {
-        // interior boundaries clamp to [lo, hi] (lo <= hi inside; outside the plane is dropped)
+        // interior boundaries clamp to [lo, hi]; the last one is >= lo (f0 + W > lo, or = n)
         const float Fn = (k + 1 < W) ? clampf(f0 + (float)(k + 1), lo, hi)
-                                     : fminf(f0 + (float)W, hi);   // f0 + W >= lo inside
+                                     : fminf(f0 + (float)W, hi);
         w[k] = Fn - F;
         F = Fn;
     }
     sw = F - F0;
+}
+
+// Footprint of one plane for the R rays of a lane.  The bounds are clamped into the volume
+// first, so a plane outside it gets weights of exactly zero from the window itself (no mask and
+// no select: every gather of its window is in the volume, and zero times a finite value adds
+// exactly nothing); inside, clamping changes no weight, because the window never reaches past
+// [0, n].  The scale uses the unclamped widths (src/cu_projection.cu:83,86).
+template <int WT, int WZ, int R>
+struct PlaneFoot {
+    float wt[WT], sw;
+    float wz[R][WZ], swz[R], sc[R];
+    unsigned off[R];   // byte offset of ray r's window corner
+};
+
+template <int WT, int WZ, int R, int LOGES>
+__device__ inline void plane_foot(const RayT &T, const RayZ (&Z)[R], float p, unsigned pofs,
+                                  unsigned sz, float ntf, float nzf, float ttop, float ztop,
+                                  PlaneFoot<WT, WZ, R> &P) {
+    const float t1 = fmaf(T.rlo, p, T.Klo), t2 = fmaf(T.rhi, p, T.Khi);
+    float f0;
+    window<WT>(clampf(t1, 0.f, ntf), clampf(t2, 0.f, ntf), ttop, P.wt, P.sw, f0);
+    const float width = t2 - t1;
+    const unsigned t0 = (unsigned)(int)f0;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const float z1 = fmaf(Z[r].rlo, p, Z[r].Klo), z2 = fmaf(Z[r].rhi, p, Z[r].Khi);
+        float g0;
+        window<WZ>(clampf(z1, 0.f, nzf), clampf(z2, 0.f, nzf), ztop, P.wz[r], P.swz[r], g0);
+        P.sc[r] = __builtin_amdgcn_rcpf(width * (z2 - z1));
+        P.off[r] = ((__umul24((unsigned)(int)g0, sz) + t0) << LOGES) + pofs;
+    }
+}
+
+// The window's values: WT consecutive elements on each of WZ z-rows, from a wave-uniform base
+// and 32-bit byte offsets (one scalar base, one vector offset per row).
+template <int WT, int WZ, typename E>
+__device__ inline void window_values(const char *__restrict__ vb, unsigned off, unsigned szB,
+                                     E (&val)[WT][WZ]) {
+#pragma unroll
+    for (int j = 0; j < WZ; ++j) {
+        const char *row = vb + (off + (unsigned)j * szB);
+#pragma unroll
+        for (int k = 0; k < WT; ++k) val[k][j] = *(const E *)(row + k * sizeof(E));
+    }
+}
+
+// PL planes from plane coordinate p: all footprints, then all gathers (in flight together),
+// then the weighted sums.
+template <int MODE, int WT, int WZ, int R, int PL>
+__device__ inline void rows_step(const char *__restrict__ vb, const RayT &T, const RayZ (&Z)[R],
+                                 float p, unsigned pofs, unsigned spB, unsigned sz, float ntf,
+                                 float nzf, float ttop, float ztop, float (&acc0)[R],
+                                 float (&acc1)[R]) {
+    constexpr int LOGES = MODE == PROJ_MOTION ? 3 : 2;
+    const unsigned szB = sz << LOGES;
+    PlaneFoot<WT, WZ, R> P[PL];
+#pragma unroll
+    for (int q = 0; q < PL; ++q)
+        plane_foot<WT, WZ, R, LOGES>(T, Z, p + (float)q, pofs + (unsigned)q * spB, sz, ntf, nzf,
+                                     ttop, ztop, P[q]);
+    if (MODE == PROJ_MOTION) {
+        float2 val[PL][R][WT][WZ];
+#pragma unroll
+        for (int q = 0; q < PL; ++q)
+#pragma unroll
+            for (int r = 0; r < R; ++r) window_values(vb, P[q].off[r], szB, val[q][r]);
+#pragma unroll
+        for (int q = 0; q < PL; ++q)
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                float s0 = 0.f, s1 = 0.f;
+#pragma unroll
+                for (int k = 0; k < WT; ++k) {
+                    float u0 = 0.f, u1 = 0.f;
+#pragma unroll
+                    for (int j = 0; j < WZ; ++j) {
+                        u0 = fmaf(P[q].wz[r][j], val[q][r][k][j].x, u0);
+                        u1 = fmaf(P[q].wz[r][j], val[q][r][k][j].y, u1);
+                    }
+                    s0 = fmaf(P[q].wt[k], u0, s0);
+                    s1 = fmaf(P[q].wt[k], u1, s1);
+                }
+                acc0[r] = fmaf(P[q].sc[r], s0, acc0[r]);
+                acc1[r] = fmaf(P[q].sc[r], s1, acc1[r]);
+            }
+    } else {
+        float val[PL][R][WT][WZ];
+#pragma unroll
+        for (int q = 0; q < PL; ++q)
+#pragma unroll
+            for (int r = 0; r < R; ++r) window_values(vb, P[q].off[r], szB, val[q][r]);
+#pragma unroll
+        for (int q = 0; q < PL; ++q)
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                float s0 = 0.f;
+#pragma unroll
+                for (int k = 0; k < WT; ++k) {
+                    float u0 = 0.f;
+#pragma unroll
+                    for (int j = 0; j < WZ; ++j) u0 = fmaf(P[q].wz[r][j], val[q][r][k][j], u0);
+                    s0 = fmaf(P[q].wt[k], u0, s0);
+                }
+                acc0[r] = fmaf(P[q].sc[r], s0, acc0[r]);
+                if (MODE == PROJ_IMAGE)   // P(1): the same weights on a volume of ones
+                    acc1[r] = fmaf(P[q].sc[r], P[q].sw * P[q].swz[r], acc1[r]);
+            }
+    }
 }
 
 template <int MODE, int WT, int WZ, int R>
@@ -324,109 +420,34 @@ __global__ void __launch_bounds__(RAYS * 16) k_project_rows(GeomArgs g, double c
         RayZ Z[R];
 #pragma unroll
         for (int r = 0; r < R; ++r) Z[r] = rz[r][tx];
+        // x-branch rays read the transposed layout, y-branch rays the natural one; both have
+        // the same index form (plane stride nt) and are addressed from one base, so every
+        // gather is that wave-uniform base plus a 32-bit byte offset
+        constexpr int LOGES = MODE == PROJ_MOTION ? 3 : 2;
         const bool xb = T.xbranch != 0;
         const int np = xb ? g.nx : g.ny;
         const int nt = xb ? g.ny : g.nx;
-        const unsigned sp = (unsigned)nt;   // plane stride: T[iy + ix*ny] / img[ix + iy*nx]
         const unsigned sz = (unsigned)g.nx * (unsigned)g.ny;
-        const float *__restrict__ v = xb ? a.vol0T : a.vol0;
-        const float2 *__restrict__ v2 = xb ? a.pairT : a.pair;
+        const unsigned spB = (unsigned)nt << LOGES;   // plane stride in bytes
         const float p0 = 0.5f - 0.5f * (float)np;   // (float)ip + 0.5 - n/2 (:52, :105)
         const float ntf = (float)nt, nzf = (float)g.nz;
         const float ttop = (float)(nt - WT), ztop = (float)(g.nz - WZ);
         const int chunk = (np + nsplit - 1) / nsplit;
         const int ip_lo = ty * chunk;
         const int ip_hi = min(np, ip_lo + chunk);
-        // PL planes per iteration, all their gathers in flight together (one plane when a lane
-        // walks four rays: sixteen or more gathers already)
+        // PL planes per step, all their gathers in flight together (one plane when a lane walks
+        // four rays: sixteen or more gathers already), the last ones one at a time.  p and the
+        // plane's byte offset are carried: p is a small integer + 0.5, exact in float.
         constexpr int PL = R >= 4 ? 1 : 2;
-        for (int ip = ip_lo; ip < ip_hi; ip += PL) {
-            const int npl = (ip + 1 < ip_hi) ? PL : 1;   // uniform over the wavefront
-            TPlane<WT> P[PL];
-            float wz[PL][R][WZ], swz[PL][R], sc[PL][R];
-            bool in[PL][R];
-            unsigned off[PL][R];
-#pragma unroll
-            for (int q = 0; q < PL; ++q) {
-                const float p = (float)(ip + q) + p0;
-                const float t1 = fmaf(T.rlo, p, T.Klo), t2 = fmaf(T.rhi, p, T.Khi);
-                float f0;
-                window<WT>(t1, t2, ttop, P[q].w, P[q].sw, f0);
-                P[q].width = t2 - t1;
-                P[q].in = (t2 > 0.f) & (t1 < ntf) & (q < npl);
-                P[q].base = (unsigned)(int)f0 + (unsigned)(ip + q) * sp;
-#pragma unroll
-                for (int r = 0; r < R; ++r) {
-                    const float z1 = fmaf(Z[r].rlo, p, Z[r].Klo), z2 = fmaf(Z[r].rhi, p, Z[r].Khi);
-                    float g0;
-                    window<WZ>(z1, z2, ztop, wz[q][r], swz[q][r], g0);
-                    in[q][r] = P[q].in & (z2 > 0.f) & (z1 < nzf);
-                    sc[q][r] = __builtin_amdgcn_rcpf(P[q].width * (z2 - z1));
-                    off[q][r] = P[q].base + __umul24((unsigned)(int)g0, sz);
-                }
-            }
-            if (MODE == PROJ_MOTION) {
-                float2 val[PL][R][WT][WZ];
-#pragma unroll
-                for (int q = 0; q < PL; ++q)
-#pragma unroll
-                    for (int r = 0; r < R; ++r) {
-                        // the second plane of a final odd chunk re-reads the first plane's voxels
-                        const float2 *qv = v2 + (q < npl ? off[q][r] : off[0][r]);
-#pragma unroll
-                        for (int k = 0; k < WT; ++k)
-#pragma unroll
-                            for (int j = 0; j < WZ; ++j) val[q][r][k][j] = qv[k + j * sz];
-                    }
-#pragma unroll
-                for (int q = 0; q < PL; ++q)
-#pragma unroll
-                    for (int r = 0; r < R; ++r) {
-                        float s0 = 0.f, s1 = 0.f;
-#pragma unroll
-                        for (int k = 0; k < WT; ++k) {
-                            float u0 = 0.f, u1 = 0.f;
-#pragma unroll
-                            for (int j = 0; j < WZ; ++j) {
-                                u0 = fmaf(wz[q][r][j], val[q][r][k][j].x, u0);
-                                u1 = fmaf(wz[q][r][j], val[q][r][k][j].y, u1);
-                            }
-                            s0 = fmaf(P[q].w[k], u0, s0);
-                            s1 = fmaf(P[q].w[k], u1, s1);
-                        }
-                        acc0[r] += masked(sc[q][r] * s0, in[q][r]);
-                        acc1[r] += masked(sc[q][r] * s1, in[q][r]);
-                    }
-            } else {
-                float val[PL][R][WT][WZ];
-#pragma unroll
-                for (int q = 0; q < PL; ++q)
-#pragma unroll
-                    for (int r = 0; r < R; ++r) {
-                        const float *qv = v + (q < npl ? off[q][r] : off[0][r]);
-#pragma unroll
-                        for (int k = 0; k < WT; ++k)
-#pragma unroll
-                            for (int j = 0; j < WZ; ++j) val[q][r][k][j] = qv[k + j * sz];
-                    }
-#pragma unroll
-                for (int q = 0; q < PL; ++q)
-#pragma unroll
-                    for (int r = 0; r < R; ++r) {
-                        float s0 = 0.f;
-#pragma unroll
-                        for (int k = 0; k < WT; ++k) {
-                            float u0 = 0.f;
-#pragma unroll
-                            for (int j = 0; j < WZ; ++j) u0 = fmaf(wz[q][r][j], val[q][r][k][j], u0);
-                            s0 = fmaf(P[q].w[k], u0, s0);
-                        }
-                        acc0[r] += masked(sc[q][r] * s0, in[q][r]);
-                        if (MODE == PROJ_IMAGE)   // P(1): the same weights on a volume of ones
-                            acc1[r] += masked(sc[q][r] * (P[q].sw * swz[q][r]), in[q][r]);
-                    }
-            }
-        }
+        int ip = ip_lo;
+        float p = (float)ip_lo + p0;
+        unsigned pofs = (xb ? a.roffT : a.roffN) + (unsigned)ip_lo * spB;
+        for (; ip + PL <= ip_hi; ip += PL, p += (float)PL, pofs += PL * spB)
+            rows_step<MODE, WT, WZ, R, PL>(a.rbase, T, Z, p, pofs, spB, sz, ntf, nzf, ttop, ztop,
+                                          acc0, acc1);
+        for (; PL > 1 && ip < ip_hi; ++ip, p += 1.f, pofs += spB)
+            rows_step<MODE, WT, WZ, R, 1>(a.rbase, T, Z, p, pofs, spB, sz, ntf, nzf, ttop, ztop,
+                                          acc0, acc1);
     }
 
     // Combine the plane chunks (fixed order: deterministic).
@@ -701,11 +722,23 @@ int launch_projector(const Geom &g, const ViewTrig &t, const ViewInfo &vi, ProjM
         return fail(MCSART_EINVAL, "projector rows must be 0 (automatic), 1, 2 or 4");
     // The rows kernel needs contiguous footprints for every ray (the transposed layout when the
     // view has x-branch rays), windows that fit in the volume and 24-bit plane offsets.
-    const bool have_t = mode == PROJ_MOTION ? a.pairT != nullptr : a.vol0T != nullptr;
-    const bool rows_ok = !wide && (!vi.has_x || have_t) && g.nx >= vi.wt && g.ny >= vi.wt &&
-                         g.nz >= vi.wz && (long long)g.nx * g.ny < (1LL << 24);
+    const uintptr_t un = (uintptr_t)(mode == PROJ_MOTION ? (const void *)a.pair : a.vol0);
+    const uintptr_t ut = (uintptr_t)(mode == PROJ_MOTION ? (const void *)a.pairT : a.vol0T);
+    const bool have_n = un != 0, have_t = ut != 0;
+    // one base for both layouts, each within 32-bit byte offsets of it
+    const uintptr_t base = !have_n ? ut : !have_t ? un : (un < ut ? un : ut);
+    const unsigned long long offN = have_n ? un - base : 0ull, offT = have_t ? ut - base : 0ull;
+    const unsigned long long vbytes = (unsigned long long)g.nvox() * (mode == PROJ_MOTION ? 8 : 4);
+    const bool rows_ok = !wide && (!vi.has_x || have_t) && (!vi.has_y || have_n) &&
+                         g.nx >= vi.wt && g.ny >= vi.wt && g.nz >= vi.wz &&
+                         (long long)g.nx * g.ny < (1LL << 24) &&
+                         (offN > offT ? offN : offT) + vbytes <= (1ull << 32);
     int rc;
     if (rows_ok) {
+        ProjArgs ar = a;
+        ar.rbase = (const char *)base;
+        ar.roffN = (unsigned)offN;
+        ar.roffT = (unsigned)offT;
         int R, S;
         pick_rows(g, rows, split, R, S);
         // The motion mode gathers (g, q) pairs: with four rays per lane its footprints' values
@@ -713,9 +746,9 @@ int launch_projector(const Geom &g, const ViewTrig &t, const ViewInfo &vi, ProjM
         // 74 us with two rays per lane, while the residual mode is fastest with four)
         if (mode == PROJ_MOTION && R == 4) R = 2;
         switch (mode) {
-            case PROJ_PLAIN: rc = launch_mode_rows<PROJ_PLAIN>(g, t, vi, a, st, R, S); break;
-            case PROJ_IMAGE: rc = launch_mode_rows<PROJ_IMAGE>(g, t, vi, a, st, R, S); break;
-            default: rc = launch_mode_rows<PROJ_MOTION>(g, t, vi, a, st, R, S); break;
+            case PROJ_PLAIN: rc = launch_mode_rows<PROJ_PLAIN>(g, t, vi, ar, st, R, S); break;
+            case PROJ_IMAGE: rc = launch_mode_rows<PROJ_IMAGE>(g, t, vi, ar, st, R, S); break;
+            default: rc = launch_mode_rows<PROJ_MOTION>(g, t, vi, ar, st, R, S); break;
         }
         if (nparts) *nparts = ((g.na + RAYS - 1) / RAYS) * ((g.nb + R - 1) / R);
         return rc;
