@@ -82,6 +82,29 @@ int ctx_scratch(mcsart_ctx *ctx, size_t bytes, void **ptr) {
     return MCSART_OK;
 }
 
+// The transposed copy of a volume for the projector's x-branch rays, in the context's scratch.
+// The rows projector addresses both layouts from one base with 32-bit byte offsets, so when the
+// caller's volume lies too far from the scratch, a copy of it goes beside the transposed one.
+static int stage_transposed(mcsart_ctx *ctx, const float *vol, int nx, int ny, int nz,
+                            const float **nat, const float **tr) {
+    const size_t bytes = sizeof(float) * (size_t)nx * ny * nz;
+    void *tmp = nullptr;
+    int rc = ctx_scratch(ctx, 2 * bytes, &tmp);
+    if (rc) return rc;
+    float *T = (float *)tmp;
+    rc = launch_transpose(T, vol, nx, ny, nz, ctx->stream);
+    if (rc) return rc;
+    const uintptr_t a = (uintptr_t)vol, b = (uintptr_t)T;
+    *tr = T;
+    *nat = vol;
+    if ((a > b ? a - b : b - a) + bytes > (1ull << 32)) {
+        float *copy = (float *)((char *)tmp + bytes);
+        MCSART_HIP_TRY(hipMemcpyAsync(copy, vol, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+        *nat = copy;
+    }
+    return MCSART_OK;
+}
+
 int ctx_reduction(mcsart_ctx *ctx, int nblocks, double **partials) {
     const size_t need = sizeof(double) * 2 * (size_t)nblocks;
     if (ctx->red_bytes < need) {
@@ -331,12 +354,8 @@ extern "C" int mcsart_projection_async(mcsart_ctx *ctx, float *d_proj, const flo
     a.vol0 = d_img;
     a.out = d_proj;
     if (vi.has_x) {   // x-branch rays read a transposed copy (contiguous footprints)
-        void *tmp = nullptr;
-        rc = ctx_scratch(ctx, sizeof(float) * (size_t)g.nvox(), &tmp);
+        rc = stage_transposed(ctx, d_img, nx, ny, nz, &a.vol0, &a.vol0T);
         if (rc) return rc;
-        rc = launch_transpose((float *)tmp, d_img, nx, ny, nz, ctx->stream);
-        if (rc) return rc;
-        a.vol0T = (const float *)tmp;
     }
     TimedRegion tr(ctx, TK_PROJ);
     return launch_projector(g, t, vi, PROJ_PLAIN, a, ctx->stream);
@@ -700,12 +719,8 @@ extern "C" int mcsart_sart_residual_async(mcsart_ctx *ctx, float *d_rp, float *d
     } else {
         a.vol0 = d_vol;
         if (vi.has_x) {
-            void *tmp = nullptr;
-            rc = ctx_scratch(ctx, sizeof(float) * (size_t)g.nvox(), &tmp);
+            rc = stage_transposed(ctx, d_vol, nx, ny, nz, &a.vol0, &a.vol0T);
             if (rc) return rc;
-            rc = launch_transpose((float *)tmp, d_vol, nx, ny, nz, ctx->stream);
-            if (rc) return rc;
-            a.vol0T = (const float *)tmp;
         }
     }
     a.meas = d_meas;

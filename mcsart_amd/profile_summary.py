@@ -57,7 +57,7 @@ def algorithmic_bytes(kernel: str, N: int, D: int):
         "k_motion_finish": 60 * N,              # fields read+written, accumulators, x
         "k_transpose": 8 * N,
     }
-    return table.get(re.sub(r"<(true|false)>$", "", kernel))
+    return table.get(re.sub(r"<(true|false)(, \d+)?>$", "", kernel))   # warps: <FAST, ROWS>
 
 
 def load_stats(d):
