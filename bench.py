@@ -39,16 +39,17 @@ def log(msg):
 def algorithmic_bytes(spec, kind, layouts=None):
     """HBM bytes one launch must move at minimum (4-byte floats).
     projector, image sweep : read x (N) and p (D), write (r, P1) (2D)      -> 4N + 12D
-    projector, motion sweep: read g, q (2N) and p (D), write (r, Pq) (2D)  -> 8N + 12D
+    projector, motion sweep: read g, Q (2N) and p (D), write (r, Pq) (2D)  -> 8N + 12D
     backprojector          : read (r, P1) (2D), write u (N)                -> 4N + 8D
     warp                   : read 6 fields and x, write x_i in each layout the view's rays
                              read (1 or 2)                                 -> 28N + 4N per layout
     warp-add               : read 6 fields, u and x, write x               -> 36N
     motion step            : the motion sweep's ratio backprojection with the view's update and
                              the next view's prep fused: read (r, Pq) (2D), read and write
-                             the two accumulators, read the constants (Ga, Gb, H, Q) and x,
-                             write (g, q) in each layout the next view's rays read
-                                                                           -> 36N + 8N per layout
+                             the two accumulators, read the constants (Ga, Gb, H) and x,
+                             write g in each layout the next view's rays read (q is the
+                             sweep's Q times a scalar, written once per sweep)
+                                                                           -> 32N + 4N per layout
                                                                               + 8D
     `layouts` (per view, bit 0 natural, bit 1 transposed) averages the per-layout terms over the
     views the kernel is launched for.
@@ -73,7 +74,7 @@ def algorithmic_bytes(spec, kind, layouts=None):
     if kind == "warp_add":
         return 36 * N
     if kind == "motion":
-        return 36 * N + 8 * N * mean_layouts(range(2, V)) + 8 * D   # prep of views 2..V-1
+        return 32 * N + 4 * N * mean_layouts(range(2, V)) + 8 * D   # prep of views 2..V-1
     raise KeyError(kind)
 
 

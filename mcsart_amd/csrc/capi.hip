@@ -138,12 +138,12 @@ TimedRegion::TimedRegion(mcsart_ctx *c, int k) : ctx(c), kind(k) {
     };
     a = take();
     b = take();
-    if (a && b) hipEventRecord(a, ctx->stream);
+    if (a && b) (void)hipEventRecord(a, ctx->stream);
 }
 
 TimedRegion::~TimedRegion() {
     if (!ctx || !ctx->timing || !a || !b) return;
-    hipEventRecord(b, ctx->stream);
+    (void)hipEventRecord(b, ctx->stream);
     ctx->pending.push_back({kind, a, b});
     if (ctx->pending.size() >= 4096) ctx_collect_timing(ctx);
 }
@@ -271,16 +271,16 @@ extern "C" int mcsart_ctx_create(mcsart_ctx **ctx, int device) {
 
 extern "C" int mcsart_ctx_destroy(mcsart_ctx *ctx) {
     if (!ctx) return MCSART_OK;
-    hipSetDevice(ctx->device);
-    hipStreamSynchronize(ctx->stream);
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
     for (auto &p : ctx->pending) {
-        hipEventDestroy(p.a);
-        hipEventDestroy(p.b);
+        (void)hipEventDestroy(p.a);
+        (void)hipEventDestroy(p.b);
     }
-    for (auto e : ctx->event_pool) hipEventDestroy(e);
-    if (ctx->scratch) hipFree(ctx->scratch);
-    if (ctx->red) hipFree(ctx->red);
-    hipStreamDestroy(ctx->stream);
+    for (auto e : ctx->event_pool) (void)hipEventDestroy(e);
+    if (ctx->scratch) (void)hipFree(ctx->scratch);
+    if (ctx->red) (void)hipFree(ctx->red);
+    (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return MCSART_OK;
 }
@@ -716,6 +716,7 @@ extern "C" int mcsart_sart_residual_async(mcsart_ctx *ctx, float *d_rp, float *d
     ProjArgs a{};
     if (motion_pairs) {
         a.pair = (const float2 *)d_vol;
+        a.qscale = 1.f;   // the pairs hold q itself
     } else {
         a.vol0 = d_vol;
         if (vi.has_x) {

@@ -28,11 +28,11 @@ struct mcsart_job {
     float *d_img = nullptr;      // N
     float *d_fields = nullptr;   // 6N: ax ay az bx by bz
     float *d_init = nullptr;     // 7N: initial image and fields (mcsart_job_reset)
-    float *d_w1 = nullptr;       // 2N: warped image (image sweep) / (g, q) pairs (motion sweep)
+    float *d_w1 = nullptr;       // 2N: warped image (image sweep) / (g, Q) pairs (motion sweep)
     float *d_w1T = nullptr;      // 2N: the same in the xy-transposed layout / transposed image
     float *d_w2 = nullptr;       // N: u
     float *d_acc = nullptr;      // 2N: motion-sweep accumulators, transposed (SVt, SFt)
-    float4 *d_cst = nullptr;     // N: motion-sweep constants (Ga, Gb, H, Q)
+    float *d_cst = nullptr;      // 3N: motion-sweep constants (Ga, Gb, H) per voxel
     float2 *d_rp = nullptr;      // D
     float *d_norm = nullptr;     // n_iter*V
     float *d_resid = nullptr;    // V*D (keep_residual)
@@ -81,7 +81,7 @@ Layout layout(const Geom &g, int V, int n_iter, int keep_residual) {
     L.w1T = take(sizeof(float) * N * 2);
     L.w2 = take(sizeof(float) * N);
     L.acc = take(sizeof(float) * N * 2);
-    L.cst = take(sizeof(float4) * N);
+    L.cst = take(sizeof(float) * N * 3);
     L.rp = take(sizeof(float2) * D);
     L.norm = take(sizeof(float) * (size_t)n_iter * V);
     L.resid = take(keep_residual ? sizeof(float) * D * V : 0);
@@ -110,7 +110,7 @@ Fields fields_of(const mcsart_job *j) {
 // Image sweep (src/AUMISART.cu:182-235; static form :40-65).
 int image_sweep(mcsart_job *j, int it) {
     const Geom &g = j->g;
-    const long long N = g.nvox(), D = g.ndet();
+    const long long D = g.ndet();
     hipStream_t st = j->ctx->stream;
     const Fields f = fields_of(j);
     const bool motion = j->opts.motion != 0;
@@ -180,25 +180,25 @@ int motion_sweep(mcsart_job *j, int it) {
     const long long D = g.ndet(), N = g.nvox();
     hipStream_t st = j->ctx->stream;
     const int stride = j->opts.motion ? 1 : 10;
+    // pairs (g, Q) in each layout: the view's prep g, the sweep's Q (q = Q (-vd - fd))
     float2 *pair = (float2 *)j->d_w1, *pairT = (float2 *)j->d_w1T;
     auto vdiff = [&](int iv) { return j->volumes[iv] - j->volumes[iv - 1]; };   // :249
     auto fdiff = [&](int iv) { return j->flows[iv] - j->flows[iv - 1]; };       // :250
     int rc, nparts = 0;
     if (j->V < 2) return MCSART_OK;   // no view to update
-    {   // constants and the prep of the first view
-        const ViewInfo &vi = j->vinfo[1];
+    {   // constants, Q and the prep of the first view
         TimedRegion tr(j->ctx, TK_MOTION);
-        rc = launch_motion_begin(fields_of(j), j->d_img, j->d_cst, vi.has_y ? pair : nullptr,
-                                 vi.has_x ? pairT : nullptr, g.nx, g.ny, g.nz, vdiff(1),
-                                 fdiff(1), st);
+        rc = launch_motion_begin(fields_of(j), j->d_img, j->d_cst, pair, pairT, g.nx, g.ny, g.nz,
+                                 vdiff(1), fdiff(1), st);
         if (rc) return rc;
     }
     for (int iv = 1; iv < j->V; iv += stride) {
         const ViewInfo &vi = j->vinfo[iv];
-        {   // r = p_i - P(x - img0), P(q), rho_i  (:246-263)
+        {   // r = p_i - P(x - img0), P(q) = (-vd - fd) P(Q), rho_i  (:246-263)
             ProjArgs a{};
             a.pair = pair;
             a.pairT = vi.has_x ? pairT : nullptr;
+            a.qscale = -vdiff(iv) + -fdiff(iv);
             a.meas = j->d_proj + D * iv;
             a.rp = j->d_rp;
             a.partials = j->d_partials;
@@ -295,7 +295,7 @@ extern "C" int mcsart_job_create(mcsart_ctx *ctx, const mcsart_geom *gp, int n_v
     j->d_w1T = (float *)(b + L.w1T);
     j->d_w2 = (float *)(b + L.w2);
     j->d_acc = (float *)(b + L.acc);
-    j->d_cst = (float4 *)(b + L.cst);
+    j->d_cst = (float *)(b + L.cst);
     j->d_rp = (float2 *)(b + L.rp);
     j->d_norm = (float *)(b + L.norm);
     j->d_resid = opts->keep_residual ? (float *)(b + L.resid) : nullptr;
@@ -303,7 +303,7 @@ extern "C" int mcsart_job_create(mcsart_ctx *ctx, const mcsart_geom *gp, int n_v
     // everything starts at zero: image, fields, norms, residuals
     if (hipMemsetAsync(j->block, 0, L.total, ctx->stream) != hipSuccess ||
         hipStreamSynchronize(ctx->stream) != hipSuccess) {
-        hipFree(j->block);
+        (void)hipFree(j->block);
         delete j;
         return fail(MCSART_EHIP, "job initialisation failed");
     }
@@ -378,8 +378,8 @@ extern "C" int mcsart_job_reset(mcsart_job *j) {
 namespace {
 
 void drop_graph(mcsart_job::SweepGraph &sg) {
-    if (sg.exec) hipGraphExecDestroy(sg.exec);
-    if (sg.graph) hipGraphDestroy(sg.graph);
+    if (sg.exec) (void)hipGraphExecDestroy(sg.exec);
+    if (sg.graph) (void)hipGraphDestroy(sg.graph);
     sg = mcsart_job::SweepGraph{};
 }
 
@@ -522,7 +522,7 @@ int time_on_stream(mcsart_job *j, int reps, F &&fn, float *ms) {
     hipEvent_t a = nullptr, b = nullptr;
     MCSART_HIP_TRY(hipEventCreate(&a));
     if (hipEventCreate(&b) != hipSuccess) {
-        hipEventDestroy(a);
+        (void)hipEventDestroy(a);
         return fail(MCSART_EHIP, "hipEventCreate failed");
     }
     int rc = MCSART_OK;
@@ -531,8 +531,8 @@ int time_on_stream(mcsart_job *j, int reps, F &&fn, float *ms) {
     if (!rc && (hipEventRecord(b, st) != hipSuccess || hipEventSynchronize(b) != hipSuccess ||
                 hipEventElapsedTime(ms, a, b) != hipSuccess))
         rc = fail(MCSART_EHIP, "event timing failed");
-    hipEventDestroy(a);
-    hipEventDestroy(b);
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
     return rc;
 }
 
@@ -621,10 +621,12 @@ extern "C" int mcsart_job_kernel_time(mcsart_job *j, int kind, int reps, double 
                 const BackTune bt{j->back_zslab_motion, j->back_tune.stage};
                 r = launch_backprojector(g, j->trig[iv], vi, BACK_RATIO, nullptr, j->d_rp,
                                          nullptr, st, nullptr, &bt, &m);
-            } else if (kind == 5) {   // projector of the motion sweep: reads (g, q) pairs
+            } else if (kind == 5) {   // projector of the motion sweep: reads (g, Q) pairs
                 ProjArgs a{};
                 a.pair = pair;
                 a.pairT = vi.has_x ? pairT : nullptr;
+                a.qscale = -(j->volumes[iv] - j->volumes[iv > 0 ? iv - 1 : 0]) +
+                           -(j->flows[iv] - j->flows[iv > 0 ? iv - 1 : 0]);
                 a.meas = j->d_proj + D * iv;
                 a.rp = j->d_rp;
                 a.partials = j->d_partials;
@@ -732,7 +734,7 @@ extern "C" int mcsart_job_autotune(mcsart_job *j, mcsart_tuning *chosen) {
             }
         }
     // the motion sweep's backprojector (fused update and next prep) on the same views (view 1
-    // at the least: it needs a previous view), into the sweep's scratch accumulators and pairs
+    // at the least: it needs a previous view), into the sweep's scratch accumulators and preps
     if (j->V >= 2) {
         const long long N = g.nvox();
         float2 *pair = (float2 *)j->d_w1, *pairT = (float2 *)j->d_w1T;
@@ -777,10 +779,10 @@ extern "C" int mcsart_job_autotune(mcsart_job *j, mcsart_tuning *chosen) {
 
 extern "C" int mcsart_job_destroy(mcsart_job *j) {
     if (!j) return MCSART_OK;
-    hipSetDevice(j->ctx->device);
-    hipStreamSynchronize(j->ctx->stream);
+    (void)hipSetDevice(j->ctx->device);
+    (void)hipStreamSynchronize(j->ctx->stream);
     for (auto &sg : j->graphs) drop_graph(sg);
-    hipFree(j->block);
+    (void)hipFree(j->block);
     delete j;
     return MCSART_OK;
 }

@@ -66,18 +66,21 @@ __device__ inline int xcd_order(int L, int T) {
 enum ProjMode {
     PROJ_PLAIN = 0,   // out[id] = P(x)
     PROJ_IMAGE = 1,   // rp[id] = (p[id] - P(x), P(1)), norms of p and r
-    PROJ_MOTION = 2,  // rp[id] = (p[id] - P(g), P(q)), norms of p and r
+    PROJ_MOTION = 2,  // rp[id] = (p[id] - P(g), qscale P(Q)), norms of p and r
 };
 
 // Volumes the projector reads.  y-branch rays read the natural layout img[ix + iy*nx + iz*nx*ny];
 // x-branch rays read the xy-transposed layout imgT[iy + ix*ny + iz*nx*ny] when it is given (it
 // makes their gathers contiguous) and the natural layout otherwise.  PLAIN/IMAGE project vol0;
-// MOTION projects the interleaved pairs (g, q).
+// MOTION projects interleaved pairs (g, Q) and scales P(Q) by qscale: the motion sweep's
+// q = Q (-vd - fd) is a per-sweep volume times a per-view scalar, so Q is written once per sweep
+// and only g per view.
 struct ProjArgs {
     const float *vol0;     // natural layout (PLAIN/IMAGE)
     const float *vol0T;    // transposed layout or null
-    const float2 *pair;    // (g, q) natural layout (MOTION)
-    const float2 *pairT;   // (g, q) transposed layout or null (MOTION)
+    const float2 *pair;    // (g, Q) natural layout (MOTION)
+    const float2 *pairT;   // (g, Q) transposed layout or null (MOTION)
+    float qscale;          // MOTION: P(q) = qscale P(Q)
     const float *meas;     // measured projection of this view (IMAGE/MOTION)
     float *out;            // PLAIN: D floats
     float2 *rp;            // IMAGE/MOTION: D float2 (residual, normaliser projection)
@@ -140,13 +143,15 @@ struct BackTune {
 // accumulators are kept transposed, SVt(p) = SV(p^T), so every access is at the voxel itself):
 //   SVt(p) += wv u(p), SFt(p) += wf u(p)                (update of view i, from zero if `first`)
 //   img0(p) = vd Ga(p) + fd Gb(p) + H(p) (vd SVt(p) + fd SFt(p))
-//   (g, q)(p) = (x(p) - img0(p), Q(p) (-vd - fd))      (prep of view i+1, unless `last`)
-// with cst = (Ga, Gb, H, Q) from launch_motion_begin; u itself is not stored.
+//   g(p) = x(p) - img0(p)                               (prep of view i+1, unless `last`)
+// with cst = (Ga, Gb, H) per voxel (3 floats) from launch_motion_begin, which also wrote the
+// sweep's Q (the projector scales P(Q) by -vd - fd); u itself is not stored.
 struct MotionStep {
     float *svt, *sft;
-    const float4 *cst;
+    const float *cst;
     const float *x;
-    float2 *pair, *pairT;   // prep outputs in the layouts the next view reads (either may be null)
+    float2 *pair, *pairT;   // the g halves of the (g, Q) pairs the next view reads (either
+                            // may be null); Q stays as launch_motion_begin wrote it
     float wv, wf, vd, fd;
     int first, last;
 };
@@ -212,8 +217,9 @@ int launch_motion_step(float *const fields[6], const float *u, const float *x, f
                        float fd, bool update, bool prep, hipStream_t st);
 
 // The motion sweep in accumulator form (x constant over the sweep): per-sweep constants
-// cst = (Ga, Gb, H, Q) from the starting fields plus the prep of the first view ...
-int launch_motion_begin(const Fields &f, const float *x, float4 *cst, float2 *pair,
+// cst = (Ga, Gb, H) (3 floats per voxel) from the starting fields, and pairs (g, Q) in both
+// layouts: Q = (D1 + D2 + D3) x and the prep g of the first view ...
+int launch_motion_begin(const Fields &f, const float *x, float *cst, float2 *pair,
                         float2 *pairT, int nx, int ny, int nz, float vd, float fd,
                         hipStream_t st);
 // ... then per updated view the ratio backprojector with a MotionStep, and at the end of the

@@ -69,15 +69,19 @@ __device__ inline float2 det_at(const void *base, unsigned off) {
 // transposed accumulators and the next view's prep at the voxel itself, instead of storing u.
 // The voxel's inputs are loaded before its backprojection (MotionIn), so their latency hides
 // behind the row loop.
+struct MotionCst {
+    float ga, gb, h;   // 12 bytes per voxel: one 3-dword load
+};
 struct MotionIn {
     float sv, sf, x;
-    float4 c;   // (Ga, Gb, H, Q)
+    MotionCst c;
 };
 __device__ inline MotionIn motion_load(const MotionStep &m, int id) {
     MotionIn in;
     in.sv = m.first ? 0.f : m.svt[id];
     in.sf = m.first ? 0.f : m.sft[id];
-    in.c = m.last ? make_float4(0.f, 0.f, 0.f, 0.f) : m.cst[id];
+    in.c = m.last ? MotionCst{0.f, 0.f, 0.f}
+                  : *reinterpret_cast<const MotionCst *>(m.cst + 3u * (unsigned)id);
     in.x = m.last ? 0.f : m.x[id];
     return in;
 }
@@ -88,11 +92,13 @@ __device__ inline void motion_step_at(const MotionStep &m, const MotionIn &in, i
     m.svt[id] = sv;
     m.sft[id] = sf;
     if (m.last) return;
-    const float4 c = in.c;
-    const float img0 = m.vd * c.x + m.fd * c.y + c.z * (m.vd * sv + m.fd * sf);
-    const float2 gq = make_float2(in.x - img0, c.w * (-m.vd + -m.fd));
-    if (m.pair) m.pair[id] = gq;
-    if (m.pairT) m.pairT[idT] = gq;
+    const MotionCst c = in.c;
+    const float img0 = m.vd * c.ga + m.fd * c.gb + c.h * (m.vd * sv + m.fd * sf);
+    // only the g half of the pair: the next view's q = Q (-vd - fd) is the sweep's Q, scaled
+    // inside the projector
+    const float gv = in.x - img0;
+    if (m.pair) reinterpret_cast<float *>(m.pair)[2u * (unsigned)id] = gv;
+    if (m.pairT) reinterpret_cast<float *>(m.pairT)[2u * (unsigned)idT] = gv;
 }
 
 // Epilogue of the ratio mode: store u, or the motion-sweep step, or the static sweep's add.

@@ -537,43 +537,18 @@ __global__ void __launch_bounds__(TT *TR) k_motion_step(float *__restrict__ ax,
 //   img0(p) = vd Ga(p) + fd Gb(p) + H(p) (vd SV(p^T) + fd SF(p^T)),
 //   Ga(p) = sum_c alpha_c0(p^T) D_c x(p),  Gb(p) = sum_c beta_c0(p^T) D_c x(p),
 //   H(p) = sum_c D_c x(p^T) D_c x(p),  Q(p) = (D1 + D2 + D3) x(p),
-// so a view step reads and writes two accumulators instead of six fields.  The per-sweep
-// constants (Ga, Gb, H, Q) are made once by k_motion_begin; each view's update and the next
-// view's prep run in the ratio backprojector's epilogue (k_backproject.hip), which keeps the
-// accumulators transposed (svt(p) = SV(p^T)); k_motion_finish writes the fields at the end.
+// so a view step reads and writes two accumulators instead of six fields, and the view's
+// q = Q (-vd - fd) is the sweep's Q times a scalar (the projector scales P(Q); q is never
+// stored).  k_motion_begin makes (Ga, Gb, H) and Q once per sweep; each view's update and the
+// next view's prep g run in the ratio backprojector's epilogue (k_backproject.hip), which keeps
+// the accumulators transposed (svt(p) = SV(p^T)); k_motion_finish writes the fields at the end.
 
-// Store pair values (g, q) of the A^T lanes: natural layout at jd[k] and/or, through LDS, the
-// transposed layout at the A lanes' indices id[k].
-__device__ inline void store_pairs(float (*t0)[TT + 1], float (*t1)[TT + 1],
-                                   float2 *__restrict__ pair, float2 *__restrict__ pairT,
-                                   const float *gv, const float *qv, const int *id, const bool *ok,
-                                   const int *jd, const bool *jok) {
-    const int tx = threadIdx.x, ty = threadIdx.y;
-    if (pair) {
-#pragma unroll
-        for (int k = 0; k < TK; ++k)
-            if (jok[k]) pair[jd[k]] = make_float2(gv[k], qv[k]);
-    }
-    if (pairT) {
-        __syncthreads();   // every earlier read of t0/t1 is done
-#pragma unroll
-        for (int k = 0; k < TK; ++k) {
-            t0[ty + k * TR][tx] = gv[k];
-            t1[ty + k * TR][tx] = qv[k];
-        }
-        __syncthreads();
-#pragma unroll
-        for (int k = 0; k < TK; ++k) {
-            const int r = ty + k * TR;
-            if (ok[k]) pairT[id[k]] = make_float2(t0[tx][r], t1[tx][r]);
-        }
-    }
-}
-
-// Per-sweep constants C = (Ga, Gb, H, Q) at every voxel from the sweep's starting fields, and the
-// prep (g, q) of the sweep's first view (both accumulators still zero).
+// Per-sweep constants (Ga, Gb, H) at every voxel from the sweep's starting fields, and the pairs
+// (g, Q) in both layouts: Q = (D1 + D2 + D3) x for the whole sweep, g the prep of its first
+// view (both accumulators still zero).  The natural layout is written by the A^T lanes at jd[k],
+// the transposed one through LDS at the A lanes' indices id[k].
 __global__ void __launch_bounds__(TT *TR) k_motion_begin(Fields f, const float *__restrict__ x,
-                                                        float4 *__restrict__ cst,
+                                                        float *__restrict__ cst,
                                                         float2 *__restrict__ pair,
                                                         float2 *__restrict__ pairT, int nx,
                                                         int ny, int nz, float vd, float fd) {
@@ -613,12 +588,28 @@ __global__ void __launch_bounds__(TT *TR) k_motion_begin(Fields f, const float *
         const float ga = t[1][tx][r] * d.d1 + t[0][tx][r] * d.d2 + t[2][tx][r] * d.d3;
         const float gb = t[4][tx][r] * d.d1 + t[3][tx][r] * d.d2 + t[5][tx][r] * d.d3;
         const float h = t[6][tx][r] * d.d1 + t[7][tx][r] * d.d2 + t[8][tx][r] * d.d3;
-        const float q = d.d1 + d.d2 + d.d3;
-        if (jok[k]) cst[jd[k]] = make_float4(ga, gb, h, q);
+        if (jok[k]) {
+            float *c = cst + 3u * (unsigned)jd[k];
+            c[0] = ga;
+            c[1] = gb;
+            c[2] = h;
+            pair[jd[k]] = make_float2(d.x - (vd * ga + fd * gb), d.d1 + d.d2 + d.d3);
+        }
         gv[k] = d.x - (vd * ga + fd * gb);
-        qv[k] = q * (-vd + -fd);
+        qv[k] = d.d1 + d.d2 + d.d3;
     }
-    store_pairs(t[0], t[1], pair, pairT, gv, qv, id, ok, jd, jok);
+    __syncthreads();   // every read of t[0], t[1] above is done
+#pragma unroll
+    for (int k = 0; k < TK; ++k) {
+        t[0][ty + k * TR][tx] = gv[k];
+        t[1][ty + k * TR][tx] = qv[k];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < TK; ++k) {
+        const int r = ty + k * TR;
+        if (ok[k]) pairT[id[k]] = make_float2(t[0][tx][r], t[1][tx][r]);
+    }
 }
 
 // End of a motion sweep in accumulator form: alpha_c = alpha_c0 + D_c x SV and
@@ -831,7 +822,7 @@ int launch_motion_update(float *const fields[6], const float *u, const float *x,
                               false, st);
 }
 
-int launch_motion_begin(const Fields &f, const float *x, float4 *cst, float2 *pair,
+int launch_motion_begin(const Fields &f, const float *x, float *cst, float2 *pair,
                         float2 *pairT, int nx, int ny, int nz, float vd, float fd,
                         hipStream_t st) {
     hipLaunchKernelGGL(k_motion_begin, tile_grid(nx, ny, nz), dim3(TT, TR), 0, st, f, x, cst,

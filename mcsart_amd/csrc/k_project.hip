@@ -19,7 +19,8 @@
 // neighbouring rays would read one float from 64 different cache lines.  They read instead an
 // xy-transposed copy (written by the kernel that produced the volume), in which their footprints
 // are contiguous; y-branch rays read the natural layout.  In the motion sweep the two volumes
-// (g, q) are interleaved as float2, so each gather is one 8-byte load for both projections.
+// (g, Q) are interleaved as float2, so each gather is one 8-byte load for both projections, and
+// P(q) = qscale P(Q) (q = Q (-vd - fd): Q is written once per sweep, g per view).
 //
 // SART modes fuse the residual r = p - P(x), the normaliser projection P(1) (image sweep) or the
 // second projection P(q) (motion sweep) and the per-view norms ||p||, ||r|| (per-workgroup FP64
@@ -232,7 +233,7 @@ __global__ void __launch_bounds__(RAYS *MAXSPLIT) k_project(GeomArgs g, double c
     if (live) {
         const float pm = a.meas[id];
         const float r = pm - acc0;
-        a.rp[id] = make_float2(r, acc1);
+        a.rp[id] = make_float2(r, MODE == PROJ_MOTION ? acc1 * a.qscale : acc1);
         if (a.resid_copy) a.resid_copy[id] = r;
         pp = (double)pm * (double)pm;
         rr = (double)r * (double)r;
@@ -487,7 +488,7 @@ __global__ void __launch_bounds__(RAYS * 16) k_project_rows(GeomArgs g, double c
         const long long id = (long long)(ib0 + r) * g.na + ia;
         const float pm = a.meas[id];
         const float res = pm - acc0[r];
-        a.rp[id] = make_float2(res, acc1[r]);
+        a.rp[id] = make_float2(res, MODE == PROJ_MOTION ? acc1[r] * a.qscale : acc1[r]);
         if (a.resid_copy) a.resid_copy[id] = res;
         pp += (double)pm * (double)pm;
         rr += (double)res * (double)res;
@@ -656,8 +657,8 @@ __global__ void __launch_bounds__(RAYS *MAXSPLIT) k_project_wide(GeomArgs g, dou
     if (!live) return;
     if (MODE == PROJ_PLAIN) {
         a.out[id] = acc0;
-    } else {
-        a.rp[id] = make_float2(acc0, acc1);   // raw sums; finished by k_sart_finish
+    } else {   // raw sums, finished by k_sart_finish
+        a.rp[id] = make_float2(acc0, MODE == PROJ_MOTION ? acc1 * a.qscale : acc1);
     }
 }
 

@@ -252,7 +252,7 @@ extern "C" int mcsart_server_create(int device, const mcsart_server_opts *o,
         if (rc) {
             if (s.ctx) mcsart_ctx_destroy(s.ctx);
             for (auto &x : sv->slots) {
-                hipEventDestroy(x.done);
+                (void)hipEventDestroy(x.done);
                 mcsart_ctx_destroy(x.ctx);
             }
             delete sv;
@@ -383,7 +383,7 @@ extern "C" int mcsart_server_destroy(mcsart_server *sv) {
     sv->cv_work.notify_all();
     if (sv->worker.joinable()) sv->worker.join();
     for (auto &s : sv->slots) {
-        hipEventDestroy(s.done);
+        (void)hipEventDestroy(s.done);
         mcsart_ctx_destroy(s.ctx);
     }
     for (auto &kv : sv->reqs) delete kv.second;

@@ -42,8 +42,8 @@ def algorithmic_bytes(kernel: str, N: int, D: int):
         return {0: 4 * N + 4 * D, 1: 4 * N + 12 * D, 2: 8 * N + 12 * D}[mode]
     m = re.match(r"k_backproject<(\d), \d+, \w+, (\d)>", kernel)
     if m:
-        if m.group(2) == "1":   # ratio mode with the motion step: one pair layout
-            return 36 * N + 8 * N + 8 * D
+        if m.group(2) == "1":   # ratio mode with the motion step: one prep layout
+            return 32 * N + 4 * N + 8 * D
         if m.group(2) == "2":   # ratio mode with the static add: x read and written, one xT
             return 12 * N + 8 * D
         return 4 * N + (4 * D if m.group(1) == "0" else 8 * D)
@@ -53,7 +53,7 @@ def algorithmic_bytes(kernel: str, N: int, D: int):
         "k_motion_step<true, true>": 72 * N,    # 6 fields read+written, u, x, pairs
         "k_motion_step<true, false>": 56 * N,
         "k_motion_step<false, true>": 40 * N,
-        "k_motion_begin": 52 * N,               # fields, x read; constants, pairs written
+        "k_motion_begin": 52 * N,               # fields, x read; constants, Q x2, g written
         "k_motion_finish": 60 * N,              # fields read+written, accumulators, x
         "k_transpose": 8 * N,
     }
