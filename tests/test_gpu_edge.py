@@ -9,7 +9,7 @@ import oracle
 from mcsart_amd import configs
 from mcsart_amd.driver import Context, Job
 from mcsart_amd.geometry import Geometry, uniform_angles
-from mcsart_amd.ops import backproject, project
+from mcsart_amd.ops import DeviceArray, backproject, project
 from _util import rel_l2, smooth_random, smooth_random_large
 
 pytestmark = pytest.mark.gpu
@@ -58,6 +58,23 @@ def _job(ctx, g, V, op_iter, motion):
     job = Job(ctx, g, ang, z, z, np.array(op_iter, np.int32), lam=1.0, motion=motion)
     job.set_projections(np.ones((V,) + g.det_shape, np.float32))
     return job, ang
+
+
+def test_projection_of_a_far_input(gpu_lib):
+    """An input more than 4 GiB away from the context's scratch, where its transposed copy goes:
+    the rows projector addresses both layouts from one base with 32-bit offsets, so the entry
+    point stages a copy of the input beside the transposed one.  Results are bit-identical to the
+    near case (usually the allocator places the input above the 5 GiB spacer, far from the
+    scratch that the first call allocated)."""
+    g = Geometry.from_user(32, 32, 16, 48, 24, da=1.0, db=1.0, SO=100.0, SD=160.0)
+    vol = smooth_random(g.vol_shape, 21)
+    angles = [np.float32(a) for a in (0.0, 0.8, 1.6, 2.4)]
+    near = [project(vol, g, a) for a in angles]
+    with DeviceArray.empty(((5 << 30) // 4,)):
+        far = [project(vol, g, a) for a in angles]
+    for a, n, f in zip(angles, near, far):
+        assert np.array_equal(f, n), a
+        assert rel_l2(f, oracle.projection(vol, g, a)) <= TOL, a
 
 
 def test_empty_and_single_view_schedules(gpu_lib):
