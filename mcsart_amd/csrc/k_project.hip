@@ -574,10 +574,10 @@ void launch_rows(const GeomArgs &ga, const ViewTrig &t, const ProjArgs &a, hipSt
 template <int MODE, int WT, int WZ>
 void launch_rows_r(const GeomArgs &ga, const ViewTrig &t, const ProjArgs &a, hipStream_t st,
                    int R, int S) {
-    if (R == 4 && MODE != PROJ_MOTION)   // the motion mode runs at most two rays per lane
-        launch_rows<MODE, WT, WZ, (MODE == PROJ_MOTION ? 2 : 4)>(ga, t, a, st, S);
-    else if (R >= 2) launch_rows<MODE, WT, WZ, 2>(ga, t, a, st, S);
-    else launch_rows<MODE, WT, WZ, 1>(ga, t, a, st, S);
+    if (MODE == PROJ_MOTION || R == 1)   // the motion mode runs one ray per lane
+        launch_rows<MODE, WT, WZ, 1>(ga, t, a, st, S);
+    else if (R == 2) launch_rows<MODE, WT, WZ, (MODE == PROJ_MOTION ? 1 : 2)>(ga, t, a, st, S);
+    else launch_rows<MODE, WT, WZ, (MODE == PROJ_MOTION ? 1 : 4)>(ga, t, a, st, S);
 }
 
 template <int MODE>
@@ -742,10 +742,11 @@ int launch_projector(const Geom &g, const ViewTrig &t, const ViewInfo &vi, ProjM
         ar.roffT = (unsigned)offT;
         int R, S;
         pick_rows(g, rows, split, R, S);
-        // The motion mode gathers (g, q) pairs: with four rays per lane its footprints' values
-        // hold twice the registers and the lanes wait on them (105 us at 256x256x128 against
-        // 74 us with two rays per lane, while the residual mode is fastest with four)
-        if (mode == PROJ_MOTION && R == 4) R = 2;
+        // The motion mode gathers (g, Q) pairs: with more rays per lane its footprints' values
+        // hold twice the registers and the lanes wait on them.  One ray per lane is its best
+        // shape at 128^3 and 256x256x128 (10.4 / 56.6 us against 15.0 / 61.4 us with two),
+        // while the residual mode is fastest with two or four at 256x256x128.
+        if (mode == PROJ_MOTION) R = 1;
         switch (mode) {
             case PROJ_PLAIN: rc = launch_mode_rows<PROJ_PLAIN>(g, t, vi, ar, st, R, S); break;
             case PROJ_IMAGE: rc = launch_mode_rows<PROJ_IMAGE>(g, t, vi, ar, st, R, S); break;
