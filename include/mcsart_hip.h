@@ -345,7 +345,7 @@ int mcsart_job_autotune(mcsart_job *job, mcsart_tuning *chosen);
  * shapes: 0 projector (residual mode), 1 backprojector (ratio mode), 2 warp of the image into the
  * layouts the view's rays read, 3 warp-add (x += lambda W(u)), 4 motion step (the motion
  * sweep's ratio backprojection with the update of view i, as accumulators, and the prep of view
- * i+1 fused), 5 the motion sweep's projector (reads the (g, q) pairs).  Every view of the scan
+ * i+1 fused), 5 the motion sweep's projector (reads the (g, Q) pairs).  Every view of the scan
  * (views 1..V-2 for kind 4) is launched back to back `reps` times between two HIP events on the
  * job's stream.  Outputs go to the job's scratch: kind 3 updates a copy of the image, kinds 4
  * and 5 the motion sweep's accumulators and pairs; the job's image, fields and norms are

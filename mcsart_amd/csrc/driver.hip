@@ -172,9 +172,10 @@ int image_sweep(mcsart_job *j, int it) {
 
 // Motion (udvf) sweep (src/AUMISART.cu:236-288; static overload :66-118 steps views by 10).
 // Per view: prep (g, q) -> projector -> backprojector -> field update.  x is constant over the
-// sweep, so the updates are carried as two accumulators: launch_motion_begin makes the per-sweep
-// constants and the first view's prep; each view's update and the next view's prep run in the
-// ratio backprojector's epilogue (MotionStep); launch_motion_finish writes the fields.
+// sweep, so the updates are carried as two accumulators and q = Q (-vd - fd) is a per-sweep
+// volume times a scalar: launch_motion_begin makes the per-sweep constants, Q and the first
+// view's prep g; each view's update and the next view's g run in the ratio backprojector's
+// epilogue (MotionStep); the projector scales P(Q); launch_motion_finish writes the fields.
 int motion_sweep(mcsart_job *j, int it) {
     const Geom &g = j->g;
     const long long D = g.ndet(), N = g.nvox();
