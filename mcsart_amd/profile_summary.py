@@ -181,9 +181,38 @@ def stats_markdown(trace, title, command_lines, notes=()):
     return "\n".join(out) + "\n"
 
 
+def lds_markdown(pmc, title, command_lines, notes=()):
+    """Per-kernel LDS and issue shares from one rocprofv3 --pmc pass with SQ_LDS_BANK_CONFLICT,
+    SQ_LDS_IDX_ACTIVE, SQ_INSTS_LDS, SQ_WAIT_INST_LDS, SQ_ACTIVE_INST_VALU, SQ_WAVE_CYCLES and
+    SQ_WAIT_INST_ANY (per-dispatch means)."""
+    per = collections.defaultdict(lambda: collections.defaultdict(list))
+    for r in csv.DictReader(open(os.path.join(pmc, "run_counter_collection.csv"))):
+        per[short(r["Kernel_Name"])][r["Counter_Name"]].append(float(r["Counter_Value"]))
+    out = [f"# {title}", "", "Command (on the MI355X box, from /tmp, PYTHONPATH = repository root):",
+           ""]
+    out += ["    " + c for c in command_lines]
+    out += ["", "Per-dispatch means over the run.  Bank-conflict ratio = SQ_LDS_BANK_CONFLICT / "
+            "(SQ_LDS_IDX_ACTIVE - SQ_LDS_BANK_CONFLICT) (the tool's own derived metric); the "
+            "wait and active columns are per-wave cycle shares (SQ_WAIT_INST_LDS, "
+            "SQ_WAIT_INST_ANY, SQ_ACTIVE_INST_VALU over SQ_WAVE_CYCLES).", ""]
+    out += ["| kernel | launches | LDS instructions | bank-conflict ratio | LDS wait share | "
+            "any-wait share | VALU-active share |", "|---|---:|---:|---:|---:|---:|---:|"]
+    for k, v in per.items():
+        m = {c: sum(x) / len(x) for c, x in v.items()}
+        if "SQ_WAVE_CYCLES" not in m or "rocclr" in k or not m["SQ_WAVE_CYCLES"]:
+            continue
+        bc, ia, wc = m["SQ_LDS_BANK_CONFLICT"], m["SQ_LDS_IDX_ACTIVE"], m["SQ_WAVE_CYCLES"]
+        out.append(f"| `{k}` | {len(v['SQ_WAVE_CYCLES'])} | {m['SQ_INSTS_LDS']:.0f} | "
+                   f"{bc / max(ia - bc, 1.0):.3f} | {m['SQ_WAIT_INST_LDS'] / wc:.3f} | "
+                   f"{m['SQ_WAIT_INST_ANY'] / wc:.3f} | {m['SQ_ACTIVE_INST_VALU'] / wc:.3f} |")
+    out += [""] + list(notes)
+    return "\n".join(out) + "\n"
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--stats", help="write only the --stats table of this trace directory")
+    ap.add_argument("--lds", help="write only the LDS/issue table of this --pmc directory")
     ap.add_argument("--title", default="rocprofv3 kernel statistics")
     ap.add_argument("--note", action="append", default=[])
     ap.add_argument("--workload", choices=sorted(WORKLOADS))
@@ -197,6 +226,14 @@ def main(argv=None):
     ap.add_argument("--command", action="append", default=[],
                     help="command line that produced the inputs (repeatable)")
     args = ap.parse_args(argv)
+    if args.lds:
+        md = lds_markdown(args.lds, args.title, args.command, args.note)
+        if args.md:
+            with open(args.md, "w") as f:
+                f.write(md)
+        else:
+            print(md)
+        return
     if args.stats:
         md = stats_markdown(args.stats, args.title, args.command, args.note)
         if args.md:
