@@ -3,8 +3,10 @@
 // Reference: kernel_backprojection, src/cu_backprojection.cu:49-170 (one thread per voxel, every
 // call allocates a CUDA array, copies the projection into it and builds a texture, :10-45).
 //
-// Work layout: a workgroup owns 32 x 8 (ix, iy) columns and a slab of ZS voxels along z; one lane
-// owns a column.  The transverse footprint (a bounds, weights) of a column does not depend on z,
+// Work layout: a workgroup owns 16 x 16 (ix, iy) columns and a slab of ZS voxels along z; one
+// lane owns a column.  A square tile's footprint on the detector is the narrowest, so its staged
+// rectangle (below) is the smallest (measured: 256x256x128 image sweep 61 -> 51 us, motion sweep
+// 87 -> 77 us against 32 x 8; 64 x 4 was slower than both).  The transverse footprint (a bounds, weights) of a column does not depend on z,
 // so it is formed once per column in FP64 (footprint.hpp); each voxel only forms its axial bounds.
 //
 // LDS staging of the detector: the workgroup first reduces, in LDS, the rectangle of detector
@@ -25,7 +27,7 @@
 namespace mcsart {
 namespace {
 
-constexpr int BX = 32, BY = 8;
+constexpr int BX = 16, BY = 16;
 constexpr int LDS_CELLS = 4096;   // staged cells per workgroup (32 KB of float2)
 
 // Axial bounds of a column's voxels relative to an integer anchor: the four lines
