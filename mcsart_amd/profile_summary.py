@@ -82,7 +82,11 @@ def load_pmc(d):
 
 SIMDS = 1024          # 256 CUs x 4 SIMDs
 SQ_INSTANCES = 32     # SQ_BUSY_CYCLES is summed over the 32 shader engines
-VALU_CYCLES = 4       # a wave64 VALU instruction occupies a SIMD16 for 4 cycles (no packed FP32)
+# SIMD cycles per wave64 VALU instruction.  Measured, not assumed: counting 4 (a 16-lane SIMD),
+# the instructions counted for the 256x256x128 motion projector and image backprojector would
+# need 1.37 and 1.05 times the SIMD cycles their launches had, so gfx950 retires its common
+# 32-bit VALU instructions in 2; the figure is an upper-bound estimate of issue occupancy.
+VALU_CYCLES = 2
 
 
 def load_sq(d):
@@ -148,8 +152,9 @@ def to_markdown(s, command_lines):
     out += ["    " + c for c in command_lines]
     out += ["", f"Counter calibration against the run's final 64 MiB device copy: FETCH_SIZE x "
             f"{s['fetch_factor']}, WRITE_SIZE x {s['write_factor']}.  VALU busy = "
-            "4 x SQ_INSTS_VALU / (1024 SIMDs x SQ_BUSY_CYCLES / 32): the share of the SIMDs' "
-            "cycles spent issuing vector instructions.", "",
+            "2 x SQ_INSTS_VALU / (1024 SIMDs x SQ_BUSY_CYCLES / 32): the share of the SIMDs' "
+            "cycles spent issuing vector instructions (2 cycles per wave64 instruction: with 4, "
+            "some kernels would exceed 1).", "",
             "| kernel | launches | avg us | algorithmic MB | achieved GB/s | frac of 8 TB/s | "
             "fetch MB | write MB | traffic / algorithmic | VALU busy |",
             "|---|---:|---:|---:|---:|---:|---:|---:|---:|---:|"]
