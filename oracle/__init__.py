@@ -69,6 +69,8 @@ def _declare(lib):
         getattr(lib, f"oracle_invert_dvf_{sfx}").argtypes = [_P] * 6 + [_I] * 4
         getattr(lib, f"oracle_generate_proj_{sfx}").argtypes = (
             [_P] * 11 + [_I, _I, R, R, R, _I, R, R, _I, R, _I, _I, _I])
+        getattr(lib, f"oracle_motion_sweep_{sfx}").argtypes = (
+            [_P] * 9 + [_I] * 7 + [_P] * 3 + [R] * 6)
         getattr(lib, f"oracle_aumisart_{sfx}").argtypes = (
             [_P] * 6 + [_I] * 7 + [_P] + [R] * 7 + [_P] * 3 + [_P] * 6 + [_I, _I])
     lib.oracle_num_threads.restype = _I
@@ -277,3 +279,20 @@ def aumisart(h_img, h_proj, g, angles, volumes, flows, op_iter, lam=1.0, fields=
         float(g.da), float(g.db), float(g.ai), float(g.bi), float(g.SO), float(g.SD), float(lam),
         _p(vol), _p(flo), _p(ang), *[_p(f) for f in fs], int(motion), int(bool(nonneg)))
     return out_img, out_norm, out_ax, out_proj
+
+
+def motion_sweep(img, proj, g, angles, volumes, flows, fields, stride=1, dtype=np.float64):
+    """One motion (udvf) sweep (``src/AUMISART.cu:242-288``; ``stride=10`` is overload 1's form,
+    ``:72-118``) started from ``fields`` = (ax, ay, az, bx, by, bz) and image ``img``.  Returns
+    ``(fields, norms)``: the six updated fields and the n_views residual ratios (entry 0 is 0)."""
+    V = len(angles)
+    img = _arr(img, dtype).reshape(g.nz, g.ny, g.nx)
+    proj = _arr(proj, dtype).reshape(V, g.nb, g.na)
+    fs = [f.copy() for f in _fields(fields, img.shape, dtype)]
+    ang, vol, flo = (_arr(x, dtype).ravel() for x in (angles, volumes, flows))
+    out_norm = np.empty(V, dtype=dtype)
+    getattr(_load(), f"oracle_motion_sweep_{_sfx(dtype)}")(
+        *[_p(f) for f in fs], _p(out_norm), _p(img), _p(proj), g.nx, g.ny, g.nz, g.na, g.nb, V,
+        int(stride), _p(ang), _p(vol), _p(flo), float(g.da), float(g.db), float(g.ai),
+        float(g.bi), float(g.SO), float(g.SD))
+    return fs, out_norm

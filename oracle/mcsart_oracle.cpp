@@ -18,6 +18,9 @@
 //     fractions);
 //   * deform2 reads that fall past the end of the volume (reachable as written, undefined in the
 //     reference) contribute a zero difference;
+//   * warp displacements beyond +-(n+1) voxels are clamped to +-(n+1) (the same edge voxel under
+//     clamp-to-edge addressing) and NaN displacements to -(n+1), so the float->int conversions of
+//     the trilinear sample are always defined (the HIP warp does the same);
 //   * the driver's host image h_outimg starts equal to h_img (uninitialised in the reference).
 #include <cmath>
 #include <cstdint>
@@ -179,15 +182,30 @@ void backprojection(T* img, const T* proj, T angle, T SO, T SD, T da, int na, T 
 // the clamp-to-edge trilinear sample of img at (i1 + my[id], i2 + mx[id], i3 + mz[id]) (the
 // texture is addressed with unnormalised coordinates and the -0.5 texel convention, :86-89).
 // ---------------------------------------------------------------------------------------------
+// A displacement beyond +-(n+1) lands outside the volume on the same side, where clamp-to-edge
+// addressing returns the edge voxel whatever its size, so it is clamped to +-(n+1) before the
+// floor and the integer conversion (which are then always in range).  A NaN displacement maps to
+// -(n+1), the lower edge, as the HIP warp does (v_med3_f32 returns its smallest operand on NaN).
 template <typename T>
-inline T trilinear(const T* img, int n0, int n1, int n2, T p0, T p1, T p2) {
+inline T clamp_disp(T m, int n) {
+    const T lim = (T)(n + 1);
+    if (std::isnan(m)) return -lim;
+    return m < -lim ? -lim : (m > lim ? lim : m);
+}
+
+// Clamp-to-edge trilinear sample of img (dim 0 fastest) at (i0 + m0, i1 + m1, i2 + m2).
+template <typename T>
+inline T trilinear(const T* img, int n0, int n1, int n2, int i0, T m0, int i1, T m1, int i2,
+                   T m2) {
+    const T p0 = (T)i0 + clamp_disp(m0, n0), p1 = (T)i1 + clamp_disp(m1, n1);
+    const T p2 = (T)i2 + clamp_disp(m2, n2);
     const T f0 = std::floor(p0), f1 = std::floor(p1), f2 = std::floor(p2);
     const T t0 = p0 - f0, t1 = p1 - f1, t2 = p2 - f2;
-    const long long i0 = (long long)f0, i1 = (long long)f1, i2 = (long long)f2;
+    const long long i0f = (long long)f0, i1f = (long long)f1, i2f = (long long)f2;
     auto cl = [](long long i, int n) { return i < 0 ? 0 : (i >= n ? n - 1 : i); };
-    const long long a0 = cl(i0, n0), b0 = cl(i0 + 1, n0);
-    const long long a1 = cl(i1, n1), b1 = cl(i1 + 1, n1);
-    const long long a2 = cl(i2, n2), b2 = cl(i2 + 1, n2);
+    const long long a0 = cl(i0f, n0), b0 = cl(i0f + 1, n0);
+    const long long a1 = cl(i1f, n1), b1 = cl(i1f + 1, n1);
+    const long long a2 = cl(i2f, n2), b2 = cl(i2f + 1, n2);
     auto at = [&](long long x, long long y, long long z) {
         return img[x + y * (long long)n0 + z * (long long)n0 * n1];
     };
@@ -213,7 +231,7 @@ void deform(T* out, const T* img, int nx, int ny, int nz, T volume, T flow, cons
         const T mx = ax[id] * volume + bx[id] * flow;   // :74
         const T my = ay[id] * volume + by[id] * flow;   // :75
         const T mz = az[id] * volume + bz[id] * flow;   // :76
-        out[id] = trilinear(src.data(), nx, ny, nz, (T)i1 + my, (T)i2 + mx, (T)i3 + mz);
+        out[id] = trilinear(src.data(), nx, ny, nz, i1, my, i2, mx, i3, mz);
     }
 }
 
@@ -521,9 +539,9 @@ void invert_dvf(T* mx2, T* my2, T* mz2, const T* mx, const T* my, const T* mz, i
             for (int ix = 0; ix < nx; ++ix) {
                 T x = 0, y = 0, z = 0;
                 for (int it = 0; it < niter; ++it) {
-                    x = -trilinear(mx, nx, ny, nz, (T)ix + x, (T)iy + y, (T)iz + z);   // :11
-                    y = -trilinear(my, nx, ny, nz, (T)ix + x, (T)iy + y, (T)iz + z);   // :12
-                    z = -trilinear(mz, nx, ny, nz, (T)ix + x, (T)iy + y, (T)iz + z);   // :13
+                    x = -trilinear(mx, nx, ny, nz, ix, x, iy, y, iz, z);   // :11
+                    y = -trilinear(my, nx, ny, nz, ix, x, iy, y, iz, z);   // :12
+                    z = -trilinear(mz, nx, ny, nz, ix, x, iy, y, iz, z);   // :13
                 }
                 const long long id = lin(ix, iy, iz, nx, ny);
                 mx2[id] = x;
@@ -629,8 +647,27 @@ void generate_proj(T* proj, const T* img, const T* const fl[6], const T* volumes
                     op_iter, lambda, volumes, flows, angles, fl, motion != 0, nonneg != 0);        \
     }
 
+// One motion (udvf) sweep of host_AUMISART on its own (src/AUMISART.cu:242-288, overload 1's
+// stride-10 form :72-118): the six fields are updated in place and the per-view residual ratios
+// go to outnorm[n_views] (view 0 stays 0).  Tests start it from any state (teacher forcing).
+#define ORACLE_SWEEP_EXPORT(T, SFX)                                                                \
+    extern "C" void oracle_motion_sweep_##SFX(                                                     \
+        T* ax, T* ay, T* az, T* bx, T* by, T* bz, T* outnorm, const T* img, const T* h_proj,       \
+        int nx, int ny, int nz, int na, int nb, int n_views, int stride, const T* angles,          \
+        const T* volumes, const T* flows, T da, T db, T ai, T bi, T SO, T SD) {                    \
+        Geo<T> g{nx, ny, nz, na, nb, da, db, ai, bi, SO, SD};                                      \
+        T* fp[6] = {ax, ay, az, bx, by, bz};                                                       \
+        const long long N = (long long)nx * ny * nz, D = (long long)na * nb;                       \
+        std::vector<T> work(3 * N + 3 * D);                                                        \
+        std::memset(outnorm, 0, sizeof(T) * (size_t)n_views);                                      \
+        sart_motion_sweep<T>(g, n_views, stride, 0, angles, volumes, flows, h_proj, img, fp,       \
+                             outnorm, work);                                                       \
+    }
+
 ORACLE_EXPORTS(double, f64)
 ORACLE_EXPORTS(float, f32)
+ORACLE_SWEEP_EXPORT(double, f64)
+ORACLE_SWEEP_EXPORT(float, f32)
 
 extern "C" int oracle_num_threads(void) { return omp_get_max_threads(); }
 extern "C" void oracle_set_num_threads(int n) { omp_set_num_threads(n); }

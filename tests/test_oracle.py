@@ -411,3 +411,164 @@ def test_generate_proj_composition():
     for i in range(3):
         np.testing.assert_allclose(p0[i], oracle.projection(img, g, ang[i]), rtol=1e-12,
                                    atol=1e-12)
+
+
+# ---- extreme displacements ---------------------------------------------------------------------
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_deform_extreme_and_nonfinite_displacements(dtype):
+    """A displacement far outside the volume samples the edge voxel on its side (clamp-to-edge
+    addressing), whatever its size: +-1e20 and +-Inf give the edge values exactly; NaN goes to the
+    lower edge, as in the HIP warp.  Before the clamp the float->int conversion of such a
+    displacement was undefined (1e20 gave index-0 values, +Inf gave NaN)."""
+    nz, ny, nx = 5, 6, 6
+    img = smooth_random((nz, ny, nx), 40).astype(dtype)
+    zero = np.zeros((nz, ny, nx), dtype)
+    # (field index, numpy axis it displaces): ay -> dim 1 (x, axis 2), ax -> dim 2 (y, axis 1),
+    # az -> dim 3 (z, axis 0)  (src/cu_deform.cu:86-89)
+    for comp, axis in ((1, 2), (0, 1), (2, 0)):
+        n = img.shape[axis]
+        for val, edge in ((1e20, n - 1), (-1e20, 0), (np.inf, n - 1), (-np.inf, 0), (np.nan, 0)):
+            f = [zero] * 6
+            f[comp] = np.full((nz, ny, nx), val, dtype)
+            out = oracle.deform(img, 1.0, 0.0, f, dtype=dtype)
+            ref = np.broadcast_to(np.take(img, [edge], axis=axis), img.shape)
+            assert np.array_equal(out, ref.astype(dtype)), (comp, val)
+    # mixed: an in-range fractional displacement on one axis, an extreme one on another
+    f = [zero] * 6
+    f[1] = np.full((nz, ny, nx), 0.25, dtype)
+    f[2] = np.full((nz, ny, nx), -3e30, dtype)
+    out = oracle.deform(img, 1.0, 0.0, f, dtype=dtype)
+    z0 = np.broadcast_to(img[:1], img.shape).astype(np.float64)
+    x = np.arange(nx)
+    lo, hi = x, np.minimum(x + 1, nx - 1)
+    ref = 0.75 * z0[:, :, lo] + 0.25 * z0[:, :, hi]
+    np.testing.assert_allclose(out, ref, rtol=0, atol=1e-6 if dtype == np.float32 else 1e-14)
+
+
+# ---- motion (udvf) sweep: closed form -----------------------------------------------------------
+
+def _T(v):
+    """xy transpose as add2 reads its second operand: T(v)[iz, iy, ix] = v[iz, ix, iy]
+    (src/cu_add.cu:58)."""
+    return np.swapaxes(v, 1, 2)
+
+
+def ramp_motion_scan(g, V, stride, slopes=(0.02, 0.035, 0.05), b0=0.5, cs=None, seed=50,
+                     volumes=None, flows=None, angles=None, model_errors=()):
+    """A motion-sweep scan whose result has a closed form.
+
+    The image is a ramp x = b0 + a1 ix + a2 iy + a3 iz, so its forward differences are the
+    constants D1 = a1, D2 = a2, D3 = a3 (zero on each axis' last plane).  Then, for view i with
+    vd = v_i - v_{i-1}, fd = f_i - f_{i-1} (src/AUMISART.cu:249-250):
+      img0 = vd (T(ay) D1 + T(ax) D2 + T(az) D3) + fd (T(by) D1 + T(bx) D2 + T(bz) D3)  (:252-258)
+      q    = (D1 + D2 + D3) (-vd - fd)                                                   (:267)
+    and measured data p_i = P_i(x - img0 + c_i q) make the residual r = p_i - P x + P img0
+    (:246-260) equal to c_i P q, so u = B r / B(P q) = c_i wherever B(P q) != 0 and 0 elsewhere
+    (0/0 -> 0, :270), by the linearity of P and B.  The update (:272-277) then adds
+    T(u)(-vd) D_c to (ay, ax, az) and T(u)(-fd) D_c to (by, bx, bz), and the view's residual ratio
+    is ||c_i P q|| / ||p_i|| (:263).
+
+    ``model_errors`` plants one mistake in this model (not in the oracle) to show the check
+    discriminates: "swap_xy" pairs ax with D1 and ay with D2, "vd_sign" flips vd in q,
+    "no_transpose" drops T in the update.  Returns (x, fields0, proj, angles, volumes, flows,
+    expected fields, expected ratios)."""
+    nz, ny, nx = g.vol_shape
+    a1, a2, a3 = slopes
+    iz, iy, ix = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    x = b0 + a1 * ix + a2 * iy + a3 * iz
+    D1 = np.where(ix < nx - 1, a1, 0.0)
+    D2 = np.where(iy < ny - 1, a2, 0.0)
+    D3 = np.where(iz < nz - 1, a3, 0.0)
+    F0 = [f.astype(np.float64) for f in random_fields(g.vol_shape, seed, amp=1.5)]
+    ang = uniform_angles(V) if angles is None else np.asarray(angles, np.float32)
+    k = np.arange(V)
+    # default trace: vd = -0.06 - 0.02 sin(k), fd = 0.03 (-1)^k, so -vd - fd >= 0.01
+    vol = np.asarray(volumes, np.float64) if volumes is not None else \
+        np.concatenate([[0.0], np.cumsum(-0.06 - 0.02 * np.sin(k[1:]))])
+    flo = np.asarray(flows, np.float64) if flows is not None else \
+        np.concatenate([[0.0], np.cumsum(0.03 * (-1.0) ** k[1:])])
+    if cs is None:
+        cs = 0.8 * np.cos(0.7 * np.arange(V)) + 0.3
+    ax, ay, az, bx, by, bz = [f.copy() for f in F0]
+    proj = np.zeros((V, g.nb, g.na))
+    ratios = np.zeros(V)
+    T = (lambda v: v) if "no_transpose" in model_errors else _T
+    pair1, pair2 = ((ax, ay), (bx, by)) if "swap_xy" in model_errors else ((ay, ax), (by, bx))
+    for i in range(1, V, stride):
+        vd, fd = vol[i] - vol[i - 1], flo[i] - flo[i - 1]
+        assert abs(-vd - fd) > 1e-3
+        img0 = (vd * (_T(ay) * D1 + _T(ax) * D2 + _T(az) * D3)
+                + fd * (_T(by) * D1 + _T(bx) * D2 + _T(bz) * D3))
+        qs = (vd - fd) if "vd_sign" in model_errors else (-vd - fd)
+        q = (D1 + D2 + D3) * (-vd - fd)
+        p = oracle.projection(x - img0 + cs[i] * q, g, ang[i])
+        proj[i] = p
+        Pq = oracle.projection((D1 + D2 + D3) * qs, g, ang[i])
+        S = oracle.backprojection(Pq, g, ang[i]) != 0
+        u = cs[i] * (-vd - fd) / qs * S
+        ratios[i] = np.linalg.norm(cs[i] * oracle.projection(q, g, ang[i])) / np.linalg.norm(p)
+        Tu = T(u)
+        c1, c2 = pair1
+        c1 += Tu * (-vd) * D1
+        c2 += Tu * (-vd) * D2
+        az += Tu * (-vd) * D3
+        d1, d2 = pair2
+        d1 += Tu * (-fd) * D1
+        d2 += Tu * (-fd) * D2
+        bz += Tu * (-fd) * D3
+    return x, F0, proj, ang, vol, flo, [ax, ay, az, bx, by, bz], ratios
+
+
+@pytest.mark.parametrize("stride", [1, 10])
+def test_motion_sweep_closed_form(stride):
+    """The oracle's motion sweep equals the ramp closed form (fields and residual ratios), with a
+    detector narrow enough that B(P q)'s support is not xy-symmetric (a dropped transpose shows),
+    fields with ax != ay and ay != T(ay), and vd != fd."""
+    g = Geometry.from_user(16, 16, 12, 6, 20, da=1.0, db=1.0, SO=60.0, SD=90.0)
+    V = 5 if stride == 1 else 13
+    x, F0, proj, ang, vol, flo, F, ratios = ramp_motion_scan(g, V, stride)
+    got, nrm = oracle.motion_sweep(x, proj, g, ang, vol, flo, F0, stride=stride)
+    for c in range(6):
+        np.testing.assert_allclose(got[c], F[c], rtol=0, atol=1e-12)
+    np.testing.assert_allclose(nrm, ratios, rtol=1e-10, atol=0)
+    upd = list(range(1, V, stride))
+    assert np.all(nrm[upd] > 0) and np.count_nonzero(nrm) == len(upd)
+    # every field moved (the increments are not trivially zero) and the support is one-sided
+    assert all(np.abs(F[c] - F0[c]).max() > 1e-4 for c in range(6))
+    inc = np.abs(F[1] - F0[1]) > 0
+    assert inc.any() and not np.array_equal(inc, _T(inc))
+    # the closed form discriminates the plausible mistakes
+    for err in ("swap_xy", "vd_sign", "no_transpose"):
+        Fw = ramp_motion_scan(g, V, stride, model_errors=(err,))[6]
+        assert max(np.abs(Fw[c] - got[c]).max() for c in range(6)) > 1e-5, err
+
+
+def test_motion_sweep_equals_driver_alpha_x():
+    """The standalone sweep is the driver's motion sweep: aumisart with op_iter [0] (overload 2)
+    gives the same alpha_x and ratios."""
+    g = Geometry.from_user(16, 16, 12, 24, 20, da=1.0, db=1.0, SO=60.0, SD=90.0)
+    V = 4
+    x, F0, proj, ang, vol, flo, _, _ = ramp_motion_scan(g, V, 1)
+    got, nrm = oracle.motion_sweep(x, proj, g, ang, vol, flo, F0)
+    _, nrm2, ax2, _ = oracle.aumisart(x, proj, g, ang, vol, flo, [0], fields=F0)
+    assert np.array_equal(got[0], ax2) and np.array_equal(nrm, nrm2)
+
+
+def test_oracle_under_address_and_ub_sanitizers(tmp_path):
+    """Every oracle function, including the warp on +-1e20 / +-Inf / NaN displacements, runs clean
+    under AddressSanitizer + UndefinedBehaviorSanitizer (float-cast-overflow included; every report
+    aborts): built by `make sanitize` from oracle/sanitize_check.cpp."""
+    import os
+    import shutil
+    import subprocess
+    if shutil.which("g++") is None:
+        pytest.skip("g++ not available")
+    here = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle")
+    subprocess.run(["make", "-s", "-C", here, "sanitize", f"OUT={tmp_path}"], check=True,
+                   timeout=300)
+    r = subprocess.run([str(tmp_path / "oracle_sanitize_check")], capture_output=True, text=True,
+                       timeout=300)
+    assert r.returncode == 0, r.stderr[-2000:]
+    assert "runtime error" not in r.stderr and "ERROR: AddressSanitizer" not in r.stderr
+    assert r.stdout.strip().endswith("SANITIZE_OK")
