@@ -192,7 +192,16 @@ def motion_update(ax, ay, az, bx, by, bz, d_u, d_img, wv, wf, nx, ny, nz):
 
 
 # ---- convenience wrappers on numpy arrays --------------------------------------------------
+def _sized(x: np.ndarray, n: int, what: str) -> np.ndarray:
+    """x as float32, refused unless it holds exactly n elements (the kernels read n of them)."""
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    if x.size != n:
+        raise ValueError(f"{what} holds {x.size} elements, {n} expected")
+    return x
+
+
 def project(img: np.ndarray, g, angle) -> np.ndarray:
+    img = _sized(img, g.nvox, "volume")
     with DeviceArray.from_host(img) as d_img, DeviceArray.empty(g.det_shape) as d_p:
         host_projection(d_p, d_img, angle, g.SO, g.SD, g.da, g.na, g.ai, g.db, g.nb, g.bi,
                         g.nx, g.ny, g.nz)
@@ -200,6 +209,7 @@ def project(img: np.ndarray, g, angle) -> np.ndarray:
 
 
 def backproject(proj: np.ndarray, g, angle) -> np.ndarray:
+    proj = _sized(proj, g.ndet, "projection")
     with DeviceArray.from_host(proj) as d_p, DeviceArray.empty(g.vol_shape) as d_img:
         host_backprojection(d_img, d_p, angle, g.SO, g.SD, g.da, g.na, g.ai, g.db, g.nb, g.bi,
                             g.nx, g.ny, g.nz)
@@ -208,7 +218,9 @@ def backproject(proj: np.ndarray, g, angle) -> np.ndarray:
 
 def deform(img: np.ndarray, volume, flow, fields: Sequence[np.ndarray]) -> np.ndarray:
     nz, ny, nx = img.shape
-    d = [DeviceArray.from_host(f) for f in fields]
+    if len(fields) != 6:
+        raise ValueError("six fields (ax, ay, az, bx, by, bz) expected")
+    d = [DeviceArray.from_host(_sized(f, img.size, "field")) for f in fields]
     with DeviceArray.from_host(img) as d_img, DeviceArray.empty(img.shape) as d_out:
         host_deform(d_out, d_img, nx, ny, nz, volume, flow, *d)
         out = d_out.to_host()
@@ -226,7 +238,7 @@ def host_multiple(img1, nx, ny, nz, img0, weight, ind):
 def invert_dvf(mx: np.ndarray, my: np.ndarray, mz: np.ndarray, niter: int):
     """Fixed-point DVF inverse on the device (mcsart_invert_dvf); returns (mx2, my2, mz2)."""
     nz, ny, nx = mx.shape
-    ins = [DeviceArray.from_host(a) for a in (mx, my, mz)]
+    ins = [DeviceArray.from_host(_sized(a, mx.size, "DVF component")) for a in (mx, my, mz)]
     outs = [DeviceArray.empty(mx.shape) for _ in range(3)]
     try:
         check(load().mcsart_invert_dvf(*[o.ptr for o in outs], *[i.ptr for i in ins], nx, ny, nz,
@@ -243,8 +255,8 @@ def generate_proj(img: np.ndarray, fields: Sequence[np.ndarray], volumes, flows,
     vol, flo, ang = (np.ascontiguousarray(x, dtype=np.float32).ravel()
                      for x in (volumes, flows, angles))
     V = int(ang.size)
-    d_f = [DeviceArray.from_host(f) for f in fields]
-    d_img = DeviceArray.from_host(img)
+    d_f = [DeviceArray.from_host(_sized(f, g.nvox, "field")) for f in fields]
+    d_img = DeviceArray.from_host(_sized(img, g.nvox, "volume"))
     d_p = DeviceArray.empty((V, g.nb, g.na))
     try:
         check(load().mcsart_generate_proj(

@@ -168,3 +168,35 @@ def test_large_job_sweeps_deterministic(gpu_lib):
     ctx.close()
     assert np.isfinite(img1).all() and np.isfinite(n1).all()
     assert np.array_equal(img1, img2) and np.array_equal(n1, n2)
+
+
+def test_warp_extreme_and_nonfinite_displacements(gpu_lib):
+    """Regression for the warp's float->int conversions: displacements of +-1e20 and +-Inf sample
+    the edge voxel on their side exactly, NaN the lower edge (the displacement is clamped to
+    +-(n+1) first, which changes no clamp-to-edge sample), as the FP64 oracle now defines; a mix
+    of an in-range fractional displacement and an extreme one matches the oracle to 1e-5."""
+    import oracle
+    from mcsart_amd.ops import deform
+    from _util import smooth_random
+    nz, ny, nx = 9, 20, 20
+    img = smooth_random((nz, ny, nx), 40)
+    zero = np.zeros((nz, ny, nx), np.float32)
+    for comp, axis in ((1, 2), (0, 1), (2, 0)):
+        n = img.shape[axis]
+        for val, edge in ((1e20, n - 1), (-1e20, 0), (np.inf, n - 1), (-np.inf, 0), (np.nan, 0)):
+            f = [zero] * 6
+            f[comp] = np.full((nz, ny, nx), val, np.float32)
+            print(f"warp: field {comp} = {val}", flush=True)
+            got = deform(img, 1.0, 0.0, f)
+            ref = np.broadcast_to(np.take(img, [edge], axis=axis), img.shape)
+            assert np.array_equal(got, ref), (comp, val)
+            assert np.array_equal(got, oracle.deform(img, 1.0, 0.0, f).astype(np.float32))
+    f = [zero] * 6
+    f[1] = np.full((nz, ny, nx), 0.25, np.float32)
+    f[2] = np.full((nz, ny, nx), -3e30, np.float32)
+    f[3] = np.broadcast_to(np.where(np.arange(nx) % 3 == 0, np.inf, 0.7),
+                           (nz, ny, nx)).astype(np.float32)
+    got = deform(img, 1.0, 0.5, f)
+    ref = oracle.deform(img, 1.0, 0.5, f)
+    assert np.isfinite(got).all()
+    assert rel_l2(got, ref) <= 1e-5
