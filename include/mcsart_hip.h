@@ -246,6 +246,25 @@ int mcsart_generate_proj_async(mcsart_ctx *ctx, float *d_proj, const float *d_im
                                int nz);
 
 /* ---------------------------------------------------------------------------------------------
+ * 2c. Allocation-free projection.  No counterpart in the reference: it is for callers that
+ *     cannot allocate device memory (code under HIP graph capture, tight serving loops,
+ *     harnesses that forbid hipMalloc).
+ * ------------------------------------------------------------------------------------------- */
+/* mcsart_projection (same arguments after `stream`, same result to FP32 rounding) without any
+ * device allocation: the xy-transposed copy of the volume that x-branch rays read goes into a
+ * fixed device array inside the library (mcsart_projection_noalloc_max_voxels() floats, 32 MiB,
+ * held by every GPU the library runs on until the process ends) instead of an allocated scratch.
+ * Asynchronous: enqueued on `stream` (a hipStream_t of the current device; NULL = the null
+ * stream); the caller synchronises.  Volumes of more voxels than the array holds are rejected
+ * with MCSART_EINVAL, whatever the angle.  Every call on one GPU shares that GPU's array, so calls
+ * that overlap on one GPU (on different streams, or from different threads) race: serialise them,
+ * e.g. on one stream. */
+int mcsart_projection_noalloc(void *stream, float *d_proj, const float *d_img, float angle,
+                              float SO, float SD, float da, int na, float ai, float db, int nb,
+                              float bi, int nx, int ny, int nz);
+size_t mcsart_projection_noalloc_max_voxels(void);
+
+/* ---------------------------------------------------------------------------------------------
  * 3. Drivers: host_AUMISART (include/AUMISART.h:17,19), host pointers, same argument order.
  * ------------------------------------------------------------------------------------------- */
 

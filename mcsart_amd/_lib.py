@@ -46,6 +46,8 @@ SIGNATURES = {
     "mcsart_invert_dvf": ([_P] * 6 + [_I, _I, _I, _I], _I),
     "mcsart_generate_proj": ([_P] * 11 + [_I, _I, _F, _F, _F, _I, _F, _F, _I, _F, _I, _I, _I], _I),
     "mcsart_sart_residual": ([_P, _P, _P, _I, _P] + [_F, _F, _F, _F, _I, _F, _F, _I, _F, _I, _I, _I], _I),
+    "mcsart_projection_noalloc": ([_P, _P, _P, _F, _F, _F, _F, _I, _F, _F, _I, _F, _I, _I, _I], _I),
+    "mcsart_projection_noalloc_max_voxels": ([], _SZ),
     "mcsart_sart_backprojection": ([_P, _P, _F, _F, _F, _F, _I, _F, _F, _I, _F, _I, _I, _I], _I),
     "mcsart_motion_prep": ([_P] * 8 + [_F, _F, _I, _I, _I], _I),
     "mcsart_motion_update": ([_P] * 8 + [_F, _F, _I, _I, _I], _I),

@@ -19,7 +19,7 @@ BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libmcsart_hip.so")
 
 SOURCES = ["capi.hip", "driver.hip", "serve.hip", "params.hip", "k_project.hip",
-           "k_backproject.hip", "k_elem.hip"]
+           "k_backproject.hip", "k_elem.hip", "noalloc.hip"]
 
 ARCH = os.environ.get("MCSART_OFFLOAD_ARCH", "gfx950")
 CXXFLAGS = [
