@@ -1,25 +1,33 @@
 #!/usr/bin/env python3
 """McSART benchmark on one MI355X.
 
-Workload: the 128^3 motion-compensated McSART scan (volume 128^3 float32 at 2 mm, detector
-256x192 at 1.5 mm, 200 views over 360 deg, 10 outer iterations of one image sweep and one motion
-sweep, lambda 1, nonnegativity clamp); `--workload clinical256` runs the clinical-size scan
-instead (256x256x128 at 1.5x1.5x2 mm, detector 512x384 at 0.78 mm, 656 views, 20 outer
-iterations).  One step is one complete reconstruction of that scan,
-with the scan's measured projections, initial image and 5DCT fields resident in HBM before the
-timed region starts.  Synthetic data: seeded ellipsoid phantom, breathing trace and Gaussian-blob
-v/f fields; the measured projections are made on the GPU with this library's warp and projector
-and Poisson noise at 1e5 photons (the FP64 CPU path is far too slow to synthesise them here).
+Workload (default): the clinical-size motion-compensated McSART scan, the largest single-scan
+workload: volume 256x256x128 float32 at 1.5x1.5x2 mm, detector 512x384 at 0.78 mm, 656 views over
+360 deg, 20 outer iterations of one image sweep and one motion sweep, lambda 1, nonnegativity
+clamp; `--workload mcsart128` runs the 128^3 scan (2 mm, 256x192 detector, 200 views, 10 outer
+iterations).  One step is one complete reconstruction of the scan, with its measured
+projections, initial image and 5DCT fields resident in HBM before the timed region starts.
+Synthetic data: seeded ellipsoid phantom, breathing trace and Gaussian-blob v/f fields; the
+measured projections are made on the GPU with this library's warp and projector and Poisson noise
+at 1e5 photons.
 
-Output: one JSON line with scans/hour (value), per-scan latency p50/p99, the roofline figures of
-every kernel of the sweeps and of the dominant one (the largest share of a scan; HIP events on
-the stream the kernels run on), the whole-step figure (the unfused loop's algorithmic bytes over
-the scan time) and the CPU baseline (the FP64 OpenMP oracle timed on a bounded sample of the same
-scan on this host).
+Output: one JSON line with scans/hour (value), back-to-back scan wall time p50/p99, the roofline
+figures of every kernel of the sweeps and of the dominant one (HIP events on the stream the
+kernels run on; counter figures from profiles/<workload>_kernels.json), the 16-scan uneven batch
+through the server (plain one slot against several) and a served Poisson stream with p50/p99
+latency, and the CPU baseline (the FP64 OpenMP oracle on a bounded sample of the same scan,
+1 warm-up + median of 5).  `--cpu-protocol` runs the CPU baseline's full protocol instead.
 """
+import os
+
+# OpenMP placement of the CPU baseline (read by the oracle's OpenMP runtime when it loads; it
+# then binds this thread to one core, so the process's CPU set is read first)
+os.environ.setdefault("OMP_PROC_BIND", "close")
+os.environ.setdefault("OMP_PLACES", "cores")
+HOST_CPUS = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+
 import argparse
 import json
-import os
 import re
 import sys
 import time
@@ -177,22 +185,77 @@ def launches_per_scan(spec):
     return n_img * spec.n_views, n_mot * mot_views
 
 
-def cpu_baseline(spec, data, proj, sample_views):
-    """FP64 oracle (OpenMP) on the first `sample_views` views of the same scan: one image sweep
-    and one motion sweep, scaled to a whole scan."""
+def cpu_env(oracle_threads):
+    """The host the CPU baseline ran on and its OpenMP settings."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"nproc": os.cpu_count(), "affinity_cpus": HOST_CPUS, "cpu_model": model,
+            "omp_threads": oracle_threads,
+            "OMP_NUM_THREADS": os.environ.get("OMP_NUM_THREADS"),
+            "OMP_PROC_BIND": os.environ.get("OMP_PROC_BIND"),
+            "OMP_PLACES": os.environ.get("OMP_PLACES")}
+
+
+def median_time(fn, reps=5):
+    """One warm-up call, then the median (and all) of `reps` timed calls (steady clock)."""
+    fn()
+    ts = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        fn()
+        ts.append(time.perf_counter() - t0)
+    return float(np.median(ts)), ts
+
+
+def cpu_operator_times(spec, data, reps=5):
+    """Per-operator seconds at the scan's size (FP64 oracle): projection, backprojection, warp
+    (DVF formed on the fly + trilinear), update_udvf; 1 warm-up then the median of `reps`."""
     import oracle
     g = spec.geometry
-    V = sample_views
-    ang = spec.angles[:V]
-    al, be = data.alpha[:V], data.beta[:V]
-    p = proj[:V]
+    ph = data.phantom
+    ang = spec.angles[min(1, spec.n_views - 1)]
+    p = oracle.projection(ph, g, ang)
+    fields = data.fields or [np.zeros(g.vol_shape, np.float32)] * 6
+    out = {}
+    out["projection"] = median_time(lambda: oracle.projection(ph, g, ang), reps)[0]
+    out["backprojection"] = median_time(lambda: oracle.backprojection(p, g, ang), reps)[0]
+    out["warp"] = median_time(lambda: oracle.deform(ph, 0.7, -0.3, fields), reps)[0]
+    out["update_udvf"] = median_time(
+        lambda: oracle.update_udvf(fields, ph, 0.9 * ph, 0.05, -0.02), reps)[0]
+    return {k: round(v, 4) for k, v in out.items()}
+
+
+def cpu_view_times(spec, data, proj, reps=5, views=2):
+    """Seconds per view of an image sweep and of a motion sweep (FP64 oracle driver) over the
+    first `views` (image) / `views` updated (motion) views of the scan: 1 warm-up, median of
+    `reps`."""
+    import oracle
+    g = spec.geometry
     zero = np.zeros(g.vol_shape)
-    t0 = time.perf_counter()
-    oracle.aumisart(zero, p, g, ang, al, be, [1], fields=data.fields, nonneg=True)
-    t_img = (time.perf_counter() - t0) / V
-    t0 = time.perf_counter()
-    oracle.aumisart(zero, p, g, ang, al, be, [0], fields=data.fields, nonneg=True)
-    t_mot = (time.perf_counter() - t0) / (V - 1)
+    V = views + 1
+    ang, al, be, p = spec.angles[:V], data.alpha[:V], data.beta[:V], proj[:V]
+    t_img = median_time(lambda: oracle.aumisart(zero, p[:views], g, ang[:views], al[:views],
+                                                be[:views], [1], fields=data.fields,
+                                                nonneg=True), reps)[0] / views
+    t_mot = median_time(lambda: oracle.motion_sweep(data.phantom, p, g, ang, al, be,
+                                                    data.fields), reps)[0] / views
+    return t_img, t_mot
+
+
+def cpu_baseline(spec, data, proj, reps=5):
+    """FP64 oracle (OpenMP, all host cores) on a bounded sample of the same scan, timed by the
+    protocol 1 warm-up + median of `reps`: per-operator times, then the seconds per image-sweep
+    view and per motion-sweep view over two views, scaled to the whole scan's views."""
+    import oracle
+    ops = cpu_operator_times(spec, data, reps)
+    t_img, t_mot = cpu_view_times(spec, data, proj, reps)
     n_img_views, n_mot_views = launches_per_scan(spec)
     sec_per_scan = n_img_views * t_img + n_mot_views * t_mot
     return {
@@ -200,24 +263,177 @@ def cpu_baseline(spec, data, proj, sample_views):
         "unit": "scans/hour",
         "cores": oracle.num_threads(),
         "kind": "port",
-        "sample": (f"FP64 oracle, {spec.name} geometry, first {V} views: one image sweep "
-                   f"({t_img:.3f} s/view) and one motion sweep ({t_mot:.3f} s/view), scaled to "
-                   f"{n_img_views} image-sweep + {n_mot_views} motion-sweep views per scan"),
+        "sample": (f"FP64 oracle, {spec.name} scan: 1 warm-up + median of {reps}; two views of an "
+                   f"image sweep ({t_img:.3f} s/view) and two of a motion sweep "
+                   f"({t_mot:.3f} s/view), extrapolated to {n_img_views} image-sweep + "
+                   f"{n_mot_views} motion-sweep views per scan"),
+        "seconds_per_view": {"image_sweep": round(t_img, 4), "motion_sweep": round(t_mot, 4)},
+        "operator_seconds": ops,
+        "host": cpu_env(oracle.num_threads()),
     }
+
+
+def cpu_protocol(reps=5):
+    """The whole CPU-baseline protocol: per-operator times at the 64^3, 128^3 and 256x256x128
+    scans, the 64^3 static reconstruction (5 SART iterations, 60 views; median of `reps` after a
+    warm-up) and the 128^3 motion-compensated reconstruction (10 outer iterations, 200 views;
+    one run) in full, the 256x256x128 reconstruction extrapolated from two views of each sweep."""
+    import oracle
+    from mcsart_amd import configs
+    from mcsart_amd.configs import make_scan_inputs
+    out = {"host": cpu_env(oracle.num_threads()), "reps": reps, "operators_s": {}}
+    specs = {"static64": configs.c1_static64(), "mcsart128": configs.c2_mcsart128(),
+             "clinical256": configs.c3_clinical()}
+    for name, spec in specs.items():
+        log(f"[bench] CPU protocol: operators at {name}")
+        out["operators_s"][name] = cpu_operator_times(spec, make_scan_inputs(spec), reps)
+    # 64^3 static: data from the oracle projector, the whole reconstruction
+    spec = specs["static64"]
+    g = spec.geometry
+    d = make_scan_inputs(spec)
+    proj = np.stack([oracle.projection(d.phantom, g, a) for a in spec.angles])
+    zero = np.zeros(g.vol_shape)
+    z = np.zeros(spec.n_views)
+    log("[bench] CPU protocol: 64^3 static reconstruction")
+    t, ts = median_time(lambda: oracle.aumisart(zero, proj, g, spec.angles, z, z, spec.op_iter,
+                                                nonneg=True), reps)
+    out["static64_recon_s"] = {"median": round(t, 3), "runs": [round(x, 3) for x in ts]}
+    # 128^3 motion-compensated: one run of the full schedule
+    spec = specs["mcsart128"]
+    g = spec.geometry
+    d = make_scan_inputs(spec)
+    log("[bench] CPU protocol: 128^3 projections (FP64 oracle, warped phantom)")
+    proj = np.stack([oracle.projection(oracle.deform(d.phantom, d.alpha[i], d.beta[i], d.fields),
+                                       g, spec.angles[i]) for i in range(spec.n_views)])
+    log("[bench] CPU protocol: 128^3 reconstruction, 10 outer iterations (one run)")
+    t0 = time.perf_counter()
+    oracle.aumisart(np.zeros(g.vol_shape), proj, g, spec.angles, d.alpha, d.beta, spec.op_iter,
+                    fields=d.fields, nonneg=True)
+    out["mcsart128_recon_s"] = round(time.perf_counter() - t0, 2)
+    # 256x256x128: extrapolated
+    spec = specs["clinical256"]
+    g = spec.geometry
+    d = make_scan_inputs(spec)
+    proj = np.stack([oracle.projection(d.phantom, g, spec.angles[i]) for i in range(3)])
+    log("[bench] CPU protocol: 256x256x128 per-view sweep times")
+    t_img, t_mot = cpu_view_times(spec, d, proj, reps)
+    n_img, n_mot = launches_per_scan(spec)
+    out["clinical256_recon_s_extrapolated"] = {
+        "value": round(n_img * t_img + n_mot * t_mot, 1), "image_view_s": round(t_img, 4),
+        "motion_view_s": round(t_mot, 4),
+        "note": f"{n_img} image-sweep + {n_mot} motion-sweep views at the sampled per-view times"}
+    return out
+
+
+def batch_specs(n_jobs, outer):
+    """The 16-scan uneven batch (volumes 128^3 .. 320x320x160, 200-900 views, detectors
+    256x192 .. 512x384) with `outer` outer iterations, noise-free data (the kernels' timing does
+    not depend on the noise)."""
+    from mcsart_amd import configs
+    specs = configs.c4_batch(n_jobs, n_outer=outer)
+    for s in specs:
+        s.noise_photons = None
+    return specs
+
+
+def make_inputs(specs):
+    from mcsart_amd.configs import make_scan_inputs
+    from mcsart_amd.simulate import measure
+    out = []
+    for k, s in enumerate(specs):
+        d = make_scan_inputs(s)
+        out.append((s, d, measure(s, d, s.seed + 404)))
+        log(f"[bench] inputs {k + 1}/{len(specs)}: {s.name} {s.nx}x{s.ny}x{s.nz} det "
+            f"{s.na}x{s.nb} V={s.n_views}")
+    return out
+
+
+def _submit(srv, sc):
+    s, d, p = sc
+    return srv.submit(s.geometry, s.angles, d.alpha, d.beta, s.op_iter, p, fields=d.fields,
+                      lam=s.lam, motion=s.motion, nonneg=s.nonneg)
+
+
+def _server_stats(infos, span_s):
+    tot = np.array([i["total_ms"] for i in infos])
+    return {"scans": len(infos), "makespan_s": round(span_s, 3),
+            "scans_per_hour": round(3600.0 * len(infos) / span_s, 1),
+            "latency_ms": {"p50": round(float(np.percentile(tot, 50)), 1),
+                           "p99": round(float(np.percentile(tot, 99)), 1)}}
+
+
+def batch_leg(inputs, slots):
+    """All scans submitted at t = 0 to the server with 1 slot (plain one-at-a-time processing)
+    and with `slots` slots; scans/hour of each, their ratio, and the largest difference between
+    the two runs' images."""
+    from mcsart_amd.serve import Server
+    runs, imgs = {}, {}
+    for n in (1, slots):
+        with Server(slots=n) as srv:
+            t0 = time.perf_counter()
+            tickets = [_submit(srv, sc) for sc in inputs]
+            srv.drain()
+            span = time.perf_counter() - t0
+            infos = [srv.info(t) for t in tickets]
+            assert all(i["state"] == "done" for i in infos), infos
+            imgs[n] = [t.image for t in tickets]
+        runs[n] = _server_stats(infos, span)
+        log(f"[bench] batch, {n} slot(s): {runs[n]}")
+    diff = max(float(np.abs(a - b).max()) for a, b in zip(imgs[1], imgs[slots]))
+    return {"plain": runs[1], "batched": dict(runs[slots], slots=slots),
+            "scans_per_hour_ratio": round(runs[slots]["scans_per_hour"] /
+                                          runs[1]["scans_per_hour"], 3),
+            "max_abs_image_difference": diff}
+
+
+def served_leg(inputs, rate, slots, seed):
+    """A Poisson arrival stream (`rate` per second) of the given scans, each submitted at its
+    arrival time, to the server with 1 slot (plain) and with `slots` slots: p50/p99 time from
+    submission to the reconstruction in the request's buffers, and scans/hour."""
+    from mcsart_amd.serve import Server
+    rng = np.random.default_rng(seed)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, size=len(inputs)))
+    out = {"rate_per_s": rate, "requests": len(inputs)}
+    for label, n in (("plain", 1), ("batched", slots)):
+        with Server(slots=n) as srv:
+            t0 = time.perf_counter()
+            tickets = []
+            for a, sc in zip(arrivals, inputs):
+                dt = a - (time.perf_counter() - t0)
+                if dt > 0:
+                    time.sleep(dt)
+                tickets.append(_submit(srv, sc))
+            srv.drain()
+            span = time.perf_counter() - t0
+            infos = [srv.info(t) for t in tickets]
+            assert all(i["state"] == "done" for i in infos), infos
+        out[label] = dict(_server_stats(infos, span), slots=n)
+        log(f"[bench] served at {rate}/s, {n} slot(s): {out[label]}")
+    out["scans_per_hour_ratio"] = round(out["batched"]["scans_per_hour"] /
+                                        out["plain"]["scans_per_hour"], 3)
+    return out
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--cpu-sample-views", type=int, default=12)
+    ap.add_argument("--workload", default="clinical256", choices=["mcsart128", "clinical256"],
+                    help="clinical256: 256x256x128, 656 views, 20 outer iterations (the headline "
+                         "single-scan workload); mcsart128: 128^3, 200 views, 10 outer iterations")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--concurrent", type=int, default=4,
-                    help="also measure this many scans in flight at once (0: skip)")
-    ap.add_argument("--workload", default="mcsart128", choices=["mcsart128", "clinical256"],
-                    help="mcsart128: 128^3, 200 views, 10 outer iterations (the headline scan); "
-                         "clinical256: 256x256x128, 656 views, 20 outer iterations")
+    ap.add_argument("--cpu-protocol", action="store_true",
+                    help="run only the full CPU-baseline protocol (per-operator times at 64^3, "
+                         "128^3 and 256x256x128, whole reconstructions) and print its JSON")
+    ap.add_argument("--batch-outer", type=int, default=2,
+                    help="outer iterations of the 16-scan uneven batch leg (0: skip the batch "
+                         "and served legs)")
+    ap.add_argument("--batch-slots", type=int, default=4)
+    ap.add_argument("--serve-rate", type=float, default=1.0,
+                    help="arrivals per second of the served leg (0: skip it)")
+    ap.add_argument("--concurrent", type=int, default=0,
+                    help="also time this many copies of the scan in flight at once (0: skip)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -225,6 +441,9 @@ def main():
               "(scans are independent: more GPUs would run more scans side by side)",
               file=sys.stderr)
         sys.exit(2)
+    if args.cpu_protocol:
+        print(json.dumps(cpu_protocol()), flush=True)
+        return
 
     from mcsart_amd import configs
     from mcsart_amd.configs import make_scan_inputs
@@ -237,7 +456,7 @@ def main():
         f"op_iter={spec.op_iter}")
     data = make_scan_inputs(spec)
     ctx = Context(0)
-    proj = measure(spec, data, spec.seed + 404)
+    proj = measure(spec, data, spec.seed + 404, log=log)
     log("[bench] measured projections ready")
 
     job = Job(ctx, g, spec.angles, data.alpha, data.beta, spec.op_iter, lam=spec.lam,
@@ -250,7 +469,7 @@ def main():
     tuning = job.autotune()
     log(f"[bench] autotuned launch shapes: {tuning}")
 
-    # Pass 1 (the measurement): each sweep replays its captured HIP graph; the warm-up captures.
+    # The measurement: each sweep replays its captured HIP graph; the warm-up captures them.
     for w in range(max(args.warmup, 1)):
         job.reset()
         job.run()
@@ -266,26 +485,23 @@ def main():
         lat.append(time.perf_counter() - ts)
         log(f"[bench] step {s + 1}/{args.steps}: {lat[-1] * 1e3:.1f} ms")
     ctx.synchronize()
-    t1 = time.perf_counter()
-    elapsed = t1 - t0
-
+    elapsed = time.perf_counter() - t0
+    K = args.steps
     img = job.image()
     finite = float(np.isfinite(img).mean())
 
-    # Pass 2 (kernel roofline): the same K scans with HIP events recorded on the job's stream
-    # around every launch (events cannot be timed inside a graph, so these launches are eager).
+    # One scan launched kernel by kernel with HIP events on the job's stream around every
+    # launch (events cannot be timed inside a graph): per-kernel-kind totals.
     ctx.enable_timing(True)
     ctx.reset_timing()
     t2 = time.perf_counter()
-    for s in range(args.steps):
-        job.reset()
-        job.run()
-        ctx.synchronize()
-    eager_ms = (time.perf_counter() - t2) * 1e3 / args.steps
-    log(f"[bench] timed eager pass: {eager_ms:.1f} ms per scan")
+    job.reset()
+    job.run()
+    ctx.synchronize()
+    eager_ms = (time.perf_counter() - t2) * 1e3
+    log(f"[bench] timed eager scan: {eager_ms:.1f} ms")
     times = {k: ctx.kernel_time(k) for k in ("projector", "backprojector", "warp", "motion")}
     ctx.enable_timing(False)
-    K = args.steps
     # Rooflines: every view of the scan launched back to back between two HIP events on the
     # job's stream (no per-launch event overhead), for each kernel of the sweeps.  The dominant
     # kernel is the one with the largest share of a scan (average launch x launches per scan).
@@ -310,50 +526,74 @@ def main():
             "launches_per_scan": per_scan[k],
             "ms_per_scan": round(us * per_scan[k] * 1e-3, 2),
             "traffic": profiled(spec.name, k),
-            "valu_busy": profiled(spec.name, k, "valu_busy")}
+            "valu_busy": profiled(spec.name, k, "valu_busy"),
+            "l2_hit_rate": profiled(spec.name, k, "l2_hit_rate")}
     dom = max(kernels_roofline, key=lambda k: kernels_roofline[k]["ms_per_scan"])
     d = kernels_roofline[dom]
     traffic = d["traffic"]
+    src = f"profiles/{spec.name}_kernels.json"
     roofline = {
         "bound": "hbm", "kernel": dom, "achieved": d["achieved"], "peak": HBM_PEAK_GBS,
         "unit": "GB/s", "frac": d["frac"], "traffic": traffic,
         "avg_launch_us": d["avg_launch_us"], "bytes_per_launch": d["bytes_per_launch"],
         "timing": "HIP events around all views launched back to back, 3 repetitions",
-        "traffic_source": (f"profiles/{spec.name}_kernels.json (rocprofv3 FETCH_SIZE + "
-                           "WRITE_SIZE, calibrated on a 64 MiB copy)") if traffic else None,
-        "valu_busy": d["valu_busy"],
-        "valu_busy_source": (f"profiles/{spec.name}_kernels.json (rocprofv3 SQ_INSTS_VALU, "
-                             "SQ_BUSY_CYCLES: share of SIMD cycles issuing vector "
-                             "instructions)") if d["valu_busy"] is not None else None,
+        "traffic_source": (f"{src} (rocprofv3 FETCH_SIZE + WRITE_SIZE, calibrated on a 64 MiB "
+                           "copy)") if traffic else None,
+        "valu_busy": d["valu_busy"], "l2_hit_rate": d["l2_hit_rate"],
+        "counter_source": f"{src} (rocprofv3 SQ and TCC counters)" if d["valu_busy"] else None,
     }
     kernel_ms = {k: {"total_ms": round(v[0], 2), "launches": v[1],
                      "avg_us": round(v[0] * 1e3 / max(v[1], 1), 2)} for k, v in times.items()}
+    job.close()
 
     # Several copies of the scan in flight at once, each a resident job on its own stream.
-    conc = concurrent_scans(ctx, spec, g, data, proj, tuning, args.concurrent, K)
-    if conc:
+    conc = None
+    if args.concurrent > 1:
+        conc = concurrent_scans(ctx, spec, g, data, proj, tuning, args.concurrent, 1)
         log(f"[bench] {args.concurrent} scans in flight: {conc}")
 
     # The drop-in boundary (host_AUMISART: host buffers in and out, device memory allocated and
-    # freed per call): one whole scan through it, PCIe transfers included.  Not the headline value.
+    # freed per call): one whole scan through it, PCIe transfers included.  Not the headline.
     host_ms = host_boundary_ms(spec, g, data, proj)
     log(f"[bench] host-buffer driver call: {host_ms:.1f} ms per scan")
+
+    # The 16-scan uneven batch through the server (plain = one slot) and a served Poisson stream
+    # of the same scans plus one large scan (512x512x256, 900 views, 1024x768 detector).
+    batch = served = None
+    if args.batch_outer > 0:
+        bspecs = batch_specs(16, args.batch_outer)
+        log(f"[bench] batch leg: making inputs of {len(bspecs)} scans")
+        binputs = make_inputs(bspecs)
+        batch = batch_leg(binputs, args.batch_slots)
+        batch["mix"] = (f"16 seeded scans, volumes 128^3..320x320x160, 200-900 views, detectors "
+                        f"256x192..512x384, [1,0]x{args.batch_outer}, noise-free data, all "
+                        f"submitted at t = 0")
+        if args.serve_rate > 0:
+            large = configs.scaled(configs.c5_large(), op_iter=[1, 0] * args.batch_outer)
+            large.noise_photons = None
+            log("[bench] served leg: making the large scan's inputs")
+            linputs = make_inputs([large])
+            stream = binputs[:8] + linputs + binputs[8:]
+            served = served_leg(stream, args.serve_rate, args.batch_slots, 20261203 + 5099)
+            served["stream"] = (f"the 16 batch scans with one 512x512x256 scan (900 views, "
+                                f"1024x768 detector) after the eighth, [1,0]x{args.batch_outer}, "
+                                f"Poisson arrivals; latency = submission to the full schedule's "
+                                f"reconstruction in the request's buffers")
 
     cpu = None
     if not args.no_cpu_baseline:
         log("[bench] timing the FP64 CPU oracle on a bounded sample ...")
-        cpu = cpu_baseline(spec, data, proj, args.cpu_sample_views)
+        cpu = cpu_baseline(spec, data, proj)
+        log(f"[bench] CPU baseline: {cpu['value']:.3f} scans/hour")
 
     vol_name = f"{g.nx}^3" if g.nx == g.ny == g.nz else f"{g.nx}x{g.ny}x{g.nz}"
     lat_ms = np.array(lat) * 1e3
     value = 3600.0 * K / elapsed
-    # Whole-step figure: the unfused loop's algorithmic bytes per scan over the measured scan time
+    # The reference loop's unfused algorithmic bytes per scan over the measured scan time.  The
+    # fused kernels move fewer bytes than that, so this is an equivalent bandwidth, not a
+    # fraction of the HBM peak.
     img_b, mot_b = step_bytes(spec)
     scan_bytes = n_img_views * img_b + n_mot_views * mot_b
-    step_ach = scan_bytes / (elapsed / K) / 1e9
-    step_roofline = {"image_view_bytes": int(img_b), "motion_view_bytes": int(mot_b),
-                     "bytes_per_scan": int(scan_bytes), "achieved": round(step_ach, 1),
-                     "unit": "GB/s", "frac": round(step_ach / HBM_PEAK_GBS, 4)}
     out = {
         "metric": (f"McSART scans/hour ({vol_name} motion-compensated scan, {spec.n_views} views, "
                    f"{len(spec.op_iter) // 2} outer iterations)"),
@@ -370,26 +610,30 @@ def main():
         "data": "synthetic (seeded phantom, trace, fields; projections made on GPU + Poisson noise)",
         "config": {"workload": spec.name, "volume": [g.nx, g.ny, g.nz], "detector": [g.na, g.nb],
                    "views": spec.n_views, "op_iter": f"[1,0]x{len(spec.op_iter) // 2}",
-                   "lambda": spec.lam,
-                   "nonneg": spec.nonneg, "jobs_in_flight": 1},
-        "latency_ms": {"p50": round(float(np.percentile(lat_ms, 50)), 2),
-                       "p99": round(float(np.percentile(lat_ms, 99)), 2)},
-        "kernels": kernel_ms,
-        "kernels_note": "eager pass with HIP events around every launch (includes event overhead)",
-        "eager_ms_per_scan_with_events": round(eager_ms, 2),
-        "isolated_launch_us": {k: round(v, 2) for k, v in iso_us.items()},
+                   "lambda": spec.lam, "nonneg": spec.nonneg, "jobs_in_flight": 1},
+        "scan_wall_ms": {"p50": round(float(np.percentile(lat_ms, 50)), 2),
+                         "p99": round(float(np.percentile(lat_ms, 99)), 2),
+                         "note": "back-to-back single scans, inputs resident"},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "batch": batch,
+        "served": served,
         "kernels_roofline": kernels_roofline,
-        "step_roofline": step_roofline,
+        "kernels_eager_scan": kernel_ms,
+        "eager_ms_per_scan_with_events": round(eager_ms, 2),
+        "step_equivalent_bandwidth": {
+            "image_view_bytes": int(img_b), "motion_view_bytes": int(mot_b),
+            "bytes_per_scan": int(scan_bytes),
+            "GBps": round(scan_bytes / (elapsed / K) / 1e9, 1),
+            "note": "the reference loop's unfused algorithmic bytes (image view 84N+28D, motion "
+                    "view 128N+44D) over the scan time; not a fraction of peak"},
         "host_boundary_ms_per_scan": round(host_ms, 1),
         "concurrent": conc,
         "launch": "one HIP graph per sweep",
         "tuning": tuning,
-        "roofline": roofline,
-        "cpu_baseline": cpu,
         "output_finite_fraction": finite,
     }
     print(json.dumps(out), flush=True)
-    job.close()
     ctx.close()
 
 

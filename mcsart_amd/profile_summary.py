@@ -104,7 +104,22 @@ def valu_busy(c):
     return VALU_CYCLES * c["SQ_INSTS_VALU"] / (SIMDS * c["SQ_BUSY_CYCLES"] / SQ_INSTANCES)
 
 
-def summarise(workload, trace, fetch, write, copy_mb=64, views=0, sq=None):
+def load_tcc(d):
+    """Per kernel, the L2 (TCC) hits and misses per dispatch from a --pmc run with TCC_HIT_sum and
+    TCC_MISS_sum."""
+    per = collections.defaultdict(lambda: collections.defaultdict(list))
+    for r in csv.DictReader(open(os.path.join(d, "run_counter_collection.csv"))):
+        per[short(r["Kernel_Name"])][r["Counter_Name"]].append(float(r["Counter_Value"]))
+    out = {}
+    for k, cs in per.items():
+        hit = cs.get("TCC_HIT_sum") or cs.get("TCC_HIT")
+        miss = cs.get("TCC_MISS_sum") or cs.get("TCC_MISS")
+        if hit and miss:
+            out[k] = (sum(hit) / len(hit), sum(miss) / len(miss))
+    return out
+
+
+def summarise(workload, trace, fetch, write, copy_mb=64, views=0, sq=None, tcc=None):
     spec = WORKLOADS[workload]()
     g = spec.geometry
     N, D = g.nvox, g.ndet
@@ -112,6 +127,7 @@ def summarise(workload, trace, fetch, write, copy_mb=64, views=0, sq=None):
     fe, fcal = load_pmc(fetch) if fetch else ({}, None)
     wr, wcal = load_pmc(write) if write else ({}, None)
     sqc = load_sq(sq) if sq else {}
+    tc = load_tcc(tcc) if tcc else {}
     true_bytes = copy_mb << 20
     ffac = true_bytes / fcal if fcal else None
     wfac = true_bytes / wcal if wcal else None
@@ -136,6 +152,10 @@ def summarise(workload, trace, fetch, write, copy_mb=64, views=0, sq=None):
             waves = sqc[k].get("SQ_WAVES")
             if waves and "SQ_INSTS_VALU" in sqc[k]:
                 row["valu_per_wave"] = round(sqc[k]["SQ_INSTS_VALU"] / waves, 1)
+        if k in tc:
+            hit, miss = tc[k]
+            row["l2_hits"], row["l2_misses"] = int(hit), int(miss)
+            row["l2_hit_rate"] = round(hit / max(hit + miss, 1.0), 3)
         if "fetch_bytes" in row and "write_bytes" in row:
             row["traffic_bytes"] = row["fetch_bytes"] + row["write_bytes"]
             if alg:
@@ -154,17 +174,18 @@ def to_markdown(s, command_lines):
             f"{s['fetch_factor']}, WRITE_SIZE x {s['write_factor']}.  VALU busy = "
             "2 x SQ_INSTS_VALU / (1024 SIMDs x SQ_BUSY_CYCLES / 32): the share of the SIMDs' "
             "cycles spent issuing vector instructions (2 cycles per wave64 instruction: with 4, "
-            "some kernels would exceed 1).", "",
+            "some kernels would exceed 1).  L2 hit rate = TCC_HIT_sum / (TCC_HIT_sum + "
+            "TCC_MISS_sum) per dispatch.", "",
             "| kernel | launches | avg us | algorithmic MB | achieved GB/s | frac of 8 TB/s | "
-            "fetch MB | write MB | traffic / algorithmic | VALU busy |",
-            "|---|---:|---:|---:|---:|---:|---:|---:|---:|---:|"]
+            "fetch MB | write MB | traffic / algorithmic | VALU busy | L2 hit rate |",
+            "|---|---:|---:|---:|---:|---:|---:|---:|---:|---:|---:|"]
     for r in s["kernels"]:
         mb = lambda b: "" if b is None else f"{b / 1e6:.2f}"
         out.append(f"| `{r['kernel']}` | {r['launches']} | {r['avg_us']} | "
                    f"{mb(r.get('algorithmic_bytes'))} | {r.get('achieved_gbs', '')} | "
                    f"{r.get('frac_hbm_peak', '')} | {mb(r.get('fetch_bytes'))} | "
                    f"{mb(r.get('write_bytes'))} | {r.get('traffic_over_algorithmic', '')} | "
-                   f"{r.get('valu_busy', '')} |")
+                   f"{r.get('valu_busy', '')} | {r.get('l2_hit_rate', '')} |")
     return "\n".join(out) + "\n"
 
 
@@ -225,6 +246,7 @@ def main(argv=None):
     ap.add_argument("--fetch")
     ap.add_argument("--write")
     ap.add_argument("--sq", help="rocprofv3 --pmc run with SQ_INSTS_VALU, SQ_BUSY_CYCLES, SQ_WAVES")
+    ap.add_argument("--tcc", help="rocprofv3 --pmc run with TCC_HIT_sum, TCC_MISS_sum")
     ap.add_argument("--views", type=int, default=0)
     ap.add_argument("--md")
     ap.add_argument("--json")
@@ -249,7 +271,8 @@ def main(argv=None):
         return
     if not args.workload or not args.trace:
         ap.error("--workload and --trace are required without --stats")
-    s = summarise(args.workload, args.trace, args.fetch, args.write, views=args.views, sq=args.sq)
+    s = summarise(args.workload, args.trace, args.fetch, args.write, views=args.views, sq=args.sq,
+                  tcc=args.tcc)
     if args.json:
         with open(args.json, "w") as f:
             json.dump(s, f, indent=1)

@@ -14,6 +14,7 @@ Conventions (the reference's, ``src/cu_deform.cu:74-89``, ``src/AUMISART.cu:160-
 from __future__ import annotations
 
 import math
+import os
 from dataclasses import dataclass
 
 import numpy as np
@@ -117,8 +118,16 @@ def ellipsoid_phantom(nx, ny, nz, vox: Voxel, seed: int, n_ellipsoids: int = 10)
         qx = ((xs - c[0]) / a[0]) ** 2
         qy = ((ys - c[1]) / a[1]) ** 2
         qz = ((zs - c[2]) / a[2]) ** 2
-        inside = (qz[:, None, None] + qy[None, :, None] + qx[None, None, :]) <= 1.0
-        img += mu * inside
+        # every term is >= 0, so only voxels with each term <= 1 can be inside: evaluate the
+        # ellipsoid's bounding box (the same voxels and sums as over the whole grid)
+        bx, by, bz = (np.nonzero(q <= 1.0)[0] for q in (qx, qy, qz))
+        if bx.size == 0 or by.size == 0 or bz.size == 0:
+            continue
+        sx = slice(bx[0], bx[-1] + 1)
+        sy = slice(by[0], by[-1] + 1)
+        sz = slice(bz[0], bz[-1] + 1)
+        inside = (qz[sz, None, None] + qy[None, sy, None] + qx[None, None, sx]) <= 1.0
+        img[sz, sy, sx] += mu * inside
     return img.astype(np.float32)
 
 
@@ -144,8 +153,26 @@ def _blob_field(nx, ny, nz, vox: Voxel, rng: SplitMix64, n_blobs=8):
         gx = np.exp(-0.5 * ((xs - c[0]) / sigma) ** 2)
         gy = np.exp(-0.5 * ((ys - c[1]) / sigma) ** 2)
         gz = np.exp(-0.5 * ((zs - c[2]) / sigma) ** 2)
-        out += amp * gz[:, None, None] * gy[None, :, None] * gx[None, None, :]
+        gzy = amp * gz[:, None] * gy[None, :]   # (amp gz) gy, then times gx per voxel
+
+        def add_slab(z0, gzy=gzy, gx=gx):   # elementwise: slabs in threads, same sums
+            out[z0:z0 + 8] += gzy[z0:z0 + 8, :, None] * gx[None, None, :]
+
+        list(_pool().map(add_slab, range(0, nz, 8)))
     return out
+
+
+_POOL = None
+
+
+def _pool():
+    """Worker threads for elementwise synthesis over volume slabs (numpy releases the GIL)."""
+    global _POOL
+    if _POOL is None:
+        from concurrent.futures import ThreadPoolExecutor
+        n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+        _POOL = ThreadPoolExecutor(max_workers=max(1, min(16, n or 1)))
+    return _POOL
 
 
 def motion_fields(nx, ny, nz, vox: Voxel, seed: int, si_mm=12.0, ap_mm=4.0, scale=1.0):
@@ -180,7 +207,7 @@ def poisson_inversion(lam: np.ndarray, u: np.ndarray) -> np.ndarray:
 
 
 def poisson_noise(proj: np.ndarray, seed: int, photons=1e5, dx_mm=1.0, log=None,
-                  block=1 << 24) -> np.ndarray:
+                  block=1 << 21) -> np.ndarray:
     """counts ~ Poisson(I0 exp(-p dx)), p~ = -ln(max(counts, 1)/I0)/dx.  The Poisson draw uses the
     normal approximation N(lam, lam) rounded to an integer (lam >= ~600 for the configs' data);
     exact inversion is used where lam < 50.  Rays are drawn in blocks of `block` (the same
@@ -191,11 +218,19 @@ def poisson_noise(proj: np.ndarray, seed: int, photons=1e5, dx_mm=1.0, log=None,
     flat = lam.ravel()
     n = flat.size
     counts = np.empty(n, dtype=np.float64)
-    for s in range(0, n, block):
+
+    def draw(s):   # a block's normals depend on its counter range only: blocks run in threads
         e = min(n, s + block)
         counts[s:e] = np.rint(flat[s:e] + np.sqrt(flat[s:e]) * rng.normal_range(n, s, e))
-        if log and e < n:
-            log(f"[poisson_noise] {e}/{n} rays")
+        return e
+
+    from concurrent.futures import ThreadPoolExecutor
+    workers = max(1, min(16, len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity")
+                         else (os.cpu_count() or 1)))
+    with ThreadPoolExecutor(max_workers=workers) as ex:
+        for e in ex.map(draw, range(0, n, block)):
+            if log and e < n:
+                log(f"[poisson_noise] {e}/{n} rays")
     rng.skip_normal(n)
     small = np.nonzero(flat < 50.0)[0]
     if small.size:
