@@ -59,6 +59,7 @@ def build(verbose: bool = False) -> str:
         raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
     _build_cli()
+    _build_compat_check()
     if verbose:
         print(f"built {LIB} and {CLI}")
     return LIB
@@ -79,6 +80,30 @@ def _build_cli() -> None:
     if r.returncode != 0:
         raise RuntimeError(f"build failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
     os.replace(CLI + ".tmp", CLI)
+
+
+COMPAT_SRC = os.path.join(ROOT, "tests", "compat_check.cpp")
+COMPAT_BIN = os.path.join(ROOT, "tests", "compat_check.bin")
+
+
+def compat_check_command(out: str = COMPAT_BIN):
+    """g++ line of the reference-side binding check (tests/compat_check.cpp): the reference's
+    host_* calls compiled against include/mcsart_compat.h and linked with libmcsart_hip."""
+    cxx = shutil.which("g++") or shutil.which("c++")
+    if cxx is None:
+        raise RuntimeError("no host C++ compiler")
+    return [cxx, "-O1", "-std=c++17", "-Wall", "-Wextra", "-Werror", "-I",
+            os.path.join(ROOT, "include"), COMPAT_SRC, "-o", out, "-L", HERE, "-lmcsart_hip",
+            "-Wl,-rpath,$ORIGIN/../mcsart_amd"]
+
+
+def _build_compat_check() -> None:
+    if not os.path.exists(COMPAT_SRC):
+        return
+    r = subprocess.run(compat_check_command(COMPAT_BIN + ".tmp"), capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"compat check build failed\n{r.stdout}\n{r.stderr}")
+    os.replace(COMPAT_BIN + ".tmp", COMPAT_BIN)
 
 
 if __name__ == "__main__":
