@@ -306,9 +306,19 @@ def cpu_protocol(reps=5):
     proj = np.stack([oracle.projection(oracle.deform(d.phantom, d.alpha[i], d.beta[i], d.fields),
                                        g, spec.angles[i]) for i in range(spec.n_views)])
     log("[bench] CPU protocol: 128^3 reconstruction, 10 outer iterations (one run)")
+    # sweep by sweep (the driver's own composition: an image sweep leaves the image, the motion
+    # sweep after it starts from that image and updates the six fields), with progress lines
     t0 = time.perf_counter()
-    oracle.aumisart(np.zeros(g.vol_shape), proj, g, spec.angles, d.alpha, d.beta, spec.op_iter,
-                    fields=d.fields, nonneg=True)
+    img = np.zeros(g.vol_shape)
+    F = [f.astype(np.float64) for f in d.fields]
+    for k, op in enumerate(spec.op_iter):
+        if op == 1:
+            img = oracle.aumisart(img, proj, g, spec.angles, d.alpha, d.beta, [1], fields=F,
+                                  nonneg=True)[0]
+        else:
+            F, _ = oracle.motion_sweep(img, proj, g, spec.angles, d.alpha, d.beta, F)
+        log(f"[bench] CPU protocol: 128^3 sweep {k + 1}/{len(spec.op_iter)} at "
+            f"{time.perf_counter() - t0:.1f} s")
     out["mcsart128_recon_s"] = round(time.perf_counter() - t0, 2)
     # 256x256x128: extrapolated
     spec = specs["clinical256"]

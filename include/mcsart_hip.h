@@ -37,7 +37,8 @@ enum {
     MCSART_ENOMEM = -2,  /* device allocation failed or HBM budget exceeded */
     MCSART_EHIP = -3,    /* HIP runtime error */
     MCSART_ENOTSUP = -4, /* unsupported configuration */
-    MCSART_EBUSY = -5    /* resource in use */
+    MCSART_EBUSY = -5,   /* resource in use */
+    MCSART_ECANCELED = -6 /* the request was cancelled (mcsart_server_cancel) */
 };
 
 const char *mcsart_status_string(int status);
@@ -377,11 +378,14 @@ int mcsart_job_view_layouts(mcsart_job *job, int *layouts);
 /* ---------------------------------------------------------------------------------------------
  * 5. Reconstruction server: continuous batching of jobs on one device (no counterpart in the
  *    reference, which reconstructs one scan per MATLAB call, src/entry_AUMISART.cu:193-207).
- *    Requests queue in arrival order; a scheduler thread admits one whenever a slot (a stream)
- *    is free and its device footprint (mcsart_job_bytes) fits in the remaining HBM budget, runs
- *    it as a device-resident job on that slot, and on completion copies the results into the
- *    request's output buffers and frees the job.  Jobs on different slots overlap on the device.
- *    A request's results do not depend on what else runs (same kernels, shapes and order).
+ *    Requests queue in arrival order; one is admitted whenever a slot (a stream driven by its own
+ *    host thread) is free and its device footprint (mcsart_job_bytes) fits in the HBM budget.
+ *    It runs as a device-resident job in a block of the server's memory pool, its sweeps
+ *    enqueued with at most two in flight; on completion the results are copied into the
+ *    request's output buffers and the block returns to the pool for a later request of the same
+ *    or a somewhat smaller footprint (idle blocks are freed only when the budget needs them).
+ *    Jobs on different slots overlap on the device.  A request's results do not depend on what
+ *    else runs (same kernels, shapes and order).
  * ------------------------------------------------------------------------------------------- */
 typedef struct mcsart_server mcsart_server;
 
@@ -409,7 +413,13 @@ typedef struct {
     float *out_fields[6];                    /* each N or NULL */
 } mcsart_request;
 
-enum { MCSART_REQ_QUEUED = 0, MCSART_REQ_RUNNING = 1, MCSART_REQ_DONE = 2, MCSART_REQ_FAILED = 3 };
+enum {
+    MCSART_REQ_QUEUED = 0,
+    MCSART_REQ_RUNNING = 1,
+    MCSART_REQ_DONE = 2,
+    MCSART_REQ_FAILED = 3,
+    MCSART_REQ_CANCELLED = 4
+};
 
 typedef struct {
     int state;          /* MCSART_REQ_* */
@@ -427,12 +437,19 @@ typedef struct {
 int mcsart_server_create(int device, const mcsart_server_opts *opts, mcsart_server **server);
 /* MCSART_ENOMEM if the request could never fit the budget. */
 int mcsart_server_submit(mcsart_server *server, const mcsart_request *req, long long *id);
-/* Block until request id has completed (timeout_s < 0: no limit; MCSART_EBUSY on timeout);
- * returns the request's own status. */
+/* Block until request id has ended: done, failed or cancelled (timeout_s < 0: no limit;
+ * MCSART_EBUSY on timeout).  Returns the request's own status, with its message in
+ * mcsart_last_error() on the calling thread. */
 int mcsart_server_wait(mcsart_server *server, long long id, double timeout_s);
+/* Cancel request id.  Queued: it leaves the queue at once.  Running: no further sweep is
+ * enqueued; its slot (and its memory) is free once the sweeps already on the device complete (at
+ * most two), without copying results.  Either way it ends as MCSART_REQ_CANCELLED and waiting
+ * on it returns MCSART_ECANCELED.  MCSART_EINVAL for unknown or already ended requests. */
+int mcsart_server_cancel(mcsart_server *server, long long id);
 int mcsart_server_drain(mcsart_server *server);   /* wait for every submitted request */
 int mcsart_server_info(mcsart_server *server, long long id, mcsart_request_info *info);
 int mcsart_server_release(mcsart_server *server, long long id);   /* forget a completed request */
+/* used: device bytes held by admitted jobs; with the pool's idle blocks it stays within budget. */
 int mcsart_server_memory(mcsart_server *server, size_t *used, size_t *budget);
 /* Runs every queued request to completion, then stops the scheduler and frees the slots. */
 int mcsart_server_destroy(mcsart_server *server);

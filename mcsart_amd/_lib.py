@@ -81,6 +81,7 @@ SIGNATURES = {
     "mcsart_server_create": ([_I, _P, ctypes.POINTER(_P)], _I),
     "mcsart_server_submit": ([_P, _P, ctypes.POINTER(ctypes.c_longlong)], _I),
     "mcsart_server_wait": ([_P, ctypes.c_longlong, ctypes.c_double], _I),
+    "mcsart_server_cancel": ([_P, ctypes.c_longlong], _I),
     "mcsart_server_drain": ([_P], _I),
     "mcsart_server_info": ([_P, ctypes.c_longlong, _P], _I),
     "mcsart_server_release": ([_P, ctypes.c_longlong], _I),

@@ -177,6 +177,7 @@ extern "C" const char *mcsart_status_string(int status) {
         case MCSART_EHIP: return "HIP runtime error";
         case MCSART_ENOTSUP: return "not supported";
         case MCSART_EBUSY: return "busy";
+        case MCSART_ECANCELED: return "cancelled";
         default: return "unknown status";
     }
 }

@@ -39,6 +39,7 @@ struct mcsart_job {
     double *d_partials = nullptr;
     void *block = nullptr;       // single allocation backing all of the above
     size_t bytes = 0;
+    bool owns_block = true;      // false: the block belongs to the caller (a server's pool)
     // One captured graph per sweep of the schedule: the kernels of a sweep and all their
     // arguments are fixed for the life of the job, so a sweep is captured on first use and
     // replayed afterwards (one launch instead of ~4 per view).  Sweeps run eagerly while the
@@ -248,10 +249,10 @@ extern "C" size_t mcsart_job_bytes(const mcsart_geom *g, int n_views, int n_iter
     return layout(geom_of(g), n_views, n_iter, keep_residual).total;
 }
 
-extern "C" int mcsart_job_create(mcsart_ctx *ctx, const mcsart_geom *gp, int n_views,
-                                 const float *angles, const float *volumes, const float *flows,
-                                 const mcsart_opts *opts, int n_iter, const int *op_iter,
-                                 mcsart_job **out) {
+int mcsart::job_create_in(mcsart_ctx *ctx, const mcsart_geom *gp, int n_views,
+                          const float *angles, const float *volumes, const float *flows,
+                          const mcsart_opts *opts, int n_iter, const int *op_iter, void *block,
+                          size_t block_bytes, mcsart_job **out) {
     if (!out) return fail(MCSART_EINVAL, "null job pointer");
     *out = nullptr;
     if (!ctx || !gp || !angles || !volumes || !flows || !opts)
@@ -281,7 +282,16 @@ extern "C" int mcsart_job_create(mcsart_ctx *ctx, const mcsart_geom *gp, int n_v
     }
     j->graphs.resize(n_iter);
     if (const char *env = getenv("MCSART_GRAPHS")) j->use_graphs = atoi(env) != 0;
-    if (hipMalloc(&j->block, L.total) != hipSuccess) {
+    if (block) {   // the caller's memory: used, never freed here
+        if (block_bytes < L.total) {
+            delete j;
+            return fail(MCSART_EINVAL, "job block of " + std::to_string(block_bytes) +
+                                           " bytes is smaller than the " +
+                                           std::to_string(L.total) + " the job needs");
+        }
+        j->block = block;
+        j->owns_block = false;
+    } else if (hipMalloc(&j->block, L.total) != hipSuccess) {
         (void)hipGetLastError();
         delete j;
         return fail(MCSART_ENOMEM, "job allocation of " + std::to_string(L.total) + " bytes failed");
@@ -304,12 +314,20 @@ extern "C" int mcsart_job_create(mcsart_ctx *ctx, const mcsart_geom *gp, int n_v
     // everything starts at zero: image, fields, norms, residuals
     if (hipMemsetAsync(j->block, 0, L.total, ctx->stream) != hipSuccess ||
         hipStreamSynchronize(ctx->stream) != hipSuccess) {
-        (void)hipFree(j->block);
+        if (j->owns_block) (void)hipFree(j->block);
         delete j;
         return fail(MCSART_EHIP, "job initialisation failed");
     }
     *out = j;
     return MCSART_OK;
+}
+
+extern "C" int mcsart_job_create(mcsart_ctx *ctx, const mcsart_geom *gp, int n_views,
+                                 const float *angles, const float *volumes, const float *flows,
+                                 const mcsart_opts *opts, int n_iter, const int *op_iter,
+                                 mcsart_job **out) {
+    return job_create_in(ctx, gp, n_views, angles, volumes, flows, opts, n_iter, op_iter, nullptr,
+                         0, out);
 }
 
 extern "C" int mcsart_job_set_projections(mcsart_job *j, const float *h_proj) {
@@ -783,7 +801,7 @@ extern "C" int mcsart_job_destroy(mcsart_job *j) {
     (void)hipSetDevice(j->ctx->device);
     (void)hipStreamSynchronize(j->ctx->stream);
     for (auto &sg : j->graphs) drop_graph(sg);
-    (void)hipFree(j->block);
+    if (j->owns_block) (void)hipFree(j->block);
     delete j;
     return MCSART_OK;
 }

@@ -265,6 +265,12 @@ struct mcsart_ctx {
 };
 
 namespace mcsart {
+// mcsart_job_create in a caller-provided device block of block_bytes (at least
+// mcsart_job_bytes): the job uses it and never frees it.  block == nullptr: the job allocates its
+// own, as mcsart_job_create does.
+int job_create_in(mcsart_ctx *ctx, const mcsart_geom *g, int n_views, const float *angles,
+                  const float *volumes, const float *flows, const mcsart_opts *opts, int n_iter,
+                  const int *op_iter, void *block, size_t block_bytes, mcsart_job **out);
 mcsart_ctx *default_ctx();   // per-device context behind the synchronous operators
 int ctx_scratch(mcsart_ctx *ctx, size_t bytes, void **ptr);
 // Reduction scratch for the projector's SART modes: 2*nblocks doubles of partials.

@@ -44,7 +44,7 @@ class CRequestInfo(ctypes.Structure):
                 ("done_at_ms", ctypes.c_double)]
 
 
-STATES = {0: "queued", 1: "running", 2: "done", 3: "failed"}
+STATES = {0: "queued", 1: "running", 2: "done", 3: "failed", 4: "cancelled"}
 
 
 def _fp(a: Optional[np.ndarray]):
@@ -108,6 +108,11 @@ class Server:
     def wait(self, t: Ticket, timeout_s: float = -1.0) -> Ticket:
         check(self._lib.mcsart_server_wait(self.ptr, t.id, float(timeout_s)), "mcsart_server_wait")
         return t
+
+    def cancel(self, t: Ticket):
+        """Cancel a queued or running request (mcsart_server_cancel): a running one stops after
+        the sweeps already on the device; its results are not copied."""
+        check(self._lib.mcsart_server_cancel(self.ptr, t.id), "mcsart_server_cancel")
 
     def drain(self):
         check(self._lib.mcsart_server_drain(self.ptr), "mcsart_server_drain")

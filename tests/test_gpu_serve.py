@@ -134,3 +134,74 @@ def test_autotuned_requests_match(scans):
             inf = srv.info(t)
             assert inf["tuning"]["proj_split"] in (1, 2, 4, 8, 16)
             assert rel_l2(t.image, img) <= 1e-5
+
+
+@pytest.fixture(scope="module")
+def long_scan(gpu_lib):
+    """A 128^3 scan with 40 warped image sweeps (a few hundred ms on the device; image sweeps
+    only, so its result stays finite over the whole schedule)."""
+    s = ScanSpec("long128", 128, 128, 128, 2.0, 2.0, 256, 192, 1.5, 200, True, [1] * 40,
+                 seed=75)
+    d = make_scan_inputs(s)
+    return s, d, measure(s, d, s.seed + 5)
+
+
+def test_short_requests_cycle_past_a_long_one(scans, long_scan):
+    """Continuous batching: with 2 slots, short requests submitted behind a long one keep cycling
+    through the free slot while the long one runs (no completion waits for the whole device), and
+    every result equals its alone run."""
+    shorts = [scans[0], scans[2], scans[3], scans[0], scans[2]]
+    alone = _alone([long_scan] + shorts)
+    with Server(slots=2) as srv:
+        tl = _submit(srv, long_scan, want_fields=True)
+        ts = [_submit(srv, sc, want_fields=True) for sc in shorts]
+        srv.drain()
+        il = srv.info(tl)
+        infos = [srv.info(t) for t in ts]
+        for t, (img, nrm, flds) in zip([tl] + ts, alone):
+            assert np.array_equal(t.image, img) and np.array_equal(t.norms, nrm)
+    print(f"long: admitted {il['admit_at_ms']:.1f} ms, done {il['done_at_ms']:.1f} ms; shorts "
+          f"admitted at {[round(i['admit_at_ms'], 1) for i in infos]}, done at "
+          f"{[round(i['done_at_ms'], 1) for i in infos]}")
+    assert all(i["state"] == "done" for i in infos + [il])
+    # the last short request could only start after the earlier ones had finished on the second
+    # slot, and still started (and finished) while the long one was running
+    assert infos[-1]["admit_at_ms"] > infos[0]["done_at_ms"]
+    assert infos[-1]["done_at_ms"] < il["done_at_ms"]
+
+
+def test_cancel_running_and_queued(scans, long_scan):
+    """A running request cancelled mid-schedule stops after the sweeps already on the device and
+    frees its slot for the next request; a queued one leaves the queue without running.  Waiting
+    on either reports the cancellation (status and message on the waiting thread)."""
+    import time
+    with Server(slots=1) as srv:
+        t_full = _submit(srv, long_scan)
+        srv.wait(t_full)
+        full_ms = srv.info(t_full)["run_ms"]
+        tl = _submit(srv, long_scan)
+        tq = _submit(srv, scans[3])   # queued behind it
+        tc = _submit(srv, scans[0])   # cancelled while queued
+        t0 = time.perf_counter()
+        while srv.info(tl)["state"] != "running" and time.perf_counter() - t0 < 30:
+            time.sleep(0.001)
+        time.sleep(0.02)
+        srv.cancel(tc)
+        srv.cancel(tl)
+        with pytest.raises(McsartError) as e:
+            srv.wait(tl)
+        assert e.value.status == -6 and "cancelled" in str(e.value)
+        with pytest.raises(McsartError):
+            srv.wait(tc)
+        srv.wait(tq)
+        il, iq, ic = srv.info(tl), srv.info(tq), srv.info(tc)
+        with pytest.raises(McsartError):   # an ended request cannot be cancelled
+            srv.cancel(tl)
+    print(f"full run {full_ms:.1f} ms; cancelled run {il['run_ms']:.1f} ms")
+    assert il["state"] == "cancelled" and ic["state"] == "cancelled" and iq["state"] == "done"
+    assert ic["slot"] == -1                     # never admitted
+    assert il["run_ms"] < 0.5 * full_ms         # stopped early
+    # the slot went straight to the next request
+    assert il["done_at_ms"] - 1.0 <= iq["admit_at_ms"] < il["done_at_ms"] + 50.0
+    img, nrm, _ = _alone([scans[3]])[0]
+    assert np.array_equal(tq.image, img) and np.array_equal(tq.norms, nrm)
