@@ -440,8 +440,8 @@ def main():
                     help="outer iterations of the 16-scan uneven batch leg (0: skip the batch "
                          "and served legs)")
     ap.add_argument("--batch-slots", type=int, default=4)
-    ap.add_argument("--serve-rate", type=float, default=1.0,
-                    help="arrivals per second of the served leg (0: skip it)")
+    ap.add_argument("--serve-rates", default="1,4",
+                    help="arrivals per second of the served leg, comma-separated (empty: skip)")
     ap.add_argument("--concurrent", type=int, default=0,
                     help="also time this many copies of the scan in flight at once (0: skip)")
     args = ap.parse_args()
@@ -578,17 +578,19 @@ def main():
         batch["mix"] = (f"16 seeded scans, volumes 128^3..320x320x160, 200-900 views, detectors "
                         f"256x192..512x384, [1,0]x{args.batch_outer}, noise-free data, all "
                         f"submitted at t = 0")
-        if args.serve_rate > 0:
+        rates = [float(r) for r in args.serve_rates.split(",") if r.strip()]
+        if rates:
             large = configs.scaled(configs.c5_large(), op_iter=[1, 0] * args.batch_outer)
             large.noise_photons = None
             log("[bench] served leg: making the large scan's inputs")
             linputs = make_inputs([large])
             stream = binputs[:8] + linputs + binputs[8:]
-            served = served_leg(stream, args.serve_rate, args.batch_slots, 20261203 + 5099)
-            served["stream"] = (f"the 16 batch scans with one 512x512x256 scan (900 views, "
-                                f"1024x768 detector) after the eighth, [1,0]x{args.batch_outer}, "
-                                f"Poisson arrivals; latency = submission to the full schedule's "
-                                f"reconstruction in the request's buffers")
+            served = {
+                "stream": (f"the 16 batch scans with one 512x512x256 scan (900 views, 1024x768 "
+                           f"detector) after the eighth, [1,0]x{args.batch_outer}, Poisson "
+                           f"arrivals; latency = submission to the full schedule's "
+                           f"reconstruction in the request's buffers"),
+                "runs": [served_leg(stream, r, args.batch_slots, 20261203 + 5099) for r in rates]}
 
     cpu = None
     if not args.no_cpu_baseline:

@@ -42,3 +42,5 @@ python3 -m mcsart_amd.profile_summary --workload "$W" --views "$V" --trace "$OUT
     --md "$OUT/${W}_kernels.md" --json "$OUT/${W}_kernels.json" \
     --command "PYTHONPATH=\$REPO TMPDIR=/tmp bash mcsart_amd/profile_workload.sh $W $T $V $OUT" \
     "${ARGS[@]}"
+# the per-dispatch trace is large and the summary uses only the statistics
+rm -f "$REPO/$OUT/trace/run_kernel_trace.csv"
