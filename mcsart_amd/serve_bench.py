@@ -3,12 +3,16 @@ one-job-at-a-time processing of the same requests.
 
     python -m mcsart_amd.serve_bench batch [--jobs 16] [--outer 10] [--slots 1,2,4,8]
     python -m mcsart_amd.serve_bench serve [--rates 0.5,1,2,4] [--requests 20] [--outer 2]
+    python -m mcsart_amd.serve_bench ceiling [--outer 2]
 
 ``batch``: the 16-scan uneven mix (volumes 128^3 .. 320x320x160, 200-900 views, detectors
 256x192 .. 512x384), all submitted at t = 0; one run per slot count (slots = 1 is plain
 one-at-a-time processing through the same code).  ``serve``: Poisson arrivals at each rate drawn
 from the same mix plus 10% large scans (512x512x256, 900 views, 1024x768 detector), each arrival
-submitted at its arrival time; plain (1 slot) and batched runs see the same stream.  Inputs are
+submitted at its arrival time; plain (1 slot) and batched runs see the same stream.
+``ceiling``: the most any packing of the batch's jobs (concurrent streams or one ragged launch)
+could gain over one at a time, from each kernel's measured cost per unit of work by job size
+(run_ceiling).  Inputs are
 synthetic (seeded phantoms, traces, fields; projections made on the GPU with Poisson noise).
 Prints one JSON object per run and a summary line.
 """
@@ -158,9 +162,88 @@ def run_serve(args):
                       "runs": runs}), flush=True)
 
 
+KINDS = ("projector", "projector_motion", "backprojector", "warp", "warp_add", "motion")
+
+
+def _work_units(spec):
+    """Work per launch of each kernel kind: ray-planes (detector cells x planes) for the
+    projectors, voxels for the rest."""
+    rays = spec.na * spec.nb * max(spec.nx, spec.ny)
+    vox = spec.nx * spec.ny * spec.nz
+    return {"projector": rays, "projector_motion": rays, "backprojector": vox, "warp": vox,
+            "warp_add": vox, "motion": vox}
+
+
+def _launches(spec):
+    """Launches of each kind in the scan's schedule (image sweep: warp, projector,
+    backprojector, warp-add per view; motion sweep: projector and fused backprojector per view
+    after the first)."""
+    n_img = sum(1 for o in spec.op_iter if o == 1)
+    n_mot = len(spec.op_iter) - n_img
+    V = spec.n_views
+    return {"projector": n_img * V, "backprojector": n_img * V, "warp": n_img * V,
+            "warp_add": n_img * V, "projector_motion": n_mot * (V - 1), "motion": n_mot * (V - 1)}
+
+
+def run_ceiling(args):
+    """Upper bound on what packing the batch's jobs together (streams or one ragged launch) can
+    gain over running them one at a time.  Every kernel kind is timed alone (all views of the job
+    back to back, the server's automatic launch shapes) for each distinct (volume, detector)
+    class of the batch and divided by its work; the lowest cost per unit over the classes is the
+    device's saturated rate for that kernel.  One-at-a-time time of the batch = sum over its
+    jobs' launches of the class's measured launch time; the packed bound = the same launches
+    costed at the saturated rate with no gaps between kernels.  ceiling = their ratio."""
+    from .driver import Context, Job
+    jobs = configs.c4_batch(args.jobs, n_outer=args.outer)
+    for s in jobs:
+        s.noise_photons = None
+    classes = {}
+    for s in jobs:
+        classes.setdefault((s.nx, s.ny, s.nz, s.na, s.nb), s)
+    ctx = Context(0)
+    per_class = {}
+    for key, s in classes.items():
+        sc = scaled(s, n_views=min(s.n_views, args.ceiling_views))
+        d = make_scan_inputs(sc)
+        p = measure(sc, d, sc.seed + 404)
+        job = Job(ctx, sc.geometry, sc.angles, d.alpha, d.beta, [1, 0], lam=sc.lam,
+                  motion=True, nonneg=sc.nonneg)
+        job.set_projections(p)
+        job.set_image(None)
+        job.set_fields(d.fields)
+        us = {k: job.kernel_time(k, reps=3) for k in KINDS}
+        job.close()
+        units = _work_units(s)
+        per_class[key] = {"us": us, "us_per_munit": {k: 1e6 * us[k] / units[k] for k in KINDS}}
+        log(f"[serve_bench] ceiling class {key}: " +
+            ", ".join(f"{k} {us[k]:.1f} us" for k in KINDS))
+    ctx.close()
+    best = {k: min(c["us_per_munit"][k] for c in per_class.values()) for k in KINDS}
+    alone_ms = packed_ms = 0.0
+    for s in jobs:
+        c = per_class[(s.nx, s.ny, s.nz, s.na, s.nb)]
+        n, u = _launches(s), _work_units(s)
+        alone_ms += sum(n[k] * c["us"][k] for k in KINDS) * 1e-3
+        packed_ms += sum(n[k] * u[k] * best[k] * 1e-6 for k in KINDS) * 1e-3
+    out = {"mode": "ceiling", "jobs": args.jobs, "outer": args.outer,
+           "classes": [{"volume": list(k[:3]), "detector": list(k[3:]),
+                        "us_per_launch": {q: round(v, 2) for q, v in c["us"].items()},
+                        "us_per_million_units": {q: round(v, 3)
+                                                 for q, v in c["us_per_munit"].items()}}
+                       for k, c in per_class.items()],
+           "saturated_us_per_million_units": {k: round(v, 3) for k, v in best.items()},
+           "kernel_ms_one_at_a_time": round(alone_ms, 1),
+           "kernel_ms_packed_bound": round(packed_ms, 1),
+           "packing_ceiling": round(alone_ms / packed_ms, 3)}
+    print(json.dumps(out), flush=True)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("mode", choices=["batch", "serve"])
+    ap.add_argument("mode", choices=["batch", "serve", "ceiling"])
+    ap.add_argument("--ceiling-views", type=int, default=128,
+                    help="ceiling: views of each class's timing job")
     ap.add_argument("--jobs", type=int, default=16)
     ap.add_argument("--outer", type=int, default=10, help="outer iterations (image+motion sweep)")
     ap.add_argument("--slots", type=lambda s: [int(x) for x in s.split(",")], default=[1, 2, 4])
@@ -172,6 +255,8 @@ def main():
     args = ap.parse_args()
     if args.mode == "batch":
         run_batch(args)
+    elif args.mode == "ceiling":
+        run_ceiling(args)
     else:
         run_serve(args)
 
