@@ -436,10 +436,12 @@ __global__ void __launch_bounds__(RAYS * 16) k_project_rows(GeomArgs g, double c
         const int chunk = (np + nsplit - 1) / nsplit;
         const int ip_lo = ty * chunk;
         const int ip_hi = min(np, ip_lo + chunk);
-        // PL planes per step, all their gathers in flight together (one plane when a lane walks
-        // four rays: sixteen or more gathers already), the last ones one at a time.  p and the
+        // PL planes per step, all their gathers in flight together, the last ones one at a time.
+        // One plane when a lane walks two or more rays: their gathers already overlap, and a
+        // second plane's registers cost occupancy (R = 2 image mode: 74 VGPRs, 6 waves per SIMD,
+        // against 53 and 8 with one plane; measured 52.0 -> 50.6 us at 256x256x128).  p and the
         // plane's byte offset are carried: p is a small integer + 0.5, exact in float.
-        constexpr int PL = R >= 4 ? 1 : 2;
+        constexpr int PL = R >= 2 ? 1 : 2;
         int ip = ip_lo;
         float p = (float)ip_lo + p0;
         unsigned pofs = (xb ? a.roffT : a.roffN) + (unsigned)ip_lo * spB;
