@@ -28,7 +28,10 @@ namespace mcsart {
 namespace {
 
 constexpr int BX = 16, BY = 16;
-constexpr int LDS_CELLS = 4096;   // staged cells per workgroup (32 KB of float2)
+// Staged cells per workgroup: 4094 float2 + the four rectangle bounds = 32 KiB exactly, so five
+// workgroups share a CU's 160 KiB (five waves per SIMD; the copy below keeps the kernel under
+// 96 VGPRs, which five waves also need).
+constexpr int LDS_CELLS = 4094;
 
 // Axial bounds of a column's voxels relative to an integer anchor: the four lines
 // q*zc + c of src/cu_backprojection.cu:107-132 (slopes Rm/db, Rp/db, Rc/db twice) evaluated in
@@ -201,14 +204,14 @@ __global__ void __launch_bounds__(BX *BY) k_backproject(GeomArgs g, double c, do
         RA = max(rA1 - A0, 0);
         RB = max(rB1 - B0, 0);
         staged = RA > 0 && RB > 0 && RA * RB <= LDS_CELLS;   // uniform over the workgroup
-        if (staged) {   // coalesced copy of the rectangle, 8 loads in flight per lane
+        if (staged) {   // coalesced copy of the rectangle, 4 loads in flight per lane
             const int total = RA * RB;
             const float inv = 1.0f / (float)RA;
-            for (int base = tid; base < total; base += BX * BY * 8) {
-                float2 v[8];
-                int e[8];
+            for (int base = tid; base < total; base += BX * BY * 4) {
+                float2 v[4];
+                int e[4];
 #pragma unroll
-                for (int k = 0; k < 8; ++k) {
+                for (int k = 0; k < 4; ++k) {
                     // past the end: element total-1 again (the same value to the same slot)
                     e[k] = min(base + k * BX * BY, total - 1);
                     int row = (int)((float)e[k] * inv);   // e < 4096: off by at most one
@@ -220,7 +223,7 @@ __global__ void __launch_bounds__(BX *BY) k_backproject(GeomArgs g, double c, do
                                               : det_at<MODE>(rp, src * 8u);
                 }
 #pragma unroll
-                for (int k = 0; k < 8; ++k) stage[e[k]] = v[k];
+                for (int k = 0; k < 4; ++k) stage[e[k]] = v[k];
             }
         }
         __syncthreads();
