@@ -1,6 +1,7 @@
 """Edge cases of the path: degenerate sizes, empty and single-view schedules, exact branch ties,
-and size-independent properties at the largest served size (512x512x256 with a 1024x768
-detector), where the FP64 oracle would take minutes per view.
+and the largest served size (512x512x256 with a 1024x768 detector): the operators against the
+FP64 oracle on a few views (seconds each on the host's cores), and size-independent properties
+of projection, backprojection and whole sweeps.
 """
 import numpy as np
 import pytest
@@ -9,7 +10,8 @@ import oracle
 from mcsart_amd import configs
 from mcsart_amd.driver import Context, Job
 from mcsart_amd.geometry import Geometry, uniform_angles
-from mcsart_amd.ops import DeviceArray, backproject, project
+from mcsart_amd.ops import DeviceArray, backproject, deform, project
+from mcsart_amd.synth import Voxel, ellipsoid_phantom
 from _util import rel_l2, smooth_random, smooth_random_large
 
 pytestmark = pytest.mark.gpu
@@ -111,6 +113,29 @@ LARGE = configs.c5_large().geometry   # 512 x 512 x 256, detector 1024 x 768
 def _rot90(img):
     """The object rotated by +90 degrees about z: out[:, a, b] = img[:, n-1-b, a] (even n)."""
     return np.ascontiguousarray(img[:, ::-1, :].transpose(0, 2, 1))
+
+
+def test_large_operators_parity(gpu_lib):
+    """At the largest served size, against the FP64 oracle: projection of a sharp-edged phantom
+    on a y-branch and an x-branch view, backprojection of that projection, and the warp with
+    smooth 5DCT fields; rel-L2 <= 1e-5 each."""
+    g = LARGE
+    ph = ellipsoid_phantom(g.nx, g.ny, g.nz, Voxel(1.0, 1.0), 20261204)
+    p = None
+    for ang in (0.3, 1.3):   # |cos| > |sin|, then |sin| > |cos|: both ray branches
+        a = np.float32(ang)
+        print(f"large: projection at {ang}", flush=True)
+        p = oracle.projection(ph, g, a)
+        e = rel_l2(project(ph, g, a), p)
+        assert e <= TOL, f"projection at {ang}: rel-L2 {e:.3e}"
+    print("large: backprojection at 1.3", flush=True)
+    e = rel_l2(backproject(p, g, np.float32(1.3)), oracle.backprojection(p, g, np.float32(1.3)))
+    assert e <= TOL, f"backprojection: rel-L2 {e:.3e}"
+    base = [smooth_random_large(g.vol_shape, 60 + k, scale=0.8) for k in range(3)]
+    fields = base + [(-0.4 * f).astype(np.float32) for f in base[::-1]]
+    print("large: warp (0.8, -0.4)", flush=True)
+    e = rel_l2(deform(ph, 0.8, -0.4, fields), oracle.deform(ph, 0.8, -0.4, fields))
+    assert e <= TOL, f"warp: rel-L2 {e:.3e}"
 
 
 def test_large_projection_properties(gpu_lib):
