@@ -746,8 +746,9 @@ int launch_projector(const Geom &g, const ViewTrig &t, const ViewInfo &vi, ProjM
         pick_rows(g, rows, split, R, S);
         // The motion mode gathers (g, Q) pairs: with more rays per lane its footprints' values
         // hold twice the registers and the lanes wait on them.  One ray per lane is its best
-        // shape at 128^3 and 256x256x128 (10.4 / 56.6 us against 15.0 / 61.4 us with two),
-        // while the residual mode is fastest with two or four at 256x256x128.
+        // shape at 128^3 and 256x256x128 (11.3 / 57.0 us against 12.3 / 57.8 us with two rays
+        // and one plane per step, 15.0 / 61.4 us with two planes), while the residual mode is
+        // fastest with two or four at 256x256x128.
         if (mode == PROJ_MOTION) R = 1;
         switch (mode) {
             case PROJ_PLAIN: rc = launch_mode_rows<PROJ_PLAIN>(g, t, vi, ar, st, R, S); break;
