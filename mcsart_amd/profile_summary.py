@@ -235,10 +235,37 @@ def lds_markdown(pmc, title, command_lines, notes=()):
     return "\n".join(out) + "\n"
 
 
+def occupancy_markdown(pmc, title, command_lines, notes=()):
+    """Mean waves resident per CU (rocprofv3's MeanOccupancyPerCU) per kernel and grid size, from
+    one --pmc pass: the grid size tells the launches of different scan sizes apart."""
+    per = collections.defaultdict(list)
+    for r in csv.DictReader(open(os.path.join(pmc, "run_counter_collection.csv"))):
+        if r["Counter_Name"] != "MeanOccupancyPerCU":
+            continue
+        k = short(r["Kernel_Name"])
+        if "rocclr" in k:
+            continue
+        per[(k, int(r["Grid_Size"]), int(r["Workgroup_Size"]))].append(float(r["Counter_Value"]))
+    out = [f"# {title}", "", "Commands (on the MI355X box, from /tmp, PYTHONPATH = repository "
+           "root):", ""]
+    out += ["    " + c for c in command_lines]
+    out += ["", "Per-dispatch means of MeanOccupancyPerCU (waves resident per CU, averaged over the "
+            "dispatch) by kernel and grid size (work-items per launch).", ""]
+    out += ["| kernel | grid | workgroup | wavefronts per launch | launches | waves per CU |",
+            "|---|---:|---:|---:|---:|---:|"]
+    for (k, grid, wg), v in sorted(per.items()):
+        out.append(f"| `{k}` | {grid} | {wg} | {grid // 64} | {len(v)} | "
+                   f"{sum(v) / len(v):.1f} |")
+    out += [""] + list(notes)
+    return "\n".join(out) + "\n"
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--stats", help="write only the --stats table of this trace directory")
     ap.add_argument("--lds", help="write only the LDS/issue table of this --pmc directory")
+    ap.add_argument("--occupancy", help="write only the occupancy table of this --pmc directory "
+                                        "(MeanOccupancyPerCU)")
     ap.add_argument("--title", default="rocprofv3 kernel statistics")
     ap.add_argument("--note", action="append", default=[])
     ap.add_argument("--workload", choices=sorted(WORKLOADS))
@@ -253,8 +280,9 @@ def main(argv=None):
     ap.add_argument("--command", action="append", default=[],
                     help="command line that produced the inputs (repeatable)")
     args = ap.parse_args(argv)
-    if args.lds:
-        md = lds_markdown(args.lds, args.title, args.command, args.note)
+    if args.lds or args.occupancy:
+        md = (lds_markdown(args.lds, args.title, args.command, args.note) if args.lds else
+              occupancy_markdown(args.occupancy, args.title, args.command, args.note))
         if args.md:
             with open(args.md, "w") as f:
                 f.write(md)
