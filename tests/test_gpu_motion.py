@@ -7,7 +7,7 @@ second view on this groups the same sums differently from the reference's add2 s
 tests run many views through it:
 
 * a well-conditioned sweep with a closed form (ramp image: u = c exactly), 23 motion views at the
-  128^3 scan's geometry, against the FP64 oracle: residual ratios to 1e-4, field increments to
+  128^3 scan's geometry and 9 at the 256x256x128 scan's, against the FP64 oracle: residual ratios to 1e-4, field increments to
   rel-L2 1e-4 away from the edge of B(P q)'s support (where u jumps between c and 0 under
   rounding, 0/0 -> 0) and within 3x the reference's own FP32 error everywhere;
 * teacher forcing at depth on the real 128^3 scan: the FP64 oracle runs the sweep to views 49, 99
@@ -60,8 +60,17 @@ def test_motion_sweep_closed_form_c2_geometry(ctx):
     """23 motion views on the 128^3 scan's geometry (128^3 at 2 mm, 256x192 detector at 1.5 mm):
     ramp image, smooth random fields, the well-conditioned trace (-vd - fd in [0.1, 0.35],
     c_i in [75, 225]), data built so that u = c_i exactly."""
-    g = configs.c2_mcsart128().geometry
-    V = 24
+    _closed_form_sweep(ctx, configs.c2_mcsart128().geometry, 24)
+
+
+def test_motion_sweep_closed_form_c3_geometry(ctx):
+    """The same construction on the 256x256x128 scan's geometry (1.5 x 1.5 x 2 mm voxels,
+    512x384 detector at 0.78 mm), 9 motion views: the headline workload's shapes (window widths,
+    x- and y-branch views, both prep layouts) through the accumulator form."""
+    _closed_form_sweep(ctx, configs.c3_clinical().geometry, 10)
+
+
+def _closed_form_sweep(ctx, g, V):
     vol0, flo0, cs = well_conditioned_trace(V)
     x, F0, proj, ang, vol, flo, F_cf, ratios_cf, shell = ramp_motion_scan(
         g, V, 1, volumes=vol0, flows=flo0, cs=cs, shell=True)
