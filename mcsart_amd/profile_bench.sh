@@ -13,7 +13,7 @@
 set -o pipefail
 W=${1:-clinical256}
 REPO=$(pwd)
-OUT=${2:-gpurun_out/bench_trace_$W}
+OUT=${2:-prof/bench_trace_$W}
 export PYTHONPATH=$REPO
 export TMPDIR=/tmp
 export MCSART_GRAPHS=0

@@ -1,8 +1,8 @@
 """Summarise rocprofv3 output of ``mcsart_amd.profile_sweep`` into a per-kernel table.
 
     python -m mcsart_amd.profile_summary --workload mcsart128 \\
-        --trace gpurun_out/trace_mcsart128 --fetch gpurun_out/pmc_FETCH_SIZE_mcsart128 \\
-        --write gpurun_out/pmc_WRITE_SIZE_mcsart128 --md profiles/x.md --json profiles/x.json
+        --trace prof/trace_mcsart128 --fetch prof/pmc_FETCH_SIZE_mcsart128 \\
+        --write prof/pmc_WRITE_SIZE_mcsart128 --md profiles/x.md --json profiles/x.json
 
 Inputs are rocprofv3 CSV directories: a ``--kernel-trace --stats`` run and one ``--pmc`` run per
 counter (FETCH_SIZE, WRITE_SIZE; KiB per dispatch).  The counters are calibrated against the last

@@ -18,7 +18,7 @@ W=$1
 T=$2
 V=$3
 REPO=$(pwd)
-OUT=${4:-gpurun_out/prof_$W}
+OUT=${4:-prof/prof_$W}
 export PYTHONPATH=$REPO
 export TMPDIR=/tmp
 mkdir -p "$REPO/$OUT"

@@ -95,7 +95,7 @@ def _closed_form_sweep(ctx, g, V):
           f"inputs) vs closed form off the shell {max(e_cf):.2e}")
     assert np.all(ng[1:] > 0)
     assert e_ratio <= 1e-4
-    # off the shell the sweep is well conditioned
+    # off the shell the sweep is well conditioned: the per-result parity bar
     assert max(e_off) <= 1e-4
     # in the shell u jumps between c_i and 0 with rounding (0/0 -> 0): the reference's own FP32
     # envelope
