@@ -120,3 +120,25 @@ def test_lds_table(tmp_path):
     md = ps.lds_markdown(str(tmp_path / "l"), "t", ["cmd"])
     row = [r for r in md.splitlines() if r.startswith("| `k_backproject")]
     assert row == ["| `k_backproject<1, 4, true, 0>` | 1 | 7 | 0.500 | 0.020 | 0.250 | 0.300 |"]
+
+
+def test_occupancy_table(tmp_path):
+    """MeanOccupancyPerCU averaged per (kernel, grid size): launches of different scan sizes
+    stay apart."""
+    k = "void mcsart::(anonymous namespace)::k_warp_tile<true, 32>(float*, float const*)"
+    path = tmp_path / "o"
+    path.mkdir()
+    with open(path / "run_counter_collection.csv", "w", newline="") as f:
+        w = csv.DictWriter(f, fieldnames=["Dispatch_Id", "Kernel_Name", "Grid_Size",
+                                          "Workgroup_Size", "Counter_Name", "Counter_Value"])
+        w.writeheader()
+        rows = [(k, 2097152, "MeanOccupancyPerCU", 14.0), (k, 2097152, "MeanOccupancyPerCU", 16.0),
+                (k, 2097152, "SQ_BUSY_CU_CYCLES", 99.0), (k, 8388608, "MeanOccupancyPerCU", 20.5),
+                ("__amd_rocclr_copyBuffer", 1024, "MeanOccupancyPerCU", 1.0)]
+        for i, (n, grid, c, v) in enumerate(rows):
+            w.writerow({"Dispatch_Id": i, "Kernel_Name": n, "Grid_Size": grid,
+                        "Workgroup_Size": 1024, "Counter_Name": c, "Counter_Value": v})
+    md = ps.occupancy_markdown(str(path), "t", ["cmd"])
+    rows = [r for r in md.splitlines() if r.startswith("| `")]
+    assert rows == ["| `k_warp_tile<true, 32>` | 2097152 | 1024 | 32768 | 2 | 15.0 |",
+                    "| `k_warp_tile<true, 32>` | 8388608 | 1024 | 131072 | 1 | 20.5 |"]
