@@ -8,7 +8,7 @@ da = pitch/dx, db = pitch/dz).
 """
 from __future__ import annotations
 
-from dataclasses import dataclass, field
+from dataclasses import dataclass
 from typing import List, Optional
 
 import numpy as np

@@ -15,7 +15,6 @@ import argparse
 import sys
 import time
 
-import numpy as np
 
 from . import configs
 from .configs import make_scan_inputs

@@ -1,6 +1,5 @@
 """GPU parity of the fused SART kernels against the FP64 oracle's composition of the reference's
 operator calls (src/AUMISART.cu:194-218 image sweep, :246-277 motion sweep)."""
-import ctypes
 
 import numpy as np
 import pytest

@@ -10,7 +10,6 @@ import pytest
 
 from mcsart_amd import _lib, configs
 from mcsart_amd.driver import MexBadInput, parse_geo, parse_iter
-from mcsart_amd.geometry import Geometry, uniform_angles
 from mcsart_amd.synth import (SplitMix64, breathing_trace, ellipsoid_phantom, motion_fields,
                               poisson_noise, seed_for, Voxel)
 

@@ -6,8 +6,6 @@ against mcsart_projection (the per-projection tolerance against the FP64 oracle)
 in a row, the second with a smaller volume and detector (the array keeps the first call's
 contents), and each against the FP64 oracle.
 """
-import ctypes
-
 import numpy as np
 import pytest
 
