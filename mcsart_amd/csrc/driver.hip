@@ -31,7 +31,7 @@ struct mcsart_job {
     float *d_w1 = nullptr;       // 2N: warped image (image sweep) / (g, Q) pairs (motion sweep)
     float *d_w1T = nullptr;      // 2N: the same in the xy-transposed layout / transposed image
     float *d_w2 = nullptr;       // N: u
-    float *d_acc = nullptr;      // 2N: motion-sweep accumulators, transposed (SVt, SFt)
+    float *d_acc = nullptr;      // 2N: motion-sweep accumulators, transposed, as (SVt, SFt) pairs
     float *d_cst = nullptr;      // 3N: motion-sweep constants (Ga, Gb, H) per voxel
     float2 *d_rp = nullptr;      // D
     float *d_norm = nullptr;     // n_iter*V
@@ -214,7 +214,7 @@ int motion_sweep(mcsart_job *j, int it) {
             const int nv = iv + stride;
             const bool more = nv < j->V;
             const ViewInfo &vn = j->vinfo[more ? nv : iv];
-            const MotionStep m{j->d_acc, j->d_acc + N, j->d_cst, j->d_img,
+            const MotionStep m{reinterpret_cast<float2 *>(j->d_acc), j->d_cst, j->d_img,
                                more && vn.has_y ? pair : nullptr,
                                more && vn.has_x ? pairT : nullptr,
                                j->volumes[iv - 1] - j->volumes[iv], j->flows[iv - 1] - j->flows[iv],
@@ -230,8 +230,8 @@ int motion_sweep(mcsart_job *j, int it) {
     }
     {   // fields = fields_0 + D_c x * accumulators
         TimedRegion tr(j->ctx, TK_MOTION);
-        rc = launch_motion_finish(j->d_acc, j->d_acc + N, j->d_img, fields_w(j), g.nx, g.ny,
-                                  g.nz, st);
+        rc = launch_motion_finish(reinterpret_cast<const float2 *>(j->d_acc), j->d_img,
+                                  fields_w(j), g.nx, g.ny, g.nz, st);
         if (rc) return rc;
     }
     return MCSART_OK;
@@ -631,7 +631,7 @@ extern "C" int mcsart_job_kernel_time(mcsart_job *j, int kind, int reps, double 
                                       // the prep of view iv + 1 (views 1..V-2)
                 if (iv == 0 || iv + 1 >= j->V) continue;
                 const ViewInfo &vn = j->vinfo[iv + 1];
-                const MotionStep m{j->d_acc, j->d_acc + N, j->d_cst, j->d_img,
+                const MotionStep m{reinterpret_cast<float2 *>(j->d_acc), j->d_cst, j->d_img,
                                    vn.has_y ? pair : nullptr, vn.has_x ? pairT : nullptr,
                                    j->volumes[iv - 1] - j->volumes[iv],
                                    j->flows[iv - 1] - j->flows[iv],
@@ -765,7 +765,7 @@ extern "C" int mcsart_job_autotune(mcsart_job *j, mcsart_tuning *chosen) {
             for (int iv0 : views) {
                 const int iv = std::max(iv0, 1), nv = std::min(iv + 1, j->V - 1);
                 const ViewInfo &vn = j->vinfo[nv];
-                const MotionStep m{j->d_acc, j->d_acc + N, j->d_cst, j->d_img,
+                const MotionStep m{reinterpret_cast<float2 *>(j->d_acc), j->d_cst, j->d_img,
                                    vn.has_y ? pair : nullptr, vn.has_x ? pairT : nullptr,
                                    j->volumes[iv - 1] - j->volumes[iv],
                                    j->flows[iv - 1] - j->flows[iv], j->volumes[nv] - j->volumes[iv],

@@ -140,14 +140,16 @@ struct BackTune {
 // vi: the view's transverse window (view_info).  nf (optional): the first wavefront of the first
 // workgroup also finishes the preceding projector's norm reduction.
 // A motion-sweep step fused into the ratio backprojector (x is constant over the sweep; the
-// accumulators are kept transposed, SVt(p) = SV(p^T), so every access is at the voxel itself):
+// accumulators are kept transposed, SVt(p) = SV(p^T), so every access is at the voxel itself,
+// and interleaved, acc[p] = (SVt(p), SFt(p)), so a voxel reads and writes both with one 8-byte
+// access):
 //   SVt(p) += wv u(p), SFt(p) += wf u(p)                (update of view i, from zero if `first`)
 //   img0(p) = vd Ga(p) + fd Gb(p) + H(p) (vd SVt(p) + fd SFt(p))
 //   g(p) = x(p) - img0(p)                               (prep of view i+1, unless `last`)
 // with cst = (Ga, Gb, H) per voxel (3 floats) from launch_motion_begin, which also wrote the
 // sweep's Q (the projector scales P(Q) by -vd - fd); u itself is not stored.
 struct MotionStep {
-    float *svt, *sft;
+    float2 *acc;   // (SVt, SFt) per voxel
     const float *cst;
     const float *x;
     float2 *pair, *pairT;   // the g halves of the (g, Q) pairs the next view reads (either
@@ -224,9 +226,9 @@ int launch_motion_begin(const Fields &f, const float *x, float *cst, float2 *pai
                         hipStream_t st);
 // ... then per updated view the ratio backprojector with a MotionStep, and at the end of the
 // sweep the fields alpha_c = alpha_c0 + D_c x SV, beta_c = beta_c0 + D_c x SF in place (f),
-// with SV(P) = svt(P^T), SF(P) = sft(P^T).
-int launch_motion_finish(const float *svt, const float *sft, const float *x, const FieldsW &f,
-                         int nx, int ny, int nz, hipStream_t st);
+// with (SV(P), SF(P)) = acc[P^T].
+int launch_motion_finish(const float2 *acc, const float *x, const FieldsW &f, int nx, int ny,
+                         int nz, hipStream_t st);
 
 // img1 *= w * D_ind(img0) (src/cu_multiple.cu:11-43)
 int launch_multiple(float *a, const float *b, int nx, int ny, int nz, float w, int ind,

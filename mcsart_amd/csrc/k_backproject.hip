@@ -83,8 +83,9 @@ struct MotionIn {
 };
 __device__ inline MotionIn motion_load(const MotionStep &m, int id) {
     MotionIn in;
-    in.sv = m.first ? 0.f : m.svt[id];
-    in.sf = m.first ? 0.f : m.sft[id];
+    const float2 a = m.first ? make_float2(0.f, 0.f) : m.acc[id];
+    in.sv = a.x;
+    in.sf = a.y;
     in.c = m.last ? MotionCst{0.f, 0.f, 0.f}
                   : *reinterpret_cast<const MotionCst *>(m.cst + 3u * (unsigned)id);
     in.x = m.last ? 0.f : m.x[id];
@@ -94,8 +95,7 @@ __device__ inline void motion_step_at(const MotionStep &m, const MotionIn &in, i
                                       float u) {
     const float sv = in.sv + m.wv * u;
     const float sf = in.sf + m.wf * u;
-    m.svt[id] = sv;
-    m.sft[id] = sf;
+    m.acc[id] = make_float2(sv, sf);
     if (m.last) return;
     const MotionCst c = in.c;
     const float img0 = m.vd * c.ga + m.fd * c.gb + c.h * (m.vd * sv + m.fd * sf);

@@ -541,7 +541,8 @@ __global__ void __launch_bounds__(TT *TR) k_motion_step(float *__restrict__ ax,
 // q = Q (-vd - fd) is the sweep's Q times a scalar (the projector scales P(Q); q is never
 // stored).  k_motion_begin makes (Ga, Gb, H) and Q once per sweep; each view's update and the
 // next view's prep g run in the ratio backprojector's epilogue (k_backproject.hip), which keeps
-// the accumulators transposed (svt(p) = SV(p^T)); k_motion_finish writes the fields at the end.
+// the accumulators transposed and interleaved (acc(p) = (SV(p^T), SF(p^T))); k_motion_finish
+// writes the fields at the end.
 
 // Per-sweep constants (Ga, Gb, H) at every voxel from the sweep's starting fields, and the pairs
 // (g, Q) in both layouts: Q = (D1 + D2 + D3) x for the whole sweep, g the prep of its first
@@ -613,18 +614,26 @@ __global__ void __launch_bounds__(TT *TR) k_motion_begin(Fields f, const float *
 }
 
 // End of a motion sweep in accumulator form: alpha_c = alpha_c0 + D_c x SV and
-// beta_c = beta_c0 + D_c x SF in place, with SV(P) = svt(P^T), SF(P) = sft(P^T) (the ratio
-// backprojector keeps the accumulators transposed).
-__global__ void __launch_bounds__(TT *TR) k_motion_finish(const float *__restrict__ svt,
-                                                         const float *__restrict__ sft,
+// beta_c = beta_c0 + D_c x SF in place, with (SV(P), SF(P)) = acc[P^T] (the ratio backprojector
+// keeps the accumulators transposed and interleaved).
+__global__ void __launch_bounds__(TT *TR) k_motion_finish(const float2 *__restrict__ acc,
                                                          const float *__restrict__ x, FieldsW f,
                                                          int nx, int ny, int nz) {
     __shared__ float t[2][TT][TT + 1];
     const int X0 = blockIdx.x * TT, Y0 = blockIdx.y * TT, i2 = blockIdx.z;
     const int tx = threadIdx.x, ty = threadIdx.y;
     const int plane = nx * ny;
-    load_transposed(t[0], svt, nx, ny, X0, Y0, i2);
-    load_transposed(t[1], sft, nx, ny, X0, Y0, i2);
+    // the tile of acc whose natural coordinates are (dim0 = Y0 + i, dim1 = X0 + j) into
+    // t[.][j][i], as load_transposed does for one volume
+#pragma unroll
+    for (int k = 0; k < TK; ++k) {
+        const int r = ty + k * TR;
+        const int d0 = Y0 + tx, d1 = X0 + r;
+        const float2 v =
+            (d0 < nx && d1 < ny) ? acc[d0 + d1 * nx + i2 * plane] : make_float2(0.f, 0.f);
+        t[0][r][tx] = v.x;
+        t[1][r][tx] = v.y;
+    }
     float *const F[6] = {f.ax, f.ay, f.az, f.bx, f.by, f.bz};
     float a[TK][6];
     Diffs d[TK];
@@ -831,10 +840,10 @@ int launch_motion_begin(const Fields &f, const float *x, float *cst, float2 *pai
     return MCSART_OK;
 }
 
-int launch_motion_finish(const float *svt, const float *sft, const float *x, const FieldsW &f,
-                         int nx, int ny, int nz, hipStream_t st) {
-    hipLaunchKernelGGL(k_motion_finish, tile_grid(nx, ny, nz), dim3(TT, TR), 0, st, svt, sft, x,
-                       f, nx, ny, nz);
+int launch_motion_finish(const float2 *acc, const float *x, const FieldsW &f, int nx, int ny,
+                         int nz, hipStream_t st) {
+    hipLaunchKernelGGL(k_motion_finish, tile_grid(nx, ny, nz), dim3(TT, TR), 0, st, acc, x, f, nx,
+                       ny, nz);
     MCSART_CHECK_LAUNCH();
     return MCSART_OK;
 }
