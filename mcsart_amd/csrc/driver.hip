@@ -32,7 +32,7 @@ struct mcsart_job {
     float *d_w1T = nullptr;      // 2N: the same in the xy-transposed layout / transposed image
     float *d_w2 = nullptr;       // N: u
     float *d_acc = nullptr;      // 2N: motion-sweep accumulators, transposed, as (SVt, SFt) pairs
-    float *d_cst = nullptr;      // 3N: motion-sweep constants (Ga, Gb, H) per voxel
+    float *d_cst = nullptr;      // 4N: motion-sweep constants and image (Ga, Gb, H, x) per voxel
     float2 *d_rp = nullptr;      // D
     float *d_norm = nullptr;     // n_iter*V
     float *d_resid = nullptr;    // V*D (keep_residual)
@@ -82,7 +82,7 @@ Layout layout(const Geom &g, int V, int n_iter, int keep_residual) {
     L.w1T = take(sizeof(float) * N * 2);
     L.w2 = take(sizeof(float) * N);
     L.acc = take(sizeof(float) * N * 2);
-    L.cst = take(sizeof(float) * N * 3);
+    L.cst = take(sizeof(float) * N * 4);
     L.rp = take(sizeof(float2) * D);
     L.norm = take(sizeof(float) * (size_t)n_iter * V);
     L.resid = take(keep_residual ? sizeof(float) * D * V : 0);
@@ -214,7 +214,8 @@ int motion_sweep(mcsart_job *j, int it) {
             const int nv = iv + stride;
             const bool more = nv < j->V;
             const ViewInfo &vn = j->vinfo[more ? nv : iv];
-            const MotionStep m{reinterpret_cast<float2 *>(j->d_acc), j->d_cst, j->d_img,
+            const MotionStep m{reinterpret_cast<float2 *>(j->d_acc),
+                               reinterpret_cast<const float4 *>(j->d_cst),
                                more && vn.has_y ? pair : nullptr,
                                more && vn.has_x ? pairT : nullptr,
                                j->volumes[iv - 1] - j->volumes[iv], j->flows[iv - 1] - j->flows[iv],
@@ -631,7 +632,8 @@ extern "C" int mcsart_job_kernel_time(mcsart_job *j, int kind, int reps, double 
                                       // the prep of view iv + 1 (views 1..V-2)
                 if (iv == 0 || iv + 1 >= j->V) continue;
                 const ViewInfo &vn = j->vinfo[iv + 1];
-                const MotionStep m{reinterpret_cast<float2 *>(j->d_acc), j->d_cst, j->d_img,
+                const MotionStep m{reinterpret_cast<float2 *>(j->d_acc),
+                                   reinterpret_cast<const float4 *>(j->d_cst),
                                    vn.has_y ? pair : nullptr, vn.has_x ? pairT : nullptr,
                                    j->volumes[iv - 1] - j->volumes[iv],
                                    j->flows[iv - 1] - j->flows[iv],
@@ -765,7 +767,8 @@ extern "C" int mcsart_job_autotune(mcsart_job *j, mcsart_tuning *chosen) {
             for (int iv0 : views) {
                 const int iv = std::max(iv0, 1), nv = std::min(iv + 1, j->V - 1);
                 const ViewInfo &vn = j->vinfo[nv];
-                const MotionStep m{reinterpret_cast<float2 *>(j->d_acc), j->d_cst, j->d_img,
+                const MotionStep m{reinterpret_cast<float2 *>(j->d_acc),
+                                   reinterpret_cast<const float4 *>(j->d_cst),
                                    vn.has_y ? pair : nullptr, vn.has_x ? pairT : nullptr,
                                    j->volumes[iv - 1] - j->volumes[iv],
                                    j->flows[iv - 1] - j->flows[iv], j->volumes[nv] - j->volumes[iv],

@@ -146,12 +146,12 @@ struct BackTune {
 //   SVt(p) += wv u(p), SFt(p) += wf u(p)                (update of view i, from zero if `first`)
 //   img0(p) = vd Ga(p) + fd Gb(p) + H(p) (vd SVt(p) + fd SFt(p))
 //   g(p) = x(p) - img0(p)                               (prep of view i+1, unless `last`)
-// with cst = (Ga, Gb, H) per voxel (3 floats) from launch_motion_begin, which also wrote the
-// sweep's Q (the projector scales P(Q) by -vd - fd); u itself is not stored.
+// with cst = (Ga, Gb, H, x) per voxel (one 16-byte record: the constants and the sweep's image)
+// from launch_motion_begin, which also wrote the sweep's Q (the projector scales P(Q) by
+// -vd - fd); u itself is not stored.
 struct MotionStep {
-    float2 *acc;   // (SVt, SFt) per voxel
-    const float *cst;
-    const float *x;
+    float2 *acc;         // (SVt, SFt) per voxel
+    const float4 *cst;   // (Ga, Gb, H, x) per voxel
     float2 *pair, *pairT;   // the g halves of the (g, Q) pairs the next view reads (either
                             // may be null); Q stays as launch_motion_begin wrote it
     float wv, wf, vd, fd;
@@ -219,7 +219,7 @@ int launch_motion_step(float *const fields[6], const float *u, const float *x, f
                        float fd, bool update, bool prep, hipStream_t st);
 
 // The motion sweep in accumulator form (x constant over the sweep): per-sweep constants
-// cst = (Ga, Gb, H) (3 floats per voxel) from the starting fields, and pairs (g, Q) in both
+// cst = (Ga, Gb, H, x) (4 floats per voxel) from the starting fields, and pairs (g, Q) in both
 // layouts: Q = (D1 + D2 + D3) x and the prep g of the first view ...
 int launch_motion_begin(const Fields &f, const float *x, float *cst, float2 *pair,
                         float2 *pairT, int nx, int ny, int nz, float vd, float fd,

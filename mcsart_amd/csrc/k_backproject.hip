@@ -74,21 +74,16 @@ __device__ inline float2 det_at(const void *base, unsigned off) {
 // transposed accumulators and the next view's prep at the voxel itself, instead of storing u.
 // The voxel's inputs are loaded before its backprojection (MotionIn), so their latency hides
 // behind the row loop.
-struct MotionCst {
-    float ga, gb, h;   // 12 bytes per voxel: one 3-dword load
-};
 struct MotionIn {
-    float sv, sf, x;
-    MotionCst c;
+    float sv, sf;
+    float4 c;   // (Ga, Gb, H, x): one 16-byte load
 };
 __device__ inline MotionIn motion_load(const MotionStep &m, int id) {
     MotionIn in;
     const float2 a = m.first ? make_float2(0.f, 0.f) : m.acc[id];
     in.sv = a.x;
     in.sf = a.y;
-    in.c = m.last ? MotionCst{0.f, 0.f, 0.f}
-                  : *reinterpret_cast<const MotionCst *>(m.cst + 3u * (unsigned)id);
-    in.x = m.last ? 0.f : m.x[id];
+    in.c = m.last ? make_float4(0.f, 0.f, 0.f, 0.f) : m.cst[id];
     return in;
 }
 __device__ inline void motion_step_at(const MotionStep &m, const MotionIn &in, int id, int idT,
@@ -97,11 +92,11 @@ __device__ inline void motion_step_at(const MotionStep &m, const MotionIn &in, i
     const float sf = in.sf + m.wf * u;
     m.acc[id] = make_float2(sv, sf);
     if (m.last) return;
-    const MotionCst c = in.c;
-    const float img0 = m.vd * c.ga + m.fd * c.gb + c.h * (m.vd * sv + m.fd * sf);
+    const float4 c = in.c;   // (Ga, Gb, H, x)
+    const float img0 = m.vd * c.x + m.fd * c.y + c.z * (m.vd * sv + m.fd * sf);
     // only the g half of the pair: the next view's q = Q (-vd - fd) is the sweep's Q, scaled
     // inside the projector
-    const float gv = in.x - img0;
+    const float gv = c.w - img0;
     if (m.pair) reinterpret_cast<float *>(m.pair)[2u * (unsigned)id] = gv;
     if (m.pairT) reinterpret_cast<float *>(m.pairT)[2u * (unsigned)idT] = gv;
 }

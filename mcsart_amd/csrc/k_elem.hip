@@ -590,10 +590,7 @@ __global__ void __launch_bounds__(TT *TR) k_motion_begin(Fields f, const float *
         const float gb = t[4][tx][r] * d.d1 + t[3][tx][r] * d.d2 + t[5][tx][r] * d.d3;
         const float h = t[6][tx][r] * d.d1 + t[7][tx][r] * d.d2 + t[8][tx][r] * d.d3;
         if (jok[k]) {
-            float *c = cst + 3u * (unsigned)jd[k];
-            c[0] = ga;
-            c[1] = gb;
-            c[2] = h;
+            reinterpret_cast<float4 *>(cst)[jd[k]] = make_float4(ga, gb, h, d.x);
             pair[jd[k]] = make_float2(d.x - (vd * ga + fd * gb), d.d1 + d.d2 + d.d3);
         }
         gv[k] = d.x - (vd * ga + fd * gb);
