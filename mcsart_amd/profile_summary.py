@@ -53,7 +53,7 @@ def algorithmic_bytes(kernel: str, N: int, D: int):
         "k_motion_step<true, true>": 72 * N,    # 6 fields read+written, u, x, pairs
         "k_motion_step<true, false>": 56 * N,
         "k_motion_step<false, true>": 40 * N,
-        "k_motion_begin": 52 * N,               # fields, x read; constants, Q x2, g written
+        "k_motion_begin": 56 * N,               # fields, x read; (Ga, Gb, H, x), Q x2, g written
         "k_motion_finish": 60 * N,              # fields read+written, accumulators, x
         "k_transpose": 8 * N,
     }

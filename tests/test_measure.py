@@ -34,7 +34,7 @@ def test_profile_byte_counts():
     # the warps carry their lane depth as a second template argument
     assert ab("k_warp_tile<true, 32>", N, D) == 32 * N
     assert ab("k_warp_add_tile<false, 32>", N, D) == 36 * N
-    assert ab("k_motion_begin", N, D) == 52 * N
+    assert ab("k_motion_begin", N, D) == 56 * N
     assert ab("k_no_such_kernel", N, D) is None
 
 
