@@ -271,7 +271,7 @@ __device__ inline float clampf(float x, float lo, float hi) {
 template <int W>
 __device__ inline void window(float lo, float hi, float top, float (&w)[W], float &sw, float &f0) {
     f0 = clampf(floorf(lo), 0.f, top);   // window start, inside [0, n - W]
-    float F = fmaxf(f0, lo);              // F(f0), = lo: f0 <= lo
+    float F = lo;   // F(f0) = max(f0, lo) = lo: 0 <= lo, so f0 <= floor(lo) <= lo
     const float F0 = F;
 #pragma unroll
     for (int k = 0; k < W; ++k) {
