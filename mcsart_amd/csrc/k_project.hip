@@ -392,7 +392,7 @@ __device__ inline void rows_step(const char *__restrict__ vb, const RayT &T, con
     }
 }
 
-template <int MODE, int WT, int WZ, int R>
+template <int MODE, int WT, int WZ, int R, bool SQ>
 __global__ void __launch_bounds__(RAYS * 16) k_project_rows(GeomArgs g, double c, double s,
                                                            ProjArgs a, int nsplit) {
     const int tx = threadIdx.x, ty = threadIdx.y;
@@ -426,15 +426,18 @@ __global__ void __launch_bounds__(RAYS * 16) k_project_rows(GeomArgs g, double c
         // gather is that wave-uniform base plus a 32-bit byte offset
         constexpr int LOGES = MODE == PROJ_MOTION ? 3 : 2;
         const bool xb = T.xbranch != 0;
-        const int np = xb ? g.nx : g.ny;
-        const int nt = xb ? g.ny : g.nx;
+        // SQ (nx == ny): both branches walk the same number of planes, a kernel argument, so the
+        // plane loop's bounds and counter stay scalar
+        const int np = SQ ? g.nx : (xb ? g.nx : g.ny);
+        const int nt = SQ ? g.nx : (xb ? g.ny : g.nx);
         const unsigned sz = (unsigned)g.nx * (unsigned)g.ny;
         const unsigned spB = (unsigned)nt << LOGES;   // plane stride in bytes
         const float p0 = 0.5f - 0.5f * (float)np;   // (float)ip + 0.5 - n/2 (:52, :105)
         const float ntf = (float)nt, nzf = (float)g.nz;
         const float ttop = (float)(nt - WT), ztop = (float)(g.nz - WZ);
         const int chunk = (np + nsplit - 1) / nsplit;
-        const int ip_lo = ty * chunk;
+        // a wavefront is one row of the block (64 lanes), so ty is uniform across it
+        const int ip_lo = __builtin_amdgcn_readfirstlane(ty) * chunk;
         const int ip_hi = min(np, ip_lo + chunk);
         // PL planes per step, all their gathers in flight together, the last ones one at a time.
         // One plane when a lane walks two or more rays: their gathers already overlap, and a
@@ -570,7 +573,12 @@ void launch_rows(const GeomArgs &ga, const ViewTrig &t, const ProjArgs &a, hipSt
     const dim3 block(RAYS, S);
     const dim3 grid((ga.na + RAYS - 1) / RAYS, (ga.nb + R - 1) / R);
     const size_t lds = S > 1 ? sizeof(float) * 2 * S * R * RAYS : 0;
-    hipLaunchKernelGGL((k_project_rows<MODE, WT, WZ, R>), grid, block, lds, st, ga, t.c, t.s, a, S);
+    if (ga.nx == ga.ny)
+        hipLaunchKernelGGL((k_project_rows<MODE, WT, WZ, R, true>), grid, block, lds, st, ga, t.c,
+                           t.s, a, S);
+    else
+        hipLaunchKernelGGL((k_project_rows<MODE, WT, WZ, R, false>), grid, block, lds, st, ga, t.c,
+                           t.s, a, S);
 }
 
 template <int MODE, int WT, int WZ>
