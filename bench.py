@@ -44,7 +44,7 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def algorithmic_bytes(spec, kind, layouts=None):
+def algorithmic_bytes(spec, kind, layouts=None, voxels=None):
     """HBM bytes one launch must move at minimum (4-byte floats).
     projector, image sweep : read x (N) and p (D), write (r, P1) (2D)      -> 4N + 12D
     projector, motion sweep: read g, Q (2N) and p (D), write (r, Pq) (2D)  -> 8N + 12D
@@ -61,14 +61,27 @@ def algorithmic_bytes(spec, kind, layouts=None):
                                                                               + 8D
     `layouts` (per view, bit 0 natural, bit 1 transposed) averages the per-layout terms over the
     views the kernel is launched for.
+
+    With culling (`voxels`: the per-view set sizes of mcsart_view_voxels, (V, 3): read set R_v,
+    voxels H_v whose ratio backprojection can be nonzero, H_v union R_{v+1}) the per-voxel terms
+    count only the voxels a launch processes:
+    warp                   : (28 + 4 per layout) R_v
+    warp-add               : 24N (fields, every voxel) + 12 H_v (u sample, x read and write where
+                             the sample can be nonzero; H_v stands for that set, which is H_v
+                             moved by the displacements) -- view 0 of a sweep is uncut: 36N
+    motion step (view v)   : 8 |H_v u R_{v+1}| (accumulators read) + 8 H_v (written)
+                             + (16 + 4 per layout) R_{v+1} (constants read, g written) + 8D
     """
     g = spec.geometry
     N, D = g.nvox, g.ndet
 
+    def n_layouts(i):
+        return 1.0 if layouts is None else float(bin(int(layouts[i]) & 3).count("1"))
+
     def mean_layouts(views):
         if layouts is None or len(views) == 0:
             return 1.0
-        return float(np.mean([bin(int(layouts[i]) & 3).count("1") for i in views]))
+        return float(np.mean([n_layouts(i) for i in views]))
 
     V = spec.n_views
     if kind == "proj_image":
@@ -77,12 +90,22 @@ def algorithmic_bytes(spec, kind, layouts=None):
         return 8 * N + 12 * D
     if kind == "back":
         return 4 * N + 8 * D
+    if voxels is None:
+        if kind == "warp":
+            return 28 * N + 4 * N * mean_layouts(range(V))
+        if kind == "warp_add":
+            return 36 * N
+        if kind == "motion":
+            return 32 * N + 4 * N * mean_layouts(range(2, V)) + 8 * D   # prep of views 2..V-1
+        raise KeyError(kind)
+    R, H, U = (voxels[:, k].astype(np.float64) for k in range(3))
     if kind == "warp":
-        return 28 * N + 4 * N * mean_layouts(range(V))
+        return float(np.mean([(28 + 4 * n_layouts(v)) * R[v] for v in range(V)]))
     if kind == "warp_add":
-        return 36 * N
-    if kind == "motion":
-        return 32 * N + 4 * N * mean_layouts(range(2, V)) + 8 * D   # prep of views 2..V-1
+        return float(np.mean([36 * N] + [24 * N + 12 * H[v] for v in range(1, V)]))
+    if kind == "motion":   # views 1..V-2, each preparing view v + 1
+        return float(np.mean([8 * U[v] + 8 * H[v] + (16 + 4 * n_layouts(v + 1)) * R[v + 1]
+                              + 8 * D for v in range(1, V - 1)]))
     raise KeyError(kind)
 
 
@@ -457,7 +480,7 @@ def main():
 
     from mcsart_amd import configs
     from mcsart_amd.configs import make_scan_inputs
-    from mcsart_amd.driver import Context, Job
+    from mcsart_amd.driver import Context, Job, view_voxels
     from mcsart_amd.simulate import measure
 
     spec = {"mcsart128": configs.c2_mcsart128, "clinical256": configs.c3_clinical}[args.workload]()
@@ -516,6 +539,7 @@ def main():
     # job's stream (no per-launch event overhead), for each kernel of the sweeps.  The dominant
     # kernel is the one with the largest share of a scan (average launch x launches per scan).
     layouts = job.view_layouts()
+    voxels = view_voxels(spec.geometry, spec.angles)   # sizes of the culled sets per view
     n_img_views, n_mot_views = launches_per_scan(spec)
     n_warps = n_img_views if spec.motion else 0
     per_scan = {"projector": n_img_views, "projector_motion": n_mot_views,
@@ -528,7 +552,7 @@ def main():
     log(f"[bench] isolated launches: {iso_us}")
     kernels_roofline = {}
     for k, us in iso_us.items():
-        nbytes = algorithmic_bytes(spec, byte_kind[k], layouts)
+        nbytes = algorithmic_bytes(spec, byte_kind[k], layouts, voxels)
         ach = nbytes / (us * 1e-6) / 1e9
         kernels_roofline[k] = {
             "avg_launch_us": round(us, 2), "bytes_per_launch": int(nbytes),

@@ -334,6 +334,24 @@ int mcsart_job_run(mcsart_job *job);
  * (environment: MCSART_GRAPHS=0 sets the default for new jobs). */
 int mcsart_job_run_sweep(mcsart_job *job, int it);
 int mcsart_job_set_graphs(mcsart_job *job, int enable);
+/* Work the sweeps skip because no later kernel can see it (no counterpart in the reference, which
+ * processes every voxel every view): a view's warp writes only the voxels its projector can read
+ * (those whose box, grown by the projector's window, meets the cone from the source to the
+ * detector's edges, and the volume's edge shell that clamped windows read); the motion step
+ * prepares g only there and updates the accumulators only where u can be nonzero; the warp-add
+ * leaves x alone where W(u) is 0 because u is 0 at every voxel its sample reads.  Results are the
+ * same bit for bit.  mode 0: off; 1: on (the default); 2: on, with every skipped write made as a
+ * NaN instead (a check that no kernel reads a skipped cell: any such read would show in the
+ * results).  Drops the captured sweeps. */
+int mcsart_job_set_culling(mcsart_job *job, int mode);
+/* Sizes of the culled sets of a scan, per view v (counts[3*n_views]), for the algorithmic bytes
+ * of the culled kernels: counts[3v] voxels in view v's read set (its warp's writes),
+ * counts[3v+1] voxels whose ratio backprojection of view v has cells and rows (u can be nonzero:
+ * the motion step's accumulator writes), counts[3v+2] the union of that set and view v+1's read
+ * set (the motion step's accumulator reads; for the last view, counts[3v+1]).  Host arithmetic
+ * only (no device needed). */
+int mcsart_view_voxels(const mcsart_geom *g, int n_views, const float *angles,
+                       long long *counts);
 int mcsart_job_get_image(mcsart_job *job, float *h_img);
 int mcsart_job_get_norms(mcsart_job *job, float *h_norm);                /* n_iter*n_views */
 int mcsart_job_get_residual(mcsart_job *job, float *h_proj);             /* n_views*na*nb */

@@ -63,6 +63,8 @@ SIGNATURES = {
     "mcsart_job_run": ([_P], _I),
     "mcsart_job_run_sweep": ([_P, _I], _I),
     "mcsart_job_set_graphs": ([_P, _I], _I),
+    "mcsart_job_set_culling": ([_P, _I], _I),
+    "mcsart_view_voxels": ([_P, _I, _P, _P], _I),
     "mcsart_job_get_image": ([_P, _P], _I),
     "mcsart_job_get_norms": ([_P, _P], _I),
     "mcsart_job_get_residual": ([_P, _P], _I),

@@ -7,7 +7,9 @@
 // HBM for the whole schedule, and enqueues each view as four kernels on the context's stream
 // with no host round trip: the norms are reduced on the device into a per-view array.
 #include <algorithm>
+#include <cmath>
 #include <cstring>
+#include <thread>
 #include <vector>
 
 #include "internal.hpp"
@@ -50,6 +52,8 @@ struct mcsart_job {
     };
     std::vector<SweepGraph> graphs;
     bool use_graphs = true;
+    int cull = 1;   // 0: every voxel every view; 1: skip what the views never see; 2: as 1, with
+                    // the skipped cells written as NaN (mcsart_job_set_culling)
     int proj_split = 0, proj_rows = 0;   // projector launch shape (0 = automatic)
     BackTune back_tune{0, 0};   // backprojector launch shape (0 = automatic)
     int back_zslab_motion = 0;  // z-slab of the motion sweep's backprojector (0 = automatic)
@@ -108,6 +112,44 @@ Fields fields_of(const mcsart_job *j) {
     return Fields{f, f + N, f + 2 * N, f + 3 * N, f + 4 * N, f + 5 * N};
 }
 
+GeomArgs geom_args(const Geom &g) {
+    return GeomArgs{g.nx, g.ny, g.nz, g.na, g.nb, g.da, g.db, g.ai, g.bi, g.SO, g.SD};
+}
+
+// Voxels the projector of view iv can read (ConeCull), under the job's cull mode.
+ConeCull read_set(const mcsart_job *j, int iv) {
+    return projector_read_set(geom_args(j->g), j->trig[iv].c, j->trig[iv].s, j->vinfo[iv],
+                              j->cull);
+}
+
+// Voxels where view iv's ratio backprojection u can be nonzero (ConeCull); mode 0 on the first
+// view of an image sweep, whose warp-add then clamps every voxel (the skip needs x clamped).
+ConeCull u_support(const mcsart_job *j, int iv) {
+    return backprojection_support(geom_args(j->g), j->trig[iv].c, j->trig[iv].s,
+                                  (j->cull && iv > 0) ? 1 : 0);
+}
+
+// The motion step of view iv, preparing view nv (nv >= V: the sweep's last view).
+MotionStep motion_step(const mcsart_job *j, int iv, int nv, float2 *pair, float2 *pairT,
+                       int first) {
+    const bool more = nv < j->V;
+    const ViewInfo &vn = j->vinfo[more ? nv : iv];
+    const float wv = j->volumes[iv - 1] - j->volumes[iv], wf = j->flows[iv - 1] - j->flows[iv];
+    MotionStep m{reinterpret_cast<float2 *>(j->d_acc),
+                 reinterpret_cast<const float4 *>(j->d_cst),
+                 more && vn.has_y ? pair : nullptr,
+                 more && vn.has_x ? pairT : nullptr,
+                 wv,
+                 wf,
+                 more ? j->volumes[nv] - j->volumes[nv - 1] : 0.f,
+                 more ? j->flows[nv] - j->flows[nv - 1] : 0.f,
+                 first,
+                 more ? 0 : 1,
+                 more ? read_set(j, nv) : ConeCull{},
+                 (j->cull && std::isfinite(wv) && std::isfinite(wf)) ? 1 : 0};
+    return m;
+}
+
 // Image sweep (src/AUMISART.cu:182-235; static form :40-65).
 int image_sweep(mcsart_job *j, int it) {
     const Geom &g = j->g;
@@ -128,9 +170,10 @@ int image_sweep(mcsart_job *j, int it) {
         ProjArgs a{};
         if (motion) {   // x_i = W(x; v_i - v_0, f_i - f_0)  (:193), in the layouts P reads
             TimedRegion tr(j->ctx, TK_WARP);
+            const ConeCull rs = read_set(j, iv);
             rc = launch_warp(vi.has_y ? j->d_w1 : nullptr, vi.has_x ? j->d_w1T : nullptr,
                              j->d_img, g.nx, g.ny, g.nz, j->volumes[iv] - j->volumes[0],
-                             j->flows[iv] - j->flows[0], f, st);
+                             j->flows[iv] - j->flows[0], f, st, &rs);
             if (rc) return rc;
             a.vol0 = j->d_w1;
         } else {
@@ -162,9 +205,10 @@ int image_sweep(mcsart_job *j, int it) {
         }
         if (motion) {   // x += lambda * W(u; v_0 - v_i, f_0 - f_i)  (:222-223)
             TimedRegion tr(j->ctx, TK_WARP);
+            const ConeCull sup = u_support(j, iv);
             rc = launch_warp_add(j->d_img, j->d_w2, g.nx, g.ny, g.nz,
                                  j->volumes[0] - j->volumes[iv], j->flows[0] - j->flows[iv], f,
-                                 j->opts.lambda, j->opts.nonneg, st);
+                                 j->opts.lambda, j->opts.nonneg, st, &sup);
             if (rc) return rc;
         }
     }
@@ -211,16 +255,7 @@ int motion_sweep(mcsart_job *j, int it) {
         }
         {   // u = B(r) / B(P q) (:265-270), rho_i; fields += u^T * (-vd | -fd) * D_c x
             // (:272-277) as accumulators; then the next view's prep
-            const int nv = iv + stride;
-            const bool more = nv < j->V;
-            const ViewInfo &vn = j->vinfo[more ? nv : iv];
-            const MotionStep m{reinterpret_cast<float2 *>(j->d_acc),
-                               reinterpret_cast<const float4 *>(j->d_cst),
-                               more && vn.has_y ? pair : nullptr,
-                               more && vn.has_x ? pairT : nullptr,
-                               j->volumes[iv - 1] - j->volumes[iv], j->flows[iv - 1] - j->flows[iv],
-                               more ? vdiff(nv) : 0.f, more ? fdiff(nv) : 0.f, iv == 1 ? 1 : 0,
-                               more ? 0 : 1};
+            const MotionStep m = motion_step(j, iv, iv + stride, pair, pairT, iv == 1 ? 1 : 0);
             const NormFinish nf{j->d_partials, nparts, j->d_norm + (long long)it * j->V + iv};
             const BackTune bt{j->back_zslab_motion, j->back_tune.stage};
             TimedRegion tr(j->ctx, TK_BACK);
@@ -429,6 +464,133 @@ int capture_sweep(mcsart_job *j, int it) {
 
 }  // namespace
 
+namespace {
+
+// First index in [0, n) where a predicate that is false..false true..true holds (n if none).
+template <class P>
+int first_true(int n, P pred) {
+    int lo = 0, hi = n;
+    while (lo < hi) {
+        const int mid = lo + (hi - lo) / 2;
+        if (pred(mid)) hi = mid;
+        else lo = mid + 1;
+    }
+    return lo;
+}
+
+struct ZRange {
+    int lo, hi;   // [lo, hi)
+};
+
+// Number of z indices covered by the union of k ranges.
+int union_length(ZRange *r, int k) {
+    std::sort(r, r + k, [](const ZRange &a, const ZRange &b) { return a.lo < b.lo; });
+    int total = 0, end = 0;
+    for (int i = 0; i < k; ++i) {
+        const int lo = std::max(r[i].lo, end), hi = r[i].hi;
+        if (hi > lo) {
+            total += hi - lo;
+            end = hi;
+        }
+    }
+    return total;
+}
+
+// The z ranges of column (ix, iy) in a read set (cone_keep): up to three.
+int read_set_ranges(const ConeCull &C, int ix, int iy, ZRange *out) {
+    const int nz = C.nz;
+    if (ix < C.sx || ix >= C.nx - C.sx || iy < C.sx || iy >= C.ny - C.sx) {
+        out[0] = ZRange{0, nz};
+        return 1;
+    }
+    const float x = (float)ix, y = (float)iy;
+    if (!(cone_form(C, 0, x, y, 0.f) >= 0.f && cone_form(C, 1, x, y, 0.f) >= 0.f)) return 0;
+    int k = 0;
+    out[k++] = ZRange{0, std::min(C.sz, nz)};
+    out[k++] = ZRange{std::max(nz - C.sz, 0), nz};
+    // form 2 grows with z, form 3 falls
+    const int lo = first_true(nz, [&](int iz) { return cone_form(C, 2, x, y, (float)iz) >= 0.f; });
+    const int hi = first_true(nz, [&](int iz) { return cone_form(C, 3, x, y, (float)iz) < 0.f; });
+    out[k++] = ZRange{lo, std::max(lo, hi)};
+    return k;
+}
+
+// The z range of column (ix, iy) whose ratio backprojection has cells and rows (u can be
+// nonzero): src/cu_backprojection.cu:150-169 with the bounds of column_setup / column_b.
+ZRange hit_range(const GeomArgs &ga, double c, double s, int ix, int iy) {
+    Column C;
+    column_setup(ga, c, s, ix, iy, C);
+    const double ia_lo = std::fmin(std::fmax(std::floor(C.Amin), 0.0), (double)ga.na);
+    if (!(ia_lo < ga.na && C.Amax > ia_lo)) return ZRange{0, 0};
+    auto bounds = [&](int iz, double &lo, double &hi) { column_b(C, ga.nz, iz, lo, hi); };
+    // both bounds grow with z
+    const int lo = first_true(ga.nz, [&](int iz) {
+        double b0, b1;
+        bounds(iz, b0, b1);
+        return b1 > 0.0;
+    });
+    const int hi = first_true(ga.nz, [&](int iz) {
+        double b0, b1;
+        bounds(iz, b0, b1);
+        return b0 >= (double)ga.nb;
+    });
+    return ZRange{lo, std::max(lo, hi)};
+}
+
+}  // namespace
+
+extern "C" int mcsart_view_voxels(const mcsart_geom *gp, int n_views, const float *angles,
+                                  long long *counts) {
+    if (!gp || !angles || !counts) return fail(MCSART_EINVAL, "null argument");
+    if (n_views <= 0) return fail(MCSART_EINVAL, "n_views must be positive");
+    const Geom g = geom_of(gp);
+    int rc = validate_geom(g);
+    if (rc) return rc;
+    const GeomArgs ga = geom_args(g);
+    std::vector<ViewTrig> trig(n_views);
+    std::vector<ConeCull> rs(n_views);
+    for (int v = 0; v < n_views; ++v) {
+        trig[v] = view_trig(angles[v]);
+        rs[v] = projector_read_set(ga, trig[v].c, trig[v].s, view_info_of(g, trig[v]), 1);
+    }
+    auto count_view = [&](int v) {
+        long long nread = 0, nhit = 0, nunion = 0;
+        for (int iy = 0; iy < g.ny; ++iy)
+            for (int ix = 0; ix < g.nx; ++ix) {
+                ZRange r[4];
+                const int k = read_set_ranges(rs[v], ix, iy, r);
+                nread += union_length(r, k);
+                ZRange u[4];
+                u[0] = hit_range(ga, trig[v].c, trig[v].s, ix, iy);
+                nhit += u[0].hi - u[0].lo;
+                const int kn = v + 1 < n_views ? read_set_ranges(rs[v + 1], ix, iy, u + 1) : 0;
+                nunion += union_length(u, 1 + kn);
+            }
+        counts[3 * (long long)v] = nread;
+        counts[3 * (long long)v + 1] = nhit;
+        counts[3 * (long long)v + 2] = nunion;
+    };
+    // views on up to 16 host threads (each writes its own entries)
+    const int nt = std::max(1, std::min({16, n_views, (int)std::thread::hardware_concurrency()}));
+    std::vector<std::thread> pool;
+    for (int t = 0; t < nt; ++t)
+        pool.emplace_back([&, t] {
+            for (int v = t; v < n_views; v += nt) count_view(v);
+        });
+    for (auto &th : pool) th.join();
+    return MCSART_OK;
+}
+
+extern "C" int mcsart_job_set_culling(mcsart_job *j, int mode) {
+    int rc = check_job(j);
+    if (rc) return rc;
+    if (mode < 0 || mode > 2) return fail(MCSART_EINVAL, "culling mode must be 0, 1 or 2");
+    if (mode != j->cull)
+        for (auto &sg : j->graphs) drop_graph(sg);   // captured launches carry the old mode
+    j->cull = mode;
+    return MCSART_OK;
+}
+
 extern "C" int mcsart_job_set_graphs(mcsart_job *j, int enable) {
     int rc = check_job(j);
     if (rc) return rc;
@@ -621,24 +783,19 @@ extern "C" int mcsart_job_kernel_time(mcsart_job *j, int kind, int reps, double 
             const ViewInfo &vi = j->vinfo[iv];
             int r;
             if (kind == 2) {   // x_i = W(x; v_i - v_0, f_i - f_0) in the layouts P reads
+                const ConeCull rs = read_set(j, iv);
                 r = launch_warp(vi.has_y ? j->d_w1 : nullptr, vi.has_x ? j->d_w1T : nullptr,
                                 j->d_img, g.nx, g.ny, g.nz, j->volumes[iv] - j->volumes[0],
-                                j->flows[iv] - j->flows[0], f, st);
+                                j->flows[iv] - j->flows[0], f, st, &rs);
             } else if (kind == 3) {   // x += lambda W(u; v_0 - v_i, f_0 - f_i), on the copy
+                const ConeCull sup = u_support(j, iv);
                 r = launch_warp_add(j->d_w1, j->d_w2, g.nx, g.ny, g.nz,
                                     j->volumes[0] - j->volumes[iv], j->flows[0] - j->flows[iv],
-                                    f, j->opts.lambda, j->opts.nonneg, st);
+                                    f, j->opts.lambda, j->opts.nonneg, st, &sup);
             } else if (kind == 4) {   // ratio backprojection of view iv with its update and
                                       // the prep of view iv + 1 (views 1..V-2)
                 if (iv == 0 || iv + 1 >= j->V) continue;
-                const ViewInfo &vn = j->vinfo[iv + 1];
-                const MotionStep m{reinterpret_cast<float2 *>(j->d_acc),
-                                   reinterpret_cast<const float4 *>(j->d_cst),
-                                   vn.has_y ? pair : nullptr, vn.has_x ? pairT : nullptr,
-                                   j->volumes[iv - 1] - j->volumes[iv],
-                                   j->flows[iv - 1] - j->flows[iv],
-                                   j->volumes[iv + 1] - j->volumes[iv],
-                                   j->flows[iv + 1] - j->flows[iv], 0, 0};
+                const MotionStep m = motion_step(j, iv, iv + 1, pair, pairT, 0);
                 const BackTune bt{j->back_zslab_motion, j->back_tune.stage};
                 r = launch_backprojector(g, j->trig[iv], vi, BACK_RATIO, nullptr, j->d_rp,
                                          nullptr, st, nullptr, &bt, &m);

@@ -295,12 +295,14 @@ __host__ __device__ inline int clampi(int v, int lo, int hi) { return v < lo ? l
 // ------------------------------------------------------------------------------------------
 struct ViewInfo {
     int has_x, has_y;
-    int wt, wz;   // projector windows
-    int wa;       // backprojector transverse window (cells)
+    int wt, wz;      // projector windows
+    int wa;          // backprojector transverse window (cells)
+    double t_box;    // how far (voxels) a projector footprint's transverse bounds reach past the
+                     // ray's own pyramid: the four-ratio box against the two edge ratios
 };
 
 inline ViewInfo view_info(const GeomArgs &g, double c, double s) {
-    ViewInfo v{0, 0, 0, 0, 0};
+    ViewInfo v{0, 0, 0, 0, 0, 0.0};
     double wt_max = 0.0, wz_max = 0.0;
     const double SO = g.SO, SD = g.SD, da = g.da, db = g.db;
     for (int ia = 0; ia < g.na; ++ia) {
@@ -325,11 +327,18 @@ inline ViewInfo view_info(const GeomArgs &g, double c, double s) {
             np = g.ny;
             v.has_y = 1;
         }
-        const double dr = fmax(fmax(r0, r1), fmax(r2, r3)) - fmin(fmin(r0, r1), fmin(r2, r3));
+        const double rmin = fmin(fmin(r0, r1), fmin(r2, r3));
+        const double rmax = fmax(fmax(r0, r1), fmax(r2, r3));
+        const double dr = rmax - rmin;
         const double p0 = 0.5 - 0.5 * np, p1 = p0 + (np - 1);
         const double xc = fmax(fabs(p0 + dist), fabs(p1 + dist));
         wt_max = fmax(wt_max, dr * xc);
         wz_max = fmax(wz_max, zslope * xc);
+        // the cell's edge rays run through (xr, yl) and (xl, yr): their ratios bound the ray's
+        // pyramid; the box of four ratios reaches past them by at most this much in any plane
+        const double e0 = xb ? yl / xr : xr / yl, e1 = xb ? yr / xl : xl / yr;
+        const double excess = fmax(fmin(e0, e1) - rmin, rmax - fmax(e0, e1));
+        v.t_box = fmax(v.t_box, excess * xc);
     }
     // a small margin absorbs the FP32 rounding of the per-plane bounds
     v.wt = (int)floor(wt_max * (1.0 + 1e-5) + 1e-5) + 2;
@@ -356,6 +365,120 @@ inline ViewInfo view_info(const GeomArgs &g, double c, double s) {
     // guard absorbs the device's own FP64 rounding of the same bounds)
     v.wa = (int)floor(wa_max * (1.0 + 1e-9) + 1e-9) + 2;
     return v;
+}
+
+// ------------------------------------------------------------------------------------------
+// The cone of a view: the points whose projection lands on the detector, i.e. detector
+// coordinates a in [(ai - 1/2) da, (na + ai - 1/2) da] and b in [(bi - 1/2) db, (nb + bi - 1/2) db]
+// at depth SD (the outer edges of the first and last cells, src/cu_projection.cu:28-36).  With
+// depth t = (P - S).e and lateral l = (P - S).u (e = (c, s), u = (-s, c), S the source), a point
+// is inside when the four linear forms SD l - a_lo t, a_hi t - SD l, SD z - b_lo t, b_hi t - SD z
+// are all >= 0.  ConeCull holds them normalised (voxel units) as n.(ix, iy, iz) + k with voxel
+// centres at i + 1/2 - floor(n/2), and k already raised by a box margin: a voxel passes when a
+// box around its centre meets the cone.
+//
+// Two uses:
+//  * the read set of a view's projector.  A window cell lies within W - 1/2 voxels (W = window
+//    size) of a point of the ray's footprint box in its plane; the box reaches t_box past the
+//    ray's pyramid transversely and, axially, b_max * (box width) / SD past it; the planes of odd
+//    sizes sit 1/2 voxel off the integer grid.  The margin covers all of it.  A window clamped at
+//    the volume's edge (a plane the ray misses, or a footprint outside [0, nz]) reads the edge
+//    shell instead: cells within W of a transverse edge (any z) and, in columns inside the fan,
+//    within W of a z edge.  Voxels outside the read set are never read by that view's projector.
+//  * the support of a ratio backprojection u = B(r)/B(P q): u is 0 exactly at voxels whose
+//    six-point footprint misses the detector (no cells or no rows, src/cu_backprojection.cu:
+//    150-169); the six points lie within 1/sqrt(2) (xy) and 1/2 (z) of the centre.
+// ------------------------------------------------------------------------------------------
+struct ConeCull {
+    float n[4][3];   // unit normals (x, y, z) of the four forms in voxel-index coordinates
+    float k[4];      // offsets, margin included
+    int sx, sz;      // edge shell: transverse cells < sx or >= n - sx; z cells < sz or >= nz - sz
+    int nx, ny, nz;
+    int mode;        // 0: every voxel passes; 1: cull; 2: cull, and write NaN where a write is
+                     // culled (a check that nothing reads those cells)
+};
+
+// Margin-raised form j at voxel index (x, y, z).
+__host__ __device__ inline float cone_form(const ConeCull &C, int j, float x, float y, float z) {
+    return fmaf(C.n[j][0], x, fmaf(C.n[j][1], y, fmaf(C.n[j][2], z, C.k[j])));
+}
+
+// Voxel (ix, iy, iz) is in the read set (or the cull is off).
+__host__ __device__ inline bool cone_keep(const ConeCull &C, int ix, int iy, int iz) {
+    if (C.mode == 0) return true;
+    if (ix < C.sx || ix >= C.nx - C.sx || iy < C.sx || iy >= C.ny - C.sx) return true;
+    const float x = (float)ix, y = (float)iy, z = (float)iz;
+    const bool fan = cone_form(C, 0, x, y, z) >= 0.f && cone_form(C, 1, x, y, z) >= 0.f;
+    const bool zin = (cone_form(C, 2, x, y, z) >= 0.f && cone_form(C, 3, x, y, z) >= 0.f) ||
+                     iz < C.sz || iz >= C.nz - C.sz;
+    return fan && zin;
+}
+
+// A box of centre (x, y, z) and half-widths (hx, hy, hz), on top of the built-in margin, meets
+// the cone (or the cull is off).
+__host__ __device__ inline bool cone_box_meets(const ConeCull &C, float x, float y, float z,
+                                               float hx, float hy, float hz) {
+    if (C.mode == 0) return true;
+    bool in = true;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const float m = fmaf(fabsf(C.n[j][0]), hx, fmaf(fabsf(C.n[j][1]), hy, fabsf(C.n[j][2]) * hz));
+        in = in && cone_form(C, j, x, y, z) + m >= 0.f;
+    }
+    return in;
+}
+
+// The cone's forms for a view, with box half-widths (ht, ht, hz) around each voxel centre built
+// into the offsets.  FP64 here; the device evaluates FP32 forms of magnitude ~n, far inside
+// the 0.05-voxel guard added to every offset.
+inline ConeCull cone_forms(const GeomArgs &g, double c, double s, double ht, double hz,
+                           int mode) {
+    ConeCull C{};
+    C.nx = g.nx;
+    C.ny = g.ny;
+    C.nz = g.nz;
+    C.mode = mode;
+    const double SO = g.SO, SD = g.SD;
+    const double alo = ((double)g.ai - 0.5) * g.da, ahi = ((double)g.na + g.ai - 0.5) * g.da;
+    const double blo = ((double)g.bi - 0.5) * g.db, bhi = ((double)g.nb + g.bi - 0.5) * g.db;
+    // t = c X + s Y + SO, l = -s X + c Y (S = -SO e); each form a.(X, Y, Z) + a0
+    const double a[4][3] = {{-SD * s - alo * c, SD * c - alo * s, 0.0},
+                            {ahi * c + SD * s, ahi * s - SD * c, 0.0},
+                            {-blo * c, -blo * s, SD},
+                            {bhi * c, bhi * s, -SD}};
+    const double a0[4] = {-alo * SO, ahi * SO, -blo * SO, bhi * SO};
+    const double o[3] = {0.5 - (double)(g.nx / 2), 0.5 - (double)(g.ny / 2),
+                         0.5 - (double)(g.nz / 2)};
+    const double h[3] = {ht, ht, hz};
+    for (int j = 0; j < 4; ++j) {
+        const double len = sqrt(a[j][0] * a[j][0] + a[j][1] * a[j][1] + a[j][2] * a[j][2]);
+        double k = a0[j];
+        for (int d = 0; d < 3; ++d) k += a[j][d] * o[d] + fabs(a[j][d]) * h[d];
+        for (int d = 0; d < 3; ++d) C.n[j][d] = (float)(a[j][d] / len);
+        C.k[j] = (float)(k / len + 0.05);
+    }
+    return C;
+}
+
+// Read set of the view's projector (first use above).
+inline ConeCull projector_read_set(const GeomArgs &g, double c, double s, const ViewInfo &vi,
+                                   int mode) {
+    const double bmax = fmax(fabs(((double)g.bi - 0.5) * g.db),
+                             fabs(((double)g.nb + g.bi - 0.5) * g.db));
+    const double ht = (vi.wt - 0.5) + vi.t_box + 0.5;
+    const double hz = (vi.wz - 0.5) + bmax * (vi.wt + vi.t_box + 0.5) / g.SD;
+    ConeCull C = cone_forms(g, c, s, ht, hz, mode);
+    C.sx = vi.wt;
+    C.sz = vi.wz;
+    return C;
+}
+
+// Support of the view's ratio backprojection (second use above); no shell.
+inline ConeCull backprojection_support(const GeomArgs &g, double c, double s, int mode) {
+    ConeCull C = cone_forms(g, c, s, 0.70710678118654752, 0.5, mode);
+    C.sx = 0;
+    C.sz = 0;
+    return C;
 }
 
 }  // namespace mcsart

@@ -149,6 +149,10 @@ struct BackTune {
 // with cst = (Ga, Gb, H, x) per voxel (one 16-byte record: the constants and the sweep's image)
 // from launch_motion_begin, which also wrote the sweep's Q (the projector scales P(Q) by
 // -vd - fd); u itself is not stored.
+// Work the next kernels never see is skipped: g only where the next view's projector can read it
+// (`next`, its read set: ConeCull), and, when `skip_zero` (wv and wf finite), the accumulators
+// only where u can be nonzero (the voxel's footprint meets the detector; elsewhere u = 0 and the
+// sums do not change).  A voxel with neither is not touched at all.
 struct MotionStep {
     float2 *acc;         // (SVt, SFt) per voxel
     const float4 *cst;   // (Ga, Gb, H, x) per voxel
@@ -156,6 +160,8 @@ struct MotionStep {
                             // may be null); Q stays as launch_motion_begin wrote it
     float wv, wf, vd, fd;
     int first, last;
+    ConeCull next;       // read set of the next view's projector (mode 0: every voxel)
+    int skip_zero;
 };
 // The static image sweep's update fused into the ratio backprojector (src/AUMISART.cu:62):
 //   x(p) = max(x(p) + lambda u(p), 0 if nonneg), and the same value at p^T in xT when given (the
@@ -180,13 +186,17 @@ struct FieldsW {   // the same six fields, writable
 };
 // out = W(src; vol, flow) in the natural layout and/or (outT) the xy-transposed layout; either
 // output may be null; neither may alias src.
+// read_set (optional): only the voxels the view's projector can read are written (ConeCull).
 int launch_warp(float *out, float *outT, const float *src, int nx, int ny, int nz, float vol,
-                float flow, const Fields &f, hipStream_t st);
+                float flow, const Fields &f, hipStream_t st, const ConeCull *read_set = nullptr);
 // dst = xy-transpose of src (dst[iy + ix*ny + iz*nx*ny] = src[ix + iy*nx + iz*nx*ny])
 int launch_transpose(float *dst, const float *src, int nx, int ny, int nz, hipStream_t st);
-// x += lambda * W(u; vol, flow), then optional clamp at zero.
+// x += lambda * W(u; vol, flow), then optional clamp at zero.  support (optional): the voxels
+// where u can be nonzero (ConeCull); x is left as it is where W(u) is 0 for that reason, which
+// with the clamp needs x already clamped everywhere.
 int launch_warp_add(float *x, const float *u, int nx, int ny, int nz, float vol, float flow,
-                    const Fields &f, float lambda, int nonneg, hipStream_t st);
+                    const Fields &f, float lambda, int nonneg, hipStream_t st,
+                    const ConeCull *support = nullptr);
 int launch_deform2(float *out, const float *src, int nx, int ny, int nz, float vol, float flow,
                    const Fields &f, hipStream_t st);
 // x += lambda * u, then optional clamp at zero.

@@ -74,29 +74,43 @@ __device__ inline float2 det_at(const void *base, unsigned off) {
 // transposed accumulators and the next view's prep at the voxel itself, instead of storing u.
 // The voxel's inputs are loaded before its backprojection (MotionIn), so their latency hides
 // behind the row loop.
+// Per voxel: `hit` when its footprint has cells and rows (otherwise u = 0 exactly), `need` when
+// the next view's projector can read its g (MotionStep::next).  The accumulators are read for
+// either, the constants for `need` only.
 struct MotionIn {
     float sv, sf;
     float4 c;   // (Ga, Gb, H, x): one 16-byte load
+    bool hit, need;
 };
-__device__ inline MotionIn motion_load(const MotionStep &m, int id) {
+__device__ inline MotionIn motion_load(const MotionStep &m, int id, int ix, int iy, int iz,
+                                       bool hit) {
     MotionIn in;
-    const float2 a = m.first ? make_float2(0.f, 0.f) : m.acc[id];
+    in.hit = hit || !m.skip_zero;
+    in.need = !m.last && cone_keep(m.next, ix, iy, iz);
+    const float2 a = (m.first || !(in.hit || in.need)) ? make_float2(0.f, 0.f) : m.acc[id];
     in.sv = a.x;
     in.sf = a.y;
-    in.c = m.last ? make_float4(0.f, 0.f, 0.f, 0.f) : m.cst[id];
+    in.c = in.need ? m.cst[id] : make_float4(0.f, 0.f, 0.f, 0.f);
     return in;
 }
 __device__ inline void motion_step_at(const MotionStep &m, const MotionIn &in, int id, int idT,
                                       float u) {
+    // u = 0 without a hit: sv + wv * 0 = sv, so the stored sums stand (the first view writes
+    // them everywhere: it starts them)
     const float sv = in.sv + m.wv * u;
     const float sf = in.sf + m.wf * u;
-    m.acc[id] = make_float2(sv, sf);
+    if (in.hit || m.first) m.acc[id] = make_float2(sv, sf);
     if (m.last) return;
-    const float4 c = in.c;   // (Ga, Gb, H, x)
-    const float img0 = m.vd * c.x + m.fd * c.y + c.z * (m.vd * sv + m.fd * sf);
     // only the g half of the pair: the next view's q = Q (-vd - fd) is the sweep's Q, scaled
     // inside the projector
-    const float gv = c.w - img0;
+    float gv = __builtin_nanf("");
+    if (in.need) {
+        const float4 c = in.c;   // (Ga, Gb, H, x)
+        const float img0 = m.vd * c.x + m.fd * c.y + c.z * (m.vd * sv + m.fd * sf);
+        gv = c.w - img0;
+    } else if (m.next.mode != 2) {
+        return;   // never read by the next view; written as NaN when checking that
+    }
     if (m.pair) reinterpret_cast<float *>(m.pair)[2u * (unsigned)id] = gv;
     if (m.pairT) reinterpret_cast<float *>(m.pairT)[2u * (unsigned)idT] = gv;
 }
@@ -256,10 +270,6 @@ __global__ void __launch_bounds__(BX *BY) k_backproject(GeomArgs g, double c, do
         };
         for (int iz = iz0; iz < iz1; ++iz) {
             const int id = ix + iy * g.nx + iz * plane;
-            MotionIn min_{};
-            float x_old = 0.f;
-            if (EPI == EPI_MOTION) min_ = motion_load(mot, id);
-            if (EPI == EPI_ADD) x_old = add.x[id];
             // axial bounds (:131-132,:137-138) in anchor-relative FP32
             const float d = (float)(iz - iz0);
             const float L0 = fmaf(bl.q0, d, bl.l0), L1 = fmaf(bl.q1, d, bl.l1);
@@ -272,6 +282,10 @@ __global__ void __launch_bounds__(BX *BY) k_backproject(GeomArgs g, double c, do
             const float top = (float)(g.nb - ib_lo);
             const float blim = (top < bmaxL) ? top : bmaxL;
             const int n = (acount > 0 && blim > 0.f) ? (int)ceilf(blim) : 0;
+            MotionIn min_{};
+            float x_old = 0.f;
+            if (EPI == EPI_MOTION) min_ = motion_load(mot, id, ix, iy, iz, n > 0);
+            if (EPI == EPI_ADD) x_old = add.x[id];
             // row weight (:164) as a difference of the clamped boundary F(b) = min(max(b, lo), hi)
             const float bhi = fminf(bmaxL, top);
             auto F = [&](int b) { return fminf(fmaxf((float)b, bminL), bhi); };

@@ -341,52 +341,90 @@ __device__ inline void store_transposed(T (*t)[TT + 1], T *__restrict__ outT, co
 // The warps run 32 x ROWS lanes per 32 x 32 tile, i.e. 32 / ROWS voxels per lane (WARP_ROWS).
 //
 // Forward warp (src/cu_deform.cu:79-90): natural output (out) and/or transposed output (outT).
+// Only the voxels the view's projector can read are warped (cc: its read set, ConeCull); the
+// others are neither read nor written (written as NaN when cc.mode == 2), so the projector sees
+// exactly the values an uncut warp gives it.
 template <bool FAST, int ROWS>
 __global__ void __launch_bounds__(TT *ROWS) k_warp_tile(float *__restrict__ out,
                                                      float *__restrict__ outT,
                                                      const float *__restrict__ src, int nx, int ny,
-                                                     int nz, float vol, float flow, Fields f) {
+                                                     int nz, float vol, float flow, Fields f,
+                                                     ConeCull cc) {
     __shared__ float t[TT][TT + 1];
     const int X0 = blockIdx.x * TT, Y0 = blockIdx.y * TT, i2 = blockIdx.z;
     const int i0 = X0 + threadIdx.x;
     constexpr int K = TT / ROWS;
+    const bool poison = cc.mode == 2;
     Dvf m[K];
     unsigned id[K];
-    bool ok[K];
+    bool ok[K], need[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) {
         const int i1 = Y0 + threadIdx.y + k * ROWS;
         ok[k] = i0 < nx && i1 < ny;
+        need[k] = ok[k] && cone_keep(cc, i0, i1, i2);
         id[k] = voxel_off<FAST>(i0, i1, i2, nx, ny);
-        m[k] = ok[k] ? dvf_at<FAST>(f, id[k], vol, flow) : Dvf{0.f, 0.f, 0.f};
+        m[k] = need[k] ? dvf_at<FAST>(f, id[k], vol, flow) : Dvf{0.f, 0.f, 0.f};
     }
     float v[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) {
         const int i1 = Y0 + threadIdx.y + k * ROWS;
         // sample at (i0 + my, i1 + mx, i2 + mz) (:85-89)
-        v[k] = ok[k] ? trilinear<FAST>(src, nx, ny, nz, i0, m[k].y, i1, m[k].x, i2, m[k].z)
-                     : 0.f;
+        v[k] = need[k] ? trilinear<FAST>(src, nx, ny, nz, i0, m[k].y, i1, m[k].x, i2, m[k].z)
+                       : __builtin_nanf("");
     }
     if (out) {
 #pragma unroll
         for (int k = 0; k < K; ++k)
-            if (ok[k]) vstore<FAST>(out, id[k], v[k]);
+            if (need[k] || (ok[k] && poison)) vstore<FAST>(out, id[k], v[k]);
     }
-    if (outT) store_transposed<float, ROWS>(t, outT, v, nx, ny, X0, Y0, i2);
+    if (outT) {   // through LDS, as store_transposed; lane (tx, r) writes voxel (X0 + r, Y0 + tx)
+#pragma unroll
+        for (int k = 0; k < K; ++k) t[threadIdx.y + k * ROWS][threadIdx.x] = v[k];
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            const int r = threadIdx.y + k * ROWS;
+            const int j0 = Y0 + threadIdx.x, j1 = X0 + r;
+            if (j0 < ny && j1 < nx && (poison || cone_keep(cc, j1, j0, i2)))
+                outT[j0 + j1 * ny + i2 * nx * ny] = t[threadIdx.x][r];
+        }
+    }
+}
+
+// The eight voxels a clamp-to-edge trilinear sample at (i0 + m0, i1 + m1, i2 + m2) reads (the
+// same clamping as trilinear()) meet the support of u (sup, ConeCull): otherwise all eight are 0
+// and so is the sample.
+__device__ inline bool sample_meets(const ConeCull &sup, int n0, int n1, int n2, int i0, float m0,
+                                    int i1, float m1, int i2, float m2) {
+    if (sup.mode == 0) return true;
+    m0 = __builtin_amdgcn_fmed3f(m0, -(float)(n0 + 1), (float)(n0 + 1));
+    m1 = __builtin_amdgcn_fmed3f(m1, -(float)(n1 + 1), (float)(n1 + 1));
+    m2 = __builtin_amdgcn_fmed3f(m2, -(float)(n2 + 1), (float)(n2 + 1));
+    const int b0 = i0 + (int)floorf(m0), b1 = i1 + (int)floorf(m1), b2 = i2 + (int)floorf(m2);
+    const int a0 = min(max(b0, 0), n0 - 1), c0 = min(max(b0 + 1, 0), n0 - 1);
+    const int a1 = min(max(b1, 0), n1 - 1), c1 = min(max(b1 + 1, 0), n1 - 1);
+    const int a2 = min(max(b2, 0), n2 - 1), c2 = min(max(b2 + 1, 0), n2 - 1);
+    return cone_box_meets(sup, 0.5f * (float)(a0 + c0), 0.5f * (float)(a1 + c1),
+                          0.5f * (float)(a2 + c2), 0.5f * (float)(c0 - a0),
+                          0.5f * (float)(c1 - a1), 0.5f * (float)(c2 - a2));
 }
 
 // x += lambda * W(u; vol, flow) (src/AUMISART.cu:222-223), then the optional clamp at zero.
+// Where the sample's voxels all lie outside the support of u (sup), W(u) is 0: x + 0 is x, so
+// that voxel is neither read nor written.  With the clamp, that skip is exact once x has been
+// clamped everywhere: the sweep's first view runs with sup.mode == 0.
 template <bool FAST, int ROWS>
 __global__ void __launch_bounds__(TT *ROWS) k_warp_add_tile(float *__restrict__ x,
                                                          const float *__restrict__ u, int nx,
                                                          int ny, int nz, float vol, float flow,
-                                                         Fields f, float lambda, int nonneg) {
+                                                         Fields f, float lambda, int nonneg,
+                                                         ConeCull sup) {
     const int X0 = blockIdx.x * TT, Y0 = blockIdx.y * TT, i2 = blockIdx.z;
     const int i0 = X0 + threadIdx.x;
     constexpr int K = TT / ROWS;
     Dvf m[K];
-    float xv[K];
     unsigned id[K];
     bool ok[K];
 #pragma unroll
@@ -395,19 +433,21 @@ __global__ void __launch_bounds__(TT *ROWS) k_warp_add_tile(float *__restrict__ 
         ok[k] = i0 < nx && i1 < ny;
         id[k] = voxel_off<FAST>(i0, i1, i2, nx, ny);
         m[k] = ok[k] ? dvf_at<FAST>(f, id[k], vol, flow) : Dvf{0.f, 0.f, 0.f};
-        xv[k] = ok[k] ? vload<FAST>(x, id[k]) : 0.f;
     }
     float v[K];
+    bool hit[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) {
         const int i1 = Y0 + threadIdx.y + k * ROWS;
-        v[k] = ok[k] ? trilinear<FAST>(u, nx, ny, nz, i0, m[k].y, i1, m[k].x, i2, m[k].z) : 0.f;
+        hit[k] = ok[k] && sample_meets(sup, nx, ny, nz, i0, m[k].y, i1, m[k].x, i2, m[k].z);
+        v[k] = hit[k] ? trilinear<FAST>(u, nx, ny, nz, i0, m[k].y, i1, m[k].x, i2, m[k].z) : 0.f;
     }
 #pragma unroll
     for (int k = 0; k < K; ++k) {
-        float y = xv[k] + v[k] * lambda;
+        if (!hit[k]) continue;
+        float y = vload<FAST>(x, id[k]) + v[k] * lambda;
         if (nonneg) y = fmaxf(y, 0.f);
-        if (ok[k]) vstore<FAST>(x, id[k], y);
+        vstore<FAST>(x, id[k], y);
     }
 }
 
@@ -717,11 +757,12 @@ inline dim3 tile_grid(int nx, int ny, int nz) {
 constexpr int WARP_ROWS = 32;
 
 int launch_warp(float *out, float *outT, const float *src, int nx, int ny, int nz, float vol,
-                float flow, const Fields &f, hipStream_t st) {
+                float flow, const Fields &f, hipStream_t st, const ConeCull *read_set) {
     const dim3 grid = tile_grid(nx, ny, nz);
+    const ConeCull cc = read_set ? *read_set : ConeCull{};
 #define MCSART_WARP(F, RW)                                                                      \
     hipLaunchKernelGGL((k_warp_tile<F, RW>), grid, dim3(TT, RW), 0, st, out, outT, src, nx, ny, \
-                       nz, vol, flow, f)
+                       nz, vol, flow, f, cc)
     if (fast_index(nx, ny, nz)) MCSART_WARP(true, WARP_ROWS);
     else MCSART_WARP(false, WARP_ROWS);
 #undef MCSART_WARP
@@ -737,11 +778,13 @@ int launch_transpose(float *dst, const float *src, int nx, int ny, int nz, hipSt
 }
 
 int launch_warp_add(float *x, const float *u, int nx, int ny, int nz, float vol, float flow,
-                    const Fields &f, float lambda, int nonneg, hipStream_t st) {
+                    const Fields &f, float lambda, int nonneg, hipStream_t st,
+                    const ConeCull *support) {
     const dim3 grid = tile_grid(nx, ny, nz);
+    const ConeCull sup = support ? *support : ConeCull{};
 #define MCSART_WARP_ADD(F, RW)                                                                  \
     hipLaunchKernelGGL((k_warp_add_tile<F, RW>), grid, dim3(TT, RW), 0, st, x, u, nx, ny, nz,    \
-                       vol, flow, f, lambda, nonneg)
+                       vol, flow, f, lambda, nonneg, sup)
     if (fast_index(nx, ny, nz)) MCSART_WARP_ADD(true, WARP_ROWS);
     else MCSART_WARP_ADD(false, WARP_ROWS);
 #undef MCSART_WARP_ADD

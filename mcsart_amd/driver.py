@@ -200,6 +200,18 @@ def job_bytes(g: Geometry, n_views: int, n_iter: int, keep_residual: bool = Fals
     return int(load().mcsart_job_bytes(ctypes.byref(cg), n_views, n_iter, int(keep_residual)))
 
 
+def view_voxels(g: Geometry, angles) -> np.ndarray:
+    """(n_views, 3) int64 sizes of the culled sets per view: voxels in its projector's read set,
+    voxels where its ratio backprojection can be nonzero, and the union of the latter with the
+    next view's read set (mcsart_view_voxels; host arithmetic, no device needed)."""
+    ang = np.ascontiguousarray(angles, np.float32)
+    out = np.zeros((ang.size, 3), np.int64)
+    cg = CGeom.of(g)
+    check(load().mcsart_view_voxels(ctypes.byref(cg), int(ang.size), ang.ctypes.data,
+                                    out.ctypes.data), "mcsart_view_voxels")
+    return out
+
+
 class Job:
     """One scan resident on the device (mcsart_job)."""
 
@@ -262,6 +274,11 @@ class Job:
 
     def set_graphs(self, enable: bool):
         check(self._lib.mcsart_job_set_graphs(self.ptr, int(enable)), "mcsart_job_set_graphs")
+
+    def set_culling(self, mode: int):
+        """0: every voxel every view; 1: skip work no later kernel sees (default); 2: as 1, with
+        the skipped writes made as NaN (checks that nothing reads them)."""
+        check(self._lib.mcsart_job_set_culling(self.ptr, int(mode)), "mcsart_job_set_culling")
 
     def set_tuning(self, proj_split: int = 0, back_zslab: int = 0, back_stage: int = 0,
                    proj_rows: int = 0, back_zslab_motion: int = 0):
