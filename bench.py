@@ -62,15 +62,11 @@ def algorithmic_bytes(spec, kind, layouts=None, voxels=None):
     `layouts` (per view, bit 0 natural, bit 1 transposed) averages the per-layout terms over the
     views the kernel is launched for.
 
-    With culling (`voxels`: the per-view set sizes of mcsart_view_voxels, (V, 3): read set R_v,
-    voxels H_v whose ratio backprojection can be nonzero, H_v union R_{v+1}) the per-voxel terms
-    count only the voxels a launch processes:
-    warp                   : (28 + 4 per layout) R_v
-    warp-add               : 24N (fields, every voxel) + 12 H_v (u sample, x read and write where
-                             the sample can be nonzero; H_v stands for that set, which is H_v
-                             moved by the displacements) -- view 0 of a sweep is uncut: 36N
-    motion step (view v)   : 8 |H_v u R_{v+1}| (accumulators read) + 8 H_v (written)
-                             + (16 + 4 per layout) R_{v+1} (constants read, g written) + 8D
+    With culling (`voxels`: the per-view set sizes of mcsart_view_voxels) the warp, the warp-add
+    and the motion step count only the voxels a launch processes (mcsart_amd/traffic.py): the
+    warp's per-voxel terms over view v's read set R_v, the warp-add's sample and x terms over
+    the voxels H_v where u can be nonzero, the motion step's accumulator terms over H_v (and
+    H_v u R_{v+1}) and its constant and g terms over R_{v+1}.
     """
     g = spec.geometry
     N, D = g.nvox, g.ndet
@@ -98,14 +94,13 @@ def algorithmic_bytes(spec, kind, layouts=None, voxels=None):
         if kind == "motion":
             return 32 * N + 4 * N * mean_layouts(range(2, V)) + 8 * D   # prep of views 2..V-1
         raise KeyError(kind)
-    R, H, U = (voxels[:, k].astype(np.float64) for k in range(3))
+    from mcsart_amd import traffic
     if kind == "warp":
-        return float(np.mean([(28 + 4 * n_layouts(v)) * R[v] for v in range(V)]))
+        return traffic.warp_bytes(voxels)
     if kind == "warp_add":
-        return float(np.mean([36 * N] + [24 * N + 12 * H[v] for v in range(1, V)]))
-    if kind == "motion":   # views 1..V-2, each preparing view v + 1
-        return float(np.mean([8 * U[v] + 8 * H[v] + (16 + 4 * n_layouts(v + 1)) * R[v + 1]
-                              + 8 * D for v in range(1, V - 1)]))
+        return traffic.warp_add_bytes(voxels, N)
+    if kind == "motion":   # views 1..V-2, each preparing view v + 1 (the isolated launches)
+        return traffic.motion_middle_bytes(voxels, N, D)
     raise KeyError(kind)
 
 

@@ -30,8 +30,10 @@ struct mcsart_job {
     float *d_img = nullptr;      // N
     float *d_fields = nullptr;   // 6N: ax ay az bx by bz
     float *d_init = nullptr;     // 7N: initial image and fields (mcsart_job_reset)
-    float *d_w1 = nullptr;       // 2N: warped image (image sweep) / (g, Q) pairs (motion sweep)
+    float *d_w1 = nullptr;       // 2N: (g, Q) pairs (motion sweep) / scratch
     float *d_w1T = nullptr;      // 2N: the same in the xy-transposed layout / transposed image
+    float *d_wi = nullptr;       // N: warped image (image sweep)
+    float *d_wiT = nullptr;      // N: the same in the xy-transposed layout
     float *d_w2 = nullptr;       // N: u
     float *d_acc = nullptr;      // 2N: motion-sweep accumulators, transposed, as (SVt, SFt) pairs
     float *d_cst = nullptr;      // 4N: motion-sweep constants and image (Ga, Gb, H, x) per voxel
@@ -64,7 +66,8 @@ namespace {
 size_t align_up(size_t v) { return (v + 255) & ~(size_t)255; }
 
 struct Layout {
-    size_t proj, img, fields, init, w1, w1T, w2, acc, cst, rp, norm, resid, partials, total;
+    size_t proj, img, fields, init, w1, w1T, wi, wiT, w2, acc, cst, rp, norm, resid, partials,
+        total;
 };
 
 Layout layout(const Geom &g, int V, int n_iter, int keep_residual) {
@@ -84,6 +87,11 @@ Layout layout(const Geom &g, int V, int n_iter, int keep_residual) {
     L.img = take(sizeof(float) * N);
     L.w1 = take(sizeof(float) * N * 2);
     L.w1T = take(sizeof(float) * N * 2);
+    // the image sweep's warped image in its own volumes: with culling a view's warp leaves the
+    // cells its projector reads with zero weights at most as an earlier view wrote them, so they
+    // only ever hold warps of the image (or the zeros the job starts with)
+    L.wi = take(sizeof(float) * N);
+    L.wiT = take(sizeof(float) * N);
     L.w2 = take(sizeof(float) * N);
     L.acc = take(sizeof(float) * N * 2);
     L.cst = take(sizeof(float) * N * 4);
@@ -171,15 +179,16 @@ int image_sweep(mcsart_job *j, int it) {
         if (motion) {   // x_i = W(x; v_i - v_0, f_i - f_0)  (:193), in the layouts P reads
             TimedRegion tr(j->ctx, TK_WARP);
             const ConeCull rs = read_set(j, iv);
-            rc = launch_warp(vi.has_y ? j->d_w1 : nullptr, vi.has_x ? j->d_w1T : nullptr,
+            rc = launch_warp(vi.has_y ? j->d_wi : nullptr, vi.has_x ? j->d_wiT : nullptr,
                              j->d_img, g.nx, g.ny, g.nz, j->volumes[iv] - j->volumes[0],
                              j->flows[iv] - j->flows[0], f, st, &rs);
             if (rc) return rc;
-            a.vol0 = j->d_w1;
+            a.vol0 = j->d_wi;
+            a.vol0T = vi.has_x ? j->d_wiT : nullptr;
         } else {
             a.vol0 = j->d_img;
+            a.vol0T = vi.has_x ? j->d_w1T : nullptr;
         }
-        a.vol0T = vi.has_x ? j->d_w1T : nullptr;
         {   // r = p_i - P x_i, P(1), rho_i  (:194-211)
             a.meas = j->d_proj + D * iv;
             a.rp = j->d_rp;
@@ -340,6 +349,8 @@ int mcsart::job_create_in(mcsart_ctx *ctx, const mcsart_geom *gp, int n_views,
     j->d_init = (float *)(b + L.init);
     j->d_w1 = (float *)(b + L.w1);
     j->d_w1T = (float *)(b + L.w1T);
+    j->d_wi = (float *)(b + L.wi);
+    j->d_wiT = (float *)(b + L.wiT);
     j->d_w2 = (float *)(b + L.w2);
     j->d_acc = (float *)(b + L.acc);
     j->d_cst = (float *)(b + L.cst);
@@ -496,23 +507,16 @@ int union_length(ZRange *r, int k) {
     return total;
 }
 
-// The z ranges of column (ix, iy) in a read set (cone_keep): up to three.
+// The z range of column (ix, iy) in a read set (cone_keep).
 int read_set_ranges(const ConeCull &C, int ix, int iy, ZRange *out) {
     const int nz = C.nz;
-    if (ix < C.sx || ix >= C.nx - C.sx || iy < C.sx || iy >= C.ny - C.sx) {
-        out[0] = ZRange{0, nz};
-        return 1;
-    }
     const float x = (float)ix, y = (float)iy;
     if (!(cone_form(C, 0, x, y, 0.f) >= 0.f && cone_form(C, 1, x, y, 0.f) >= 0.f)) return 0;
-    int k = 0;
-    out[k++] = ZRange{0, std::min(C.sz, nz)};
-    out[k++] = ZRange{std::max(nz - C.sz, 0), nz};
     // form 2 grows with z, form 3 falls
     const int lo = first_true(nz, [&](int iz) { return cone_form(C, 2, x, y, (float)iz) >= 0.f; });
     const int hi = first_true(nz, [&](int iz) { return cone_form(C, 3, x, y, (float)iz) < 0.f; });
-    out[k++] = ZRange{lo, std::max(lo, hi)};
-    return k;
+    out[0] = ZRange{lo, std::max(lo, hi)};
+    return 1;
 }
 
 // The z range of column (ix, iy) whose ratio backprojection has cells and rows (u can be
@@ -549,9 +553,12 @@ extern "C" int mcsart_view_voxels(const mcsart_geom *gp, int n_views, const floa
     const GeomArgs ga = geom_args(g);
     std::vector<ViewTrig> trig(n_views);
     std::vector<ConeCull> rs(n_views);
+    std::vector<int> layouts(n_views);
     for (int v = 0; v < n_views; ++v) {
         trig[v] = view_trig(angles[v]);
-        rs[v] = projector_read_set(ga, trig[v].c, trig[v].s, view_info_of(g, trig[v]), 1);
+        const ViewInfo vi = view_info_of(g, trig[v]);
+        rs[v] = projector_read_set(ga, trig[v].c, trig[v].s, vi, 1);
+        layouts[v] = (vi.has_y ? 1 : 0) | (vi.has_x ? 2 : 0);
     }
     auto count_view = [&](int v) {
         long long nread = 0, nhit = 0, nunion = 0;
@@ -566,9 +573,10 @@ extern "C" int mcsart_view_voxels(const mcsart_geom *gp, int n_views, const floa
                 const int kn = v + 1 < n_views ? read_set_ranges(rs[v + 1], ix, iy, u + 1) : 0;
                 nunion += union_length(u, 1 + kn);
             }
-        counts[3 * (long long)v] = nread;
-        counts[3 * (long long)v + 1] = nhit;
-        counts[3 * (long long)v + 2] = nunion;
+        counts[4 * (long long)v] = nread;
+        counts[4 * (long long)v + 1] = nhit;
+        counts[4 * (long long)v + 2] = nunion;
+        counts[4 * (long long)v + 3] = layouts[v];
     };
     // views on up to 16 host threads (each writes its own entries)
     const int nt = std::max(1, std::min({16, n_views, (int)std::thread::hardware_concurrency()}));
@@ -784,7 +792,7 @@ extern "C" int mcsart_job_kernel_time(mcsart_job *j, int kind, int reps, double 
             int r;
             if (kind == 2) {   // x_i = W(x; v_i - v_0, f_i - f_0) in the layouts P reads
                 const ConeCull rs = read_set(j, iv);
-                r = launch_warp(vi.has_y ? j->d_w1 : nullptr, vi.has_x ? j->d_w1T : nullptr,
+                r = launch_warp(vi.has_y ? j->d_wi : nullptr, vi.has_x ? j->d_wiT : nullptr,
                                 j->d_img, g.nx, g.ny, g.nz, j->volumes[iv] - j->volumes[0],
                                 j->flows[iv] - j->flows[0], f, st, &rs);
             } else if (kind == 3) {   // x += lambda W(u; v_0 - v_i, f_0 - f_i), on the copy
@@ -922,14 +930,10 @@ extern "C" int mcsart_job_autotune(mcsart_job *j, mcsart_tuning *chosen) {
         MCSART_HIP_TRY(hipMemsetAsync(j->d_acc, 0, sizeof(float) * 2 * N, st));
         auto motion = [&](const BackTune &bt) {
             for (int iv0 : views) {
+                // as the sweep launches it (culled), preparing the next view (or the view
+                // itself at the end of the scan)
                 const int iv = std::max(iv0, 1), nv = std::min(iv + 1, j->V - 1);
-                const ViewInfo &vn = j->vinfo[nv];
-                const MotionStep m{reinterpret_cast<float2 *>(j->d_acc),
-                                   reinterpret_cast<const float4 *>(j->d_cst),
-                                   vn.has_y ? pair : nullptr, vn.has_x ? pairT : nullptr,
-                                   j->volumes[iv - 1] - j->volumes[iv],
-                                   j->flows[iv - 1] - j->flows[iv], j->volumes[nv] - j->volumes[iv],
-                                   j->flows[nv] - j->flows[iv], 0, 0};
+                const MotionStep m = motion_step(j, iv, nv, pair, pairT, 0);
                 const int r = launch_backprojector(g, j->trig[iv], j->vinfo[iv], BACK_RATIO,
                                                    nullptr, j->d_rp, nullptr, st, nullptr, &bt,
                                                    &m);

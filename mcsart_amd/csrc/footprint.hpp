@@ -383,8 +383,10 @@ inline ViewInfo view_info(const GeomArgs &g, double c, double s) {
 //    ray's pyramid transversely and, axially, b_max * (box width) / SD past it; the planes of odd
 //    sizes sit 1/2 voxel off the integer grid.  The margin covers all of it.  A window clamped at
 //    the volume's edge (a plane the ray misses, or a footprint outside [0, nz]) reads the edge
-//    shell instead: cells within W of a transverse edge (any z) and, in columns inside the fan,
-//    within W of a z edge.  Voxels outside the read set are never read by that view's projector.
+//    shell instead (cells within W of a transverse edge, any z; in columns inside the fan, within
+//    W of a z edge), and only with weights of exactly zero: there any finite value, such as an
+//    earlier view's, adds exactly nothing, so the shell is not part of the read set.  Other voxels
+//    outside the read set are never read by that view's projector.
 //  * the support of a ratio backprojection u = B(r)/B(P q): u is 0 exactly at voxels whose
 //    six-point footprint misses the detector (no cells or no rows, src/cu_backprojection.cu:
 //    150-169); the six points lie within 1/sqrt(2) (xy) and 1/2 (z) of the centre.
@@ -395,7 +397,7 @@ struct ConeCull {
     int sx, sz;      // edge shell: transverse cells < sx or >= n - sx; z cells < sz or >= nz - sz
     int nx, ny, nz;
     int mode;        // 0: every voxel passes; 1: cull; 2: cull, and write NaN where a write is
-                     // culled (a check that nothing reads those cells)
+                     // culled outside the shell (a check that nothing reads those cells)
 };
 
 // Margin-raised form j at voxel index (x, y, z).
@@ -406,12 +408,18 @@ __host__ __device__ inline float cone_form(const ConeCull &C, int j, float x, fl
 // Voxel (ix, iy, iz) is in the read set (or the cull is off).
 __host__ __device__ inline bool cone_keep(const ConeCull &C, int ix, int iy, int iz) {
     if (C.mode == 0) return true;
-    if (ix < C.sx || ix >= C.nx - C.sx || iy < C.sx || iy >= C.ny - C.sx) return true;
     const float x = (float)ix, y = (float)iy, z = (float)iz;
-    const bool fan = cone_form(C, 0, x, y, z) >= 0.f && cone_form(C, 1, x, y, z) >= 0.f;
-    const bool zin = (cone_form(C, 2, x, y, z) >= 0.f && cone_form(C, 3, x, y, z) >= 0.f) ||
-                     iz < C.sz || iz >= C.nz - C.sz;
-    return fan && zin;
+    return cone_form(C, 0, x, y, z) >= 0.f && cone_form(C, 1, x, y, z) >= 0.f &&
+           cone_form(C, 2, x, y, z) >= 0.f && cone_form(C, 3, x, y, z) >= 0.f;
+}
+
+// A culled voxel whose write is made as a NaN in mode 2: one outside the read set and outside
+// the edge shell (shell cells are read with zero weights, so they keep their earlier values).
+__host__ __device__ inline bool cone_poison(const ConeCull &C, int ix, int iy, int iz) {
+    if (C.mode != 2) return false;
+    if (ix < C.sx || ix >= C.nx - C.sx || iy < C.sx || iy >= C.ny - C.sx) return false;
+    if (iz < C.sz || iz >= C.nz - C.sz) return false;
+    return !cone_keep(C, ix, iy, iz);
 }
 
 // A box of centre (x, y, z) and half-widths (hx, hy, hz), on top of the built-in margin, meets

@@ -342,8 +342,8 @@ __device__ inline void store_transposed(T (*t)[TT + 1], T *__restrict__ outT, co
 //
 // Forward warp (src/cu_deform.cu:79-90): natural output (out) and/or transposed output (outT).
 // Only the voxels the view's projector can read are warped (cc: its read set, ConeCull); the
-// others are neither read nor written (written as NaN when cc.mode == 2), so the projector sees
-// exactly the values an uncut warp gives it.
+// others are neither read nor written (outside the edge shell, written as NaN when cc.mode == 2),
+// so the projector's nonzero weights meet exactly the values an uncut warp gives it.
 template <bool FAST, int ROWS>
 __global__ void __launch_bounds__(TT *ROWS) k_warp_tile(float *__restrict__ out,
                                                      float *__restrict__ outT,
@@ -351,10 +351,10 @@ __global__ void __launch_bounds__(TT *ROWS) k_warp_tile(float *__restrict__ out,
                                                      int nz, float vol, float flow, Fields f,
                                                      ConeCull cc) {
     __shared__ float t[TT][TT + 1];
-    const int X0 = blockIdx.x * TT, Y0 = blockIdx.y * TT, i2 = blockIdx.z;
+    const int X0 = spread_x(blockIdx.x, blockIdx.y, blockIdx.z, gridDim.x) * TT;
+    const int Y0 = blockIdx.y * TT, i2 = blockIdx.z;
     const int i0 = X0 + threadIdx.x;
     constexpr int K = TT / ROWS;
-    const bool poison = cc.mode == 2;
     Dvf m[K];
     unsigned id[K];
     bool ok[K], need[K];
@@ -377,7 +377,8 @@ __global__ void __launch_bounds__(TT *ROWS) k_warp_tile(float *__restrict__ out,
     if (out) {
 #pragma unroll
         for (int k = 0; k < K; ++k)
-            if (need[k] || (ok[k] && poison)) vstore<FAST>(out, id[k], v[k]);
+            if (need[k] || (ok[k] && cone_poison(cc, i0, Y0 + threadIdx.y + k * ROWS, i2)))
+                vstore<FAST>(out, id[k], v[k]);
     }
     if (outT) {   // through LDS, as store_transposed; lane (tx, r) writes voxel (X0 + r, Y0 + tx)
 #pragma unroll
@@ -387,7 +388,7 @@ __global__ void __launch_bounds__(TT *ROWS) k_warp_tile(float *__restrict__ out,
         for (int k = 0; k < K; ++k) {
             const int r = threadIdx.y + k * ROWS;
             const int j0 = Y0 + threadIdx.x, j1 = X0 + r;
-            if (j0 < ny && j1 < nx && (poison || cone_keep(cc, j1, j0, i2)))
+            if (j0 < ny && j1 < nx && (cone_keep(cc, j1, j0, i2) || cone_poison(cc, j1, j0, i2)))
                 outT[j0 + j1 * ny + i2 * nx * ny] = t[threadIdx.x][r];
         }
     }
@@ -421,7 +422,8 @@ __global__ void __launch_bounds__(TT *ROWS) k_warp_add_tile(float *__restrict__ 
                                                          int ny, int nz, float vol, float flow,
                                                          Fields f, float lambda, int nonneg,
                                                          ConeCull sup) {
-    const int X0 = blockIdx.x * TT, Y0 = blockIdx.y * TT, i2 = blockIdx.z;
+    const int X0 = spread_x(blockIdx.x, blockIdx.y, blockIdx.z, gridDim.x) * TT;
+    const int Y0 = blockIdx.y * TT, i2 = blockIdx.z;
     const int i0 = X0 + threadIdx.x;
     constexpr int K = TT / ROWS;
     Dvf m[K];

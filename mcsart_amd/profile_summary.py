@@ -34,8 +34,16 @@ def short(name: str) -> str:
     return re.sub(r"\(.*", "", n).strip()
 
 
-def algorithmic_bytes(kernel: str, N: int, D: int):
-    """Minimum bytes one launch must move (reads + writes, 4-byte floats)."""
+def algorithmic_bytes(kernel: str, N: int, D: int, culled=None):
+    """Minimum bytes one launch must move (reads + writes, 4-byte floats).  `culled`: the
+    per-launch averages of the culled kernels over the profiled sweeps (culled_bytes), which
+    replace the uncut figures of the warps and the motion step."""
+    if culled:
+        if re.match(r"k_backproject<1, \d+, \w+, 1>", kernel):
+            return culled["motion"]
+        base = re.sub(r"<(true|false)(, \d+)?>$", "", kernel)
+        if base in ("k_warp_tile", "k_warp_add_tile"):
+            return culled[base]
     m = re.match(r"k_project(?:_rows)?<(\d)", kernel)
     if m:
         mode = int(m.group(1))
@@ -119,10 +127,27 @@ def load_tcc(d):
     return out
 
 
-def summarise(workload, trace, fetch, write, copy_mb=64, views=0, sq=None, tcc=None):
+def culled_bytes(spec):
+    """Per-launch algorithmic bytes of the culled kernels averaged over the profiled sweeps (an
+    image sweep over views 0..V-1, a motion sweep over views 1..V-1), from the views' set sizes
+    (mcsart_view_voxels; mcsart_amd/traffic.py)."""
+    from . import traffic
+    from .driver import view_voxels
+    g = spec.geometry
+    vox = view_voxels(g, spec.angles)
+    return {"k_warp_tile": traffic.warp_bytes(vox),
+            "k_warp_add_tile": traffic.warp_add_bytes(vox, g.nvox),
+            "motion": traffic.motion_sweep_bytes(vox, g.nvox, g.ndet)}
+
+
+def summarise(workload, trace, fetch, write, copy_mb=64, views=0, sq=None, tcc=None,
+              culled=True):
     spec = WORKLOADS[workload]()
+    if views:
+        spec = configs.scaled(spec, n_views=views)
     g = spec.geometry
     N, D = g.nvox, g.ndet
+    cb = culled_bytes(spec) if culled else None
     stats = load_stats(trace)
     fe, fcal = load_pmc(fetch) if fetch else ({}, None)
     wr, wcal = load_pmc(write) if write else ({}, None)
@@ -135,7 +160,7 @@ def summarise(workload, trace, fetch, write, copy_mb=64, views=0, sq=None, tcc=N
     for k, (calls, us) in sorted(stats.items(), key=lambda kv: -kv[1][0] * kv[1][1]):
         if k.startswith("__amd"):
             continue
-        alg = algorithmic_bytes(k, N, D)
+        alg = algorithmic_bytes(k, N, D, cb)
         row = {"kernel": k, "launches": calls, "avg_us": round(us, 2),
                "algorithmic_bytes": alg}
         if alg:
@@ -162,6 +187,7 @@ def summarise(workload, trace, fetch, write, copy_mb=64, views=0, sq=None, tcc=N
                 row["traffic_over_algorithmic"] = round(row["traffic_bytes"] / alg, 2)
         rows.append(row)
     return {"workload": workload, "N": N, "D": D, "views": views or spec.n_views,
+            "culled": bool(culled),
             "fetch_factor": ffac and round(ffac, 4), "write_factor": wfac and round(wfac, 4),
             "kernels": rows}
 
@@ -175,7 +201,10 @@ def to_markdown(s, command_lines):
             "2 x SQ_INSTS_VALU / (1024 SIMDs x SQ_BUSY_CYCLES / 32): the share of the SIMDs' "
             "cycles spent issuing vector instructions (2 cycles per wave64 instruction: with 4, "
             "some kernels would exceed 1).  L2 hit rate = TCC_HIT_sum / (TCC_HIT_sum + "
-            "TCC_MISS_sum) per dispatch.", "",
+            "TCC_MISS_sum) per dispatch." + (
+                "  The warps and the motion step skip the voxels no later kernel sees "
+                "(culling): their algorithmic bytes count the voxels they process, averaged over "
+                "the profiled views (mcsart_amd/traffic.py)." if s.get("culled") else ""), "",
             "| kernel | launches | avg us | algorithmic MB | achieved GB/s | frac of 8 TB/s | "
             "fetch MB | write MB | traffic / algorithmic | VALU busy | L2 hit rate |",
             "|---|---:|---:|---:|---:|---:|---:|---:|---:|---:|---:|"]

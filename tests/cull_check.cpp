@@ -1,6 +1,7 @@
 // Host check of the culling geometry (ConeCull in mcsart_amd/csrc/footprint.hpp): every cell a
-// projector kernel reads lies in the view's read set, and every voxel whose backprojection can be
-// nonzero lies in the view's u support.
+// projector kernel reads with a nonzero weight lies in the view's read set, every cell it reads
+// with zero weights only lies in the read set or the edge shell, and every voxel whose
+// backprojection can be nonzero lies in the view's u support.
 //
 // The projector's reads are replayed here on the CPU from the same per-ray constants the kernels
 // use (ray_dir / ray_t_setup / ray_z_setup / ray_setup, FP32 plane bounds):
@@ -28,13 +29,29 @@ using namespace mcsart;
 
 namespace {
 
+// Per voxel: bit 0 read at all, bit 1 read with a nonzero weight.
 struct Marks {
     int nx, ny, nz;
     std::vector<unsigned char> m;
-    void set(int ix, int iy, int iz) { m[(size_t)ix + (size_t)nx * (iy + (size_t)ny * iz)] = 1; }
+    void set(int ix, int iy, int iz, bool nonzero) {
+        m[(size_t)ix + (size_t)nx * (iy + (size_t)ny * iz)] |= nonzero ? 3 : 1;
+    }
 };
 
 float clampf(float x, float lo, float hi) { return std::fmin(std::fmax(x, lo), hi); }
+
+// The rows kernel's window weights (k_project.hip, window()): F(b) = clamp(b, lo, hi) with the
+// window's interior boundaries, the last one min(f0 + W, hi).
+void window_weights(float lo, float hi, float top, int W, float *w, float &f0) {
+    f0 = clampf(std::floor(lo), 0.f, top);
+    float F = lo;
+    for (int k = 0; k < W; ++k) {
+        const float Fn = (k + 1 < W) ? clampf(f0 + (float)(k + 1), lo, hi)
+                                     : std::fmin(f0 + (float)W, hi);
+        w[k] = Fn - F;
+        F = Fn;
+    }
+}
 
 // k_project_rows: windows W x WZ from the clamped lower bounds
 void rows_reads(const GeomArgs &g, double c, double s, int W, int WZ, Marks &M) {
@@ -52,15 +69,17 @@ void rows_reads(const GeomArgs &g, double c, double s, int W, int WZ, Marks &M) 
             ray_z_setup(g, d, ib, Z);
             for (int ip = 0; ip < np; ++ip) {
                 const float p = (float)ip + p0;
-                const float t1 = std::fma(T.rlo, p, T.Klo);
-                const float z1 = std::fma(Z.rlo, p, Z.Klo);
-                const float f0 = clampf(std::floor(clampf(t1, 0.f, ntf)), 0.f, ttop);
-                const float g0 = clampf(std::floor(clampf(z1, 0.f, nzf)), 0.f, ztop);
+                const float t1 = std::fma(T.rlo, p, T.Klo), t2 = std::fma(T.rhi, p, T.Khi);
+                const float z1 = std::fma(Z.rlo, p, Z.Klo), z2 = std::fma(Z.rhi, p, Z.Khi);
+                float wt[3], wz[3], f0, g0;
+                window_weights(clampf(t1, 0.f, ntf), clampf(t2, 0.f, ntf), ttop, W, wt, f0);
+                window_weights(clampf(z1, 0.f, nzf), clampf(z2, 0.f, nzf), ztop, WZ, wz, g0);
                 for (int k = 0; k < W; ++k)
                     for (int j = 0; j < WZ; ++j) {
                         const int it = (int)f0 + k, iz = (int)g0 + j;
-                        if (xb) M.set(ip, it, iz);
-                        else M.set(it, ip, iz);
+                        const bool nz = wt[k] != 0.f && wz[j] != 0.f;
+                        if (xb) M.set(ip, it, iz, nz);
+                        else M.set(it, ip, iz, nz);
                     }
             }
         }
@@ -81,13 +100,23 @@ void masked_reads(const GeomArgs &g, double c, double s, int WT, int WZ, Marks &
                 const float ta = std::fma(R.rA, p, R.KA), tb = std::fma(R.rB, p, R.KB);
                 const float za = std::fma(R.rzA, p, R.KzA), zb = std::fma(R.rzB, p, R.KzB);
                 const float t1 = std::fmin(ta, tb), z1 = std::fmin(za, zb);
-                const int T0 = (int)clampf(std::floor(t1), -4.f, (float)nt);
-                const int Z0 = (int)clampf(std::floor(z1), -4.f, (float)g.nz);
+                const float t2 = std::fmax(ta, tb), z2 = std::fmax(za, zb);
+                const bool in = (t2 > 0.f) & (t1 < (float)nt) & (z2 > 0.f) & (z1 < (float)g.nz);
+                const float ft = clampf(std::floor(t1), -4.f, (float)nt);
+                const float fz = clampf(std::floor(z1), -4.f, (float)g.nz);
+                const int T0 = (int)ft, Z0 = (int)fz;
                 for (int k = 0; k < WT; ++k)
                     for (int j = 0; j < WZ; ++j) {
+                        // plane_weights(): masked to zero outside the plane or the volume
+                        const float lk = (k == 0) ? t1 : ft + (float)k;
+                        const float lj = (j == 0) ? z1 : fz + (float)j;
+                        const float wk = std::fmax(std::fmin(ft + (float)(k + 1), t2) - lk, 0.f);
+                        const float wj = std::fmax(std::fmin(fz + (float)(j + 1), z2) - lj, 0.f);
+                        const bool vk = T0 + k >= 0 && T0 + k < nt, vj = Z0 + j >= 0 && Z0 + j < g.nz;
+                        const bool nz = in && vk && vj && wk != 0.f && wj != 0.f;
                         const int it = clampi(T0 + k, 0, nt - 1), iz = clampi(Z0 + j, 0, g.nz - 1);
-                        if (xb) M.set(ip, it, iz);
-                        else M.set(it, ip, iz);
+                        if (xb) M.set(ip, it, iz, nz);
+                        else M.set(it, ip, iz, nz);
                     }
             }
         }
@@ -111,8 +140,8 @@ void wide_reads(const GeomArgs &g, double c, double s, Marks &M) {
                 const int Zh = std::min(g.nz - 1, (int)std::fmin(std::floor(F.z2), (float)g.nz));
                 for (int i = Tl; i <= Th; ++i)
                     for (int k = Zl; k <= Zh; ++k) {
-                        if (xb) M.set(ip, i, k);
-                        else M.set(i, ip, k);
+                        if (xb) M.set(ip, i, k, true);
+                        else M.set(i, ip, k, true);
                     }
             }
         }
@@ -169,9 +198,13 @@ int main(int argc, char **argv) {
             for (int iy = 0; iy < g.ny; ++iy)
                 for (int ix = 0; ix < g.nx; ++ix) {
                     const bool keep = cone_keep(rs, ix, iy, iz);
+                    const bool shell = ix < rs.sx || ix >= g.nx - rs.sx || iy < rs.sx ||
+                                       iy >= g.ny - rs.sx || iz < rs.sz || iz >= g.nz - rs.sz;
                     nkeep += keep;
                     keepm[(size_t)ix + (size_t)g.nx * (iy + (size_t)g.ny * iz)] = keep;
-                    if (M.m[(size_t)ix + (size_t)g.nx * (iy + (size_t)g.ny * iz)] && !keep) ++viol;
+                    const unsigned char mk = M.m[(size_t)ix + (size_t)g.nx * (iy + (size_t)g.ny * iz)];
+                    // read with a nonzero weight: in the read set; read at all: or in the shell
+                    if (((mk & 2) && !keep) || ((mk & 1) && !keep && !shell)) ++viol;
                 }
         for (int iy = 0; iy < g.ny; ++iy)
             for (int ix = 0; ix < g.nx; ++ix) {

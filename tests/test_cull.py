@@ -92,7 +92,7 @@ def test_set_sizes_match_brute_force(cull_check, name):
     brute = np.array([[int(x) for x in line.split()[2:]] for line in r.stdout.splitlines()
                       if line.startswith("count")])
     ang = np.concatenate([2 * np.pi * np.arange(8) / 8, np.array(extra, float)]).astype(np.float32)
-    assert np.array_equal(view_voxels(g, ang), brute)
+    assert np.array_equal(view_voxels(g, ang)[:, :3], brute)
 
 
 @pytest.mark.parametrize("name", ["128^3", "offset truncated detector"])

@@ -43,9 +43,10 @@ def test_job_bytes_accounting():
     g = configs.c2_mcsart128().geometry
     b = job_bytes(g, 200, 20)
     N, D = g.nvox, g.ndet
-    # projections + image + 6 fields + 7 pristine + 5 work volumes + 2 motion accumulators +
-    # 4 motion constants (Ga, Gb, H and x) + (r, n) pairs + norms
-    floor = 4 * (200 * D + N + 6 * N + 7 * N + 5 * N + 2 * N + 4 * N + 2 * D + 20 * 200)
+    # projections + image + 6 fields + 7 pristine + 5 work volumes (pairs in two layouts, u) +
+    # the image sweep's warped image in two layouts + 2 motion accumulators + 4 motion constants
+    # (Ga, Gb, H and x) + (r, n) pairs + norms
+    floor = 4 * (200 * D + N + 6 * N + 7 * N + 5 * N + 2 * N + 2 * N + 4 * N + 2 * D + 20 * 200)
     assert floor <= b <= floor + (1 << 20)
 
 
