@@ -3,7 +3,9 @@ hardware-counter passes).
 
     python -m mcsart_amd.profile_sweep --workload mcsart128 --sweeps 1,0 --copy-mb 64
 
-Builds the named workload's scan (synthetic inputs, measured projections made on the GPU),
+Builds the named workload's scan (synthetic inputs, measured projections made on the GPU, or
+read from a file that an earlier --save-proj run wrote, so that the profile holds the sweeps'
+kernels only),
 autotunes the footprint kernels' launch shapes as the benchmark does, then runs the given sweeps
 (1 = image sweep, 0 = motion sweep) kernel by kernel (no graphs, so every kernel is its own
 dispatch), optionally the isolated launches the benchmark times (--isolated: every view of the
@@ -15,6 +17,7 @@ import argparse
 import sys
 import time
 
+import numpy as np
 
 from . import configs
 from .configs import make_scan_inputs
@@ -36,6 +39,12 @@ def main(argv=None):
     ap.add_argument("--isolated", default="",
                     help="comma-separated kernels to launch over all views back to back after "
                          "the sweeps: projector, backprojector, warp, warp_add, motion")
+    ap.add_argument("--save-proj", default="",
+                    help="make the scan's measured projections, save them to this .npy file and "
+                         "exit (run outside the profiler: making them launches warps and "
+                         "projections of their own)")
+    ap.add_argument("--load-proj", default="",
+                    help="measured projections from a --save-proj file instead of making them")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--tuning", default="",
                     help="fixed launch shapes proj_split,back_zslab,back_stage,proj_rows"
@@ -48,7 +57,15 @@ def main(argv=None):
     sweeps = [int(s) for s in args.sweeps.split(",") if s != ""]
     g = spec.geometry
     data = make_scan_inputs(spec)
-    proj = measure(spec, data, spec.seed + 404)
+    if args.load_proj:
+        proj = np.load(args.load_proj)
+        assert proj.shape == (spec.n_views,) + g.det_shape, proj.shape
+    else:
+        proj = measure(spec, data, spec.seed + 404)
+    if args.save_proj:
+        np.save(args.save_proj, proj)
+        print(f"[profile_sweep] measured projections saved to {args.save_proj}", flush=True)
+        return 0
     ctx = Context(0)
     job = Job(ctx, g, spec.angles, data.alpha, data.beta, sweeps, lam=spec.lam,
               motion=spec.motion, nonneg=spec.nonneg)
