@@ -344,13 +344,14 @@ int mcsart_job_set_graphs(mcsart_job *job, int enable);
  * NaN instead (a check that no kernel reads a skipped cell: any such read would show in the
  * results).  Drops the captured sweeps. */
 int mcsart_job_set_culling(mcsart_job *job, int mode);
-/* Sizes of the culled sets of a scan, per view v (counts[4*n_views]), for the algorithmic bytes
- * of the culled kernels: counts[4v] voxels in view v's read set (its warp's writes),
- * counts[4v+1] voxels whose ratio backprojection of view v has cells and rows (u can be nonzero:
- * the motion step's accumulator writes), counts[4v+2] the union of that set and view v+1's read
- * set (the motion step's accumulator reads; for the last view, counts[4v+1]), counts[4v+3] the
- * volume layouts view v's rays read (as mcsart_job_view_layouts).  Host arithmetic only (no
- * device needed). */
+/* Sizes of the culled sets of a scan, per view v (counts[5*n_views]), for the algorithmic bytes
+ * of the culled kernels: counts[5v] voxels in view v's read set (its warp's writes),
+ * counts[5v+1] the read set with the volume's edge shell (where a motion step prepares g for
+ * view v), counts[5v+2] voxels whose ratio backprojection of view v has cells and rows (u can be
+ * nonzero: the motion step's accumulator writes), counts[5v+3] the union of that set and view
+ * v+1's read set with its shell (the motion step's accumulator reads; for the last view,
+ * counts[5v+2]), counts[5v+4] the volume layouts view v's rays read (as
+ * mcsart_job_view_layouts).  Host arithmetic only (no device needed). */
 int mcsart_view_voxels(const mcsart_geom *g, int n_views, const float *angles,
                        long long *counts);
 int mcsart_job_get_image(mcsart_job *job, float *h_img);

@@ -507,16 +507,26 @@ int union_length(ZRange *r, int k) {
     return total;
 }
 
-// The z range of column (ix, iy) in a read set (cone_keep).
-int read_set_ranges(const ConeCull &C, int ix, int iy, ZRange *out) {
+// The z ranges of column (ix, iy) in a read set (cone_keep), with the edge shell (cone_shell)
+// when `shell`: up to three.
+int read_set_ranges(const ConeCull &C, int ix, int iy, ZRange *out, bool shell) {
     const int nz = C.nz;
+    if (shell && (ix < C.sx || ix >= C.nx - C.sx || iy < C.sx || iy >= C.ny - C.sx)) {
+        out[0] = ZRange{0, nz};
+        return 1;
+    }
+    int k = 0;
+    if (shell) {
+        out[k++] = ZRange{0, std::min(C.sz, nz)};
+        out[k++] = ZRange{std::max(nz - C.sz, 0), nz};
+    }
     const float x = (float)ix, y = (float)iy;
-    if (!(cone_form(C, 0, x, y, 0.f) >= 0.f && cone_form(C, 1, x, y, 0.f) >= 0.f)) return 0;
+    if (!(cone_form(C, 0, x, y, 0.f) >= 0.f && cone_form(C, 1, x, y, 0.f) >= 0.f)) return k;
     // form 2 grows with z, form 3 falls
     const int lo = first_true(nz, [&](int iz) { return cone_form(C, 2, x, y, (float)iz) >= 0.f; });
     const int hi = first_true(nz, [&](int iz) { return cone_form(C, 3, x, y, (float)iz) < 0.f; });
-    out[0] = ZRange{lo, std::max(lo, hi)};
-    return 1;
+    out[k++] = ZRange{lo, std::max(lo, hi)};
+    return k;
 }
 
 // The z range of column (ix, iy) whose ratio backprojection has cells and rows (u can be
@@ -561,22 +571,27 @@ extern "C" int mcsart_view_voxels(const mcsart_geom *gp, int n_views, const floa
         layouts[v] = (vi.has_y ? 1 : 0) | (vi.has_x ? 2 : 0);
     }
     auto count_view = [&](int v) {
-        long long nread = 0, nhit = 0, nunion = 0;
+        long long nread = 0, nprep = 0, nhit = 0, nunion = 0;
         for (int iy = 0; iy < g.ny; ++iy)
             for (int ix = 0; ix < g.nx; ++ix) {
                 ZRange r[4];
-                const int k = read_set_ranges(rs[v], ix, iy, r);
+                int k = read_set_ranges(rs[v], ix, iy, r, false);
                 nread += union_length(r, k);
+                k = read_set_ranges(rs[v], ix, iy, r, true);
+                nprep += union_length(r, k);
                 ZRange u[4];
                 u[0] = hit_range(ga, trig[v].c, trig[v].s, ix, iy);
                 nhit += u[0].hi - u[0].lo;
-                const int kn = v + 1 < n_views ? read_set_ranges(rs[v + 1], ix, iy, u + 1) : 0;
+                const int kn =
+                    v + 1 < n_views ? read_set_ranges(rs[v + 1], ix, iy, u + 1, true) : 0;
                 nunion += union_length(u, 1 + kn);
             }
-        counts[4 * (long long)v] = nread;
-        counts[4 * (long long)v + 1] = nhit;
-        counts[4 * (long long)v + 2] = nunion;
-        counts[4 * (long long)v + 3] = layouts[v];
+        long long *c = counts + 5 * (long long)v;
+        c[0] = nread;
+        c[1] = nprep;
+        c[2] = nhit;
+        c[3] = nunion;
+        c[4] = layouts[v];
     };
     // views on up to 16 host threads (each writes its own entries)
     const int nt = std::max(1, std::min({16, n_views, (int)std::thread::hardware_concurrency()}));

@@ -386,7 +386,9 @@ inline ViewInfo view_info(const GeomArgs &g, double c, double s) {
 //    shell instead (cells within W of a transverse edge, any z; in columns inside the fan, within
 //    W of a z edge), and only with weights of exactly zero: there any finite value, such as an
 //    earlier view's, adds exactly nothing, so the shell is not part of the read set.  Other voxels
-//    outside the read set are never read by that view's projector.
+//    outside the read set are never read by that view's projector.  (A volume whose earlier
+//    values may not be finite, as the motion sweep's g once its fields overflow, has its shell
+//    written as well: 0 times Inf is NaN.)
 //  * the support of a ratio backprojection u = B(r)/B(P q): u is 0 exactly at voxels whose
 //    six-point footprint misses the detector (no cells or no rows, src/cu_backprojection.cu:
 //    150-169); the six points lie within 1/sqrt(2) (xy) and 1/2 (z) of the centre.
@@ -413,13 +415,17 @@ __host__ __device__ inline bool cone_keep(const ConeCull &C, int ix, int iy, int
            cone_form(C, 2, x, y, z) >= 0.f && cone_form(C, 3, x, y, z) >= 0.f;
 }
 
+// Voxel (ix, iy, iz) lies in the edge shell: within sx of a transverse edge or sz of a z edge
+// (a superset of the cells clamped windows read).
+__host__ __device__ inline bool cone_shell(const ConeCull &C, int ix, int iy, int iz) {
+    return ix < C.sx || ix >= C.nx - C.sx || iy < C.sx || iy >= C.ny - C.sx || iz < C.sz ||
+           iz >= C.nz - C.sz;
+}
+
 // A culled voxel whose write is made as a NaN in mode 2: one outside the read set and outside
 // the edge shell (shell cells are read with zero weights, so they keep their earlier values).
 __host__ __device__ inline bool cone_poison(const ConeCull &C, int ix, int iy, int iz) {
-    if (C.mode != 2) return false;
-    if (ix < C.sx || ix >= C.nx - C.sx || iy < C.sx || iy >= C.ny - C.sx) return false;
-    if (iz < C.sz || iz >= C.nz - C.sz) return false;
-    return !cone_keep(C, ix, iy, iz);
+    return C.mode == 2 && !cone_shell(C, ix, iy, iz) && !cone_keep(C, ix, iy, iz);
 }
 
 // A box of centre (x, y, z) and half-widths (hx, hy, hz), on top of the built-in margin, meets

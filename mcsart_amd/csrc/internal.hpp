@@ -156,9 +156,9 @@ struct BackTune {
 // from launch_motion_begin, which also wrote the sweep's Q (the projector scales P(Q) by
 // -vd - fd); u itself is not stored.
 // Work the next kernels never see is skipped: g only where the next view's projector can read it
-// (`next`, its read set: ConeCull), and, when `skip_zero` (wv and wf finite), the accumulators
-// only where u can be nonzero (the voxel's footprint meets the detector; elsewhere u = 0 and the
-// sums do not change).  A voxel with neither is not touched at all.
+// (`next`, its read set and edge shell: ConeCull), and, when `skip_zero` (wv and wf finite), the
+// accumulators only where u can be nonzero (the voxel's footprint meets the detector; elsewhere
+// u = 0 and the sums do not change).  A voxel with neither is not touched at all.
 struct MotionStep {
     float2 *acc;         // (SVt, SFt) per voxel
     const float4 *cst;   // (Ga, Gb, H, x) per voxel

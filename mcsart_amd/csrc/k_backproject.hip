@@ -86,7 +86,9 @@ __device__ inline MotionIn motion_load(const MotionStep &m, int id, int ix, int 
                                        bool hit) {
     MotionIn in;
     in.hit = hit || !m.skip_zero;
-    in.need = !m.last && cone_keep(m.next, ix, iy, iz);
+    // the shell too: an earlier view's g there may have overflowed (the next projector reads
+    // shell cells with zero weights, and 0 times Inf is NaN)
+    in.need = !m.last && (cone_keep(m.next, ix, iy, iz) || cone_shell(m.next, ix, iy, iz));
     in.poison = !m.last && cone_poison(m.next, ix, iy, iz);   // never read: NaN in mode 2
     const float2 a = (m.first || !(in.hit || in.need)) ? make_float2(0.f, 0.f) : m.acc[id];
     in.sv = a.x;

@@ -201,12 +201,13 @@ def job_bytes(g: Geometry, n_views: int, n_iter: int, keep_residual: bool = Fals
 
 
 def view_voxels(g: Geometry, angles) -> np.ndarray:
-    """(n_views, 4) int64 per view: sizes of the culled sets (voxels in its projector's read
-    set, voxels where its ratio backprojection can be nonzero, the union of the latter with the
-    next view's read set) and the layouts its rays read (bit 0 natural, bit 1 transposed)
-    (mcsart_view_voxels; host arithmetic, no device needed)."""
+    """(n_views, 5) int64 per view: sizes of the culled sets (voxels in its projector's read
+    set; the same with the volume's edge shell; voxels where its ratio backprojection can be
+    nonzero; the union of the latter with the next view's read set and shell) and the layouts its
+    rays read (bit 0 natural, bit 1 transposed) (mcsart_view_voxels; host arithmetic, no device
+    needed)."""
     ang = np.ascontiguousarray(angles, np.float32)
-    out = np.zeros((ang.size, 4), np.int64)
+    out = np.zeros((ang.size, 5), np.int64)
     cg = CGeom.of(g)
     check(load().mcsart_view_voxels(ctypes.byref(cg), int(ang.size), ang.ctypes.data,
                                     out.ctypes.data), "mcsart_view_voxels")

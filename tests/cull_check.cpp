@@ -171,10 +171,11 @@ int main(int argc, char **argv) {
     std::vector<unsigned char> prev_hit;
     long long prev_nhit = 0;
     int view = 0;
-    auto print_counts = [&](int v, long long nread, long long nhit, long long nunion) {
-        std::printf("count %d %lld %lld %lld\n", v, nread, nhit, nunion);
+    auto print_counts = [&](int v, long long nread, long long nprep, long long nhit,
+                            long long nunion) {
+        std::printf("count %d %lld %lld %lld %lld\n", v, nread, nprep, nhit, nunion);
     };
-    std::vector<long long> reads;
+    std::vector<long long> reads, preps;
     for (float ang : angles) {
         const double c = std::cos((double)ang), s = std::sin((double)ang);
         const ViewInfo vi = view_info(g, c, s);
@@ -192,8 +193,8 @@ int main(int argc, char **argv) {
         } else {
             wide_reads(g, c, s, M);
         }
-        long long nkeep = 0, nsup = 0, viol = 0, vsup = 0, nhit = 0;
-        std::vector<unsigned char> keepm(N, 0), hitm(N, 0);
+        long long nkeep = 0, nprep = 0, nsup = 0, viol = 0, vsup = 0, nhit = 0;
+        std::vector<unsigned char> keepm(N, 0), hitm(N, 0);   // keepm: read set and shell
         for (int iz = 0; iz < g.nz; ++iz)
             for (int iy = 0; iy < g.ny; ++iy)
                 for (int ix = 0; ix < g.nx; ++ix) {
@@ -201,7 +202,9 @@ int main(int argc, char **argv) {
                     const bool shell = ix < rs.sx || ix >= g.nx - rs.sx || iy < rs.sx ||
                                        iy >= g.ny - rs.sx || iz < rs.sz || iz >= g.nz - rs.sz;
                     nkeep += keep;
-                    keepm[(size_t)ix + (size_t)g.nx * (iy + (size_t)g.ny * iz)] = keep;
+                    nprep += keep || shell;
+                    if (shell != cone_shell(rs, ix, iy, iz)) ++viol;   // the two agree
+                    keepm[(size_t)ix + (size_t)g.nx * (iy + (size_t)g.ny * iz)] = keep || shell;
                     const unsigned char mk = M.m[(size_t)ix + (size_t)g.nx * (iy + (size_t)g.ny * iz)];
                     // read with a nonzero weight: in the read set; read at all: or in the shell
                     if (((mk & 2) && !keep) || ((mk & 1) && !keep && !shell)) ++viol;
@@ -230,9 +233,10 @@ int main(int argc, char **argv) {
         if (view > 0) {   // the previous view's union with this view's read set
             long long nu = 0;
             for (size_t i = 0; i < N; ++i) nu += prev_hit[i] || keepm[i];
-            print_counts(view - 1, reads.back(), prev_nhit, nu);
+            print_counts(view - 1, reads.back(), preps.back(), prev_nhit, nu);
         }
         reads.push_back(nkeep);
+        preps.push_back(nprep);
         prev_hit.swap(hitm);
         prev_nhit = nhit;
         ++view;
@@ -245,7 +249,7 @@ int main(int argc, char **argv) {
                         "outside the support (windows %d x %d)\n",
                         (double)ang, viol, vsup, vi.wt, vi.wz);
     }
-    if (view > 0) print_counts(view - 1, reads.back(), prev_nhit, prev_nhit);
+    if (view > 0) print_counts(view - 1, reads.back(), preps.back(), prev_nhit, prev_nhit);
     std::printf("views %zu kept %.4f support %.4f bad_read %lld bad_support %lld\n",
                 angles.size(), kept / angles.size(), support / angles.size(), bad_read,
                 bad_support);

@@ -79,8 +79,8 @@ def test_read_set_at_256x256x128(cull_check):
                                   "coarse detector (wide windows)"])
 def test_set_sizes_match_brute_force(cull_check, name):
     """mcsart_view_voxels (per column: z ranges from binary searches) equals the replay's voxel
-    by voxel counts of the read set, the voxels with cells and rows, and their union with the next
-    view's read set."""
+    by voxel counts of the read set, the read set with the edge shell, the voxels with cells and
+    rows, and their union with the next view's read set and shell."""
     import numpy as np
     from mcsart_amd.driver import view_voxels
     g = GEOMETRIES[name]
@@ -92,7 +92,7 @@ def test_set_sizes_match_brute_force(cull_check, name):
     brute = np.array([[int(x) for x in line.split()[2:]] for line in r.stdout.splitlines()
                       if line.startswith("count")])
     ang = np.concatenate([2 * np.pi * np.arange(8) / 8, np.array(extra, float)]).astype(np.float32)
-    assert np.array_equal(view_voxels(g, ang)[:, :3], brute)
+    assert np.array_equal(view_voxels(g, ang)[:, :4], brute)
 
 
 @pytest.mark.parametrize("name", ["128^3", "offset truncated detector"])

@@ -70,22 +70,23 @@ def test_culled_byte_counts(bench):
     spec = configs.c2_mcsart128()
     g = spec.geometry
     N, D = g.nvox, g.ndet
-    full = np.tile(np.array([N, N, N, 1], np.int64), (spec.n_views, 1))
+    full = np.tile(np.array([N, N, N, N, 1], np.int64), (spec.n_views, 1))
     assert bench.algorithmic_bytes(spec, "warp", None, full) == 32 * N
     assert bench.algorithmic_bytes(spec, "warp_add", None, full) == 36 * N
     assert bench.algorithmic_bytes(spec, "motion", None, full) == 36 * N + 8 * D
     N, D = 1000, 10
-    # per view: read set R, support H, union U, layouts (1 natural, 2 transposed, 3 both)
-    vox = np.array([[600, 500, 650, 1], [620, 510, 640, 3], [630, 520, 660, 2],
-                    [640, 530, 530, 1]], np.int64)
+    # per view: read set R, with shell S, support H, union U, layouts (1 natural, 2 transposed,
+    # 3 both)
+    vox = np.array([[600, 700, 500, 650, 1], [620, 720, 510, 640, 3], [630, 730, 520, 660, 2],
+                    [640, 740, 530, 530, 1]], np.int64)
     assert traffic.warp_bytes(vox) == (32 * 600 + 36 * 620 + 32 * 630 + 32 * 640) / 4
     assert traffic.warp_add_bytes(vox, N) == (36 * N + 3 * 24 * N + 12 * (510 + 520 + 530)) / 4
-    first = 8 * N + (16 + 4 * 1) * 630 + 8 * D             # view 1 preparing view 2 (1 layout)
-    middle = 8 * 660 + 8 * 520 + (16 + 4 * 1) * 640 + 8 * D   # view 2 preparing view 3
+    first = 8 * N + (16 + 4 * 1) * 730 + 8 * D             # view 1 preparing view 2 (1 layout)
+    middle = 8 * 660 + 8 * 520 + (16 + 4 * 1) * 740 + 8 * D   # view 2 preparing view 3
     last = 16 * 530 + 8 * D                                 # view 3: no prep
     assert traffic.motion_sweep_bytes(vox, N, D) == (first + middle + last) / 3
     assert traffic.motion_middle_bytes(vox, N, D) == (
-        8 * 640 + 8 * 510 + (16 + 4 * 1) * 630 + 8 * D + middle) / 2
+        8 * 640 + 8 * 510 + (16 + 4 * 1) * 730 + 8 * D + middle) / 2
 
 
 def test_launches_per_scan(bench):
