@@ -402,30 +402,57 @@ struct ConeCull {
                      // culled outside the shell (a check that nothing reads those cells)
 };
 
-// Margin-raised form j at voxel index (x, y, z).
+// Margin-raised form j at voxel index (x, y, z); the z term last, so that a column's part
+// (cone_column) is the same partial sum.
+__host__ __device__ inline float cone_form_xy(const ConeCull &C, int j, float x, float y) {
+    return fmaf(C.n[j][0], x, fmaf(C.n[j][1], y, C.k[j]));
+}
 __host__ __device__ inline float cone_form(const ConeCull &C, int j, float x, float y, float z) {
-    return fmaf(C.n[j][0], x, fmaf(C.n[j][1], y, fmaf(C.n[j][2], z, C.k[j])));
+    return fmaf(C.n[j][2], z, cone_form_xy(C, j, x, y));
+}
+
+// A column's share of the read-set test: the fan forms (0, 1: no z term) and the xy parts of the
+// axial forms (2, 3), so that each voxel of the column costs two FMAs and compares.
+struct ConeColumn {
+    float p2, p3;
+    bool fan, shell_xy;
+};
+__host__ __device__ inline ConeColumn cone_column(const ConeCull &C, int ix, int iy) {
+    const float x = (float)ix, y = (float)iy;
+    ConeColumn c;
+    c.fan = cone_form_xy(C, 0, x, y) >= 0.f && cone_form_xy(C, 1, x, y) >= 0.f;
+    c.p2 = cone_form_xy(C, 2, x, y);
+    c.p3 = cone_form_xy(C, 3, x, y);
+    c.shell_xy = ix < C.sx || ix >= C.nx - C.sx || iy < C.sx || iy >= C.ny - C.sx;
+    return c;
+}
+__host__ __device__ inline bool cone_keep_at(const ConeCull &C, const ConeColumn &c, int iz) {
+    if (C.mode == 0) return true;
+    const float z = (float)iz;
+    return c.fan && fmaf(C.n[2][2], z, c.p2) >= 0.f && fmaf(C.n[3][2], z, c.p3) >= 0.f;
+}
+__host__ __device__ inline bool cone_shell_at(const ConeCull &C, const ConeColumn &c, int iz) {
+    return c.shell_xy || iz < C.sz || iz >= C.nz - C.sz;
 }
 
 // Voxel (ix, iy, iz) is in the read set (or the cull is off).
 __host__ __device__ inline bool cone_keep(const ConeCull &C, int ix, int iy, int iz) {
-    if (C.mode == 0) return true;
-    const float x = (float)ix, y = (float)iy, z = (float)iz;
-    return cone_form(C, 0, x, y, z) >= 0.f && cone_form(C, 1, x, y, z) >= 0.f &&
-           cone_form(C, 2, x, y, z) >= 0.f && cone_form(C, 3, x, y, z) >= 0.f;
+    return cone_keep_at(C, cone_column(C, ix, iy), iz);
 }
 
 // Voxel (ix, iy, iz) lies in the edge shell: within sx of a transverse edge or sz of a z edge
 // (a superset of the cells clamped windows read).
 __host__ __device__ inline bool cone_shell(const ConeCull &C, int ix, int iy, int iz) {
-    return ix < C.sx || ix >= C.nx - C.sx || iy < C.sx || iy >= C.ny - C.sx || iz < C.sz ||
-           iz >= C.nz - C.sz;
+    return cone_shell_at(C, cone_column(C, ix, iy), iz);
 }
 
 // A culled voxel whose write is made as a NaN in mode 2: one outside the read set and outside
 // the edge shell (shell cells are read with zero weights, so they keep their earlier values).
+__host__ __device__ inline bool cone_poison_at(const ConeCull &C, const ConeColumn &c, int iz) {
+    return C.mode == 2 && !cone_shell_at(C, c, iz) && !cone_keep_at(C, c, iz);
+}
 __host__ __device__ inline bool cone_poison(const ConeCull &C, int ix, int iy, int iz) {
-    return C.mode == 2 && !cone_shell(C, ix, iy, iz) && !cone_keep(C, ix, iy, iz);
+    return cone_poison_at(C, cone_column(C, ix, iy), iz);
 }
 
 // A box of centre (x, y, z) and half-widths (hx, hy, hz), on top of the built-in margin, meets

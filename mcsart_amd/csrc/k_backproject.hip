@@ -82,14 +82,15 @@ struct MotionIn {
     float4 c;   // (Ga, Gb, H, x): one 16-byte load
     bool hit, need, poison;
 };
-__device__ inline MotionIn motion_load(const MotionStep &m, int id, int ix, int iy, int iz,
+// nc: the voxel's column's share of the next view's read-set test (cone_column).
+__device__ inline MotionIn motion_load(const MotionStep &m, const ConeColumn &nc, int id, int iz,
                                        bool hit) {
     MotionIn in;
     in.hit = hit || !m.skip_zero;
     // the shell too: an earlier view's g there may have overflowed (the next projector reads
     // shell cells with zero weights, and 0 times Inf is NaN)
-    in.need = !m.last && (cone_keep(m.next, ix, iy, iz) || cone_shell(m.next, ix, iy, iz));
-    in.poison = !m.last && cone_poison(m.next, ix, iy, iz);   // never read: NaN in mode 2
+    in.need = !m.last && (cone_keep_at(m.next, nc, iz) || cone_shell_at(m.next, nc, iz));
+    in.poison = !m.last && cone_poison_at(m.next, nc, iz);   // never read: NaN in mode 2
     const float2 a = (m.first || !(in.hit || in.need)) ? make_float2(0.f, 0.f) : m.acc[id];
     in.sv = a.x;
     in.sf = a.y;
@@ -173,6 +174,9 @@ __global__ void __launch_bounds__(BX *BY) k_backproject(GeomArgs g, double c, do
         col[k] = min(ia_lo + k, last);
     }
     const bool wide = acount > WA;   // rare: handled by the generic loop below
+    // the motion step's test of the next view's read set: its column part once per column
+    ConeColumn ncol{};
+    if (EPI == EPI_MOTION && live) ncol = cone_column(mot.next, ix, iy);
 
     // ---- optional LDS staging of the workgroup's detector rectangle
     __shared__ float2 stage[STAGE ? LDS_CELLS : 1];
@@ -289,7 +293,7 @@ __global__ void __launch_bounds__(BX *BY) k_backproject(GeomArgs g, double c, do
             const int n = (acount > 0 && blim > 0.f) ? (int)ceilf(blim) : 0;
             MotionIn min_{};
             float x_old = 0.f;
-            if (EPI == EPI_MOTION) min_ = motion_load(mot, id, ix, iy, iz, n > 0);
+            if (EPI == EPI_MOTION) min_ = motion_load(mot, ncol, id, iz, n > 0);
             if (EPI == EPI_ADD) x_old = add.x[id];
             // row weight (:164) as a difference of the clamped boundary F(b) = min(max(b, lo), hi)
             const float bhi = fminf(bmaxL, top);
