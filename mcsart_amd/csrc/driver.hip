@@ -515,13 +515,13 @@ int read_set_ranges(const ConeCull &C, int ix, int iy, ZRange *out, bool shell) 
         out[0] = ZRange{0, nz};
         return 1;
     }
+    const float x = (float)ix, y = (float)iy;
+    if (!(cone_form(C, 0, x, y, 0.f) >= 0.f && cone_form(C, 1, x, y, 0.f) >= 0.f)) return 0;
     int k = 0;
-    if (shell) {
+    if (shell) {   // z edges, in the fan's columns
         out[k++] = ZRange{0, std::min(C.sz, nz)};
         out[k++] = ZRange{std::max(nz - C.sz, 0), nz};
     }
-    const float x = (float)ix, y = (float)iy;
-    if (!(cone_form(C, 0, x, y, 0.f) >= 0.f && cone_form(C, 1, x, y, 0.f) >= 0.f)) return k;
     // form 2 grows with z, form 3 falls
     const int lo = first_true(nz, [&](int iz) { return cone_form(C, 2, x, y, (float)iz) >= 0.f; });
     const int hi = first_true(nz, [&](int iz) { return cone_form(C, 3, x, y, (float)iz) < 0.f; });

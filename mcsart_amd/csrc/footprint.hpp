@@ -432,7 +432,7 @@ __host__ __device__ inline bool cone_keep_at(const ConeCull &C, const ConeColumn
     return c.fan && fmaf(C.n[2][2], z, c.p2) >= 0.f && fmaf(C.n[3][2], z, c.p3) >= 0.f;
 }
 __host__ __device__ inline bool cone_shell_at(const ConeCull &C, const ConeColumn &c, int iz) {
-    return c.shell_xy || iz < C.sz || iz >= C.nz - C.sz;
+    return c.shell_xy || (c.fan && (iz < C.sz || iz >= C.nz - C.sz));
 }
 
 // Voxel (ix, iy, iz) is in the read set (or the cull is off).
@@ -440,8 +440,9 @@ __host__ __device__ inline bool cone_keep(const ConeCull &C, int ix, int iy, int
     return cone_keep_at(C, cone_column(C, ix, iy), iz);
 }
 
-// Voxel (ix, iy, iz) lies in the edge shell: within sx of a transverse edge or sz of a z edge
-// (a superset of the cells clamped windows read).
+// Voxel (ix, iy, iz) lies in the edge shell: within sx of a transverse edge, or within sz of a
+// z edge in a column that passes the fan forms (a superset of the cells clamped windows read: a
+// window clamped along z sits beside its ray's footprint transversely).
 __host__ __device__ inline bool cone_shell(const ConeCull &C, int ix, int iy, int iz) {
     return cone_shell_at(C, cone_column(C, ix, iy), iz);
 }

@@ -199,11 +199,9 @@ int main(int argc, char **argv) {
             for (int iy = 0; iy < g.ny; ++iy)
                 for (int ix = 0; ix < g.nx; ++ix) {
                     const bool keep = cone_keep(rs, ix, iy, iz);
-                    const bool shell = ix < rs.sx || ix >= g.nx - rs.sx || iy < rs.sx ||
-                                       iy >= g.ny - rs.sx || iz < rs.sz || iz >= g.nz - rs.sz;
+                    const bool shell = cone_shell(rs, ix, iy, iz);
                     nkeep += keep;
                     nprep += keep || shell;
-                    if (shell != cone_shell(rs, ix, iy, iz)) ++viol;   // the two agree
                     keepm[(size_t)ix + (size_t)g.nx * (iy + (size_t)g.ny * iz)] = keep || shell;
                     const unsigned char mk = M.m[(size_t)ix + (size_t)g.nx * (iy + (size_t)g.ny * iz)];
                     // read with a nonzero weight: in the read set; read at all: or in the shell
