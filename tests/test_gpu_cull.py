@@ -110,6 +110,29 @@ def test_culling_changes_nothing_offset_detector(ctx):
     _check_modes(ctx, g, ang, vol, flo, [0, 1], proj, fields, nonneg=False)
 
 
+@pytest.mark.parametrize("shape", ["odd sizes", "coarse detector"])
+def test_culling_changes_nothing_other_kernels(ctx, shape):
+    """Odd volume sizes (planes half a voxel off the integer grid), and a detector coarser than
+    the voxels, whose footprints outgrow the fixed windows (the wide projector kernel reads
+    exactly each footprint's cells)."""
+    if shape == "odd sizes":
+        g = Geometry.from_user(37, 37, 21, 61, 29, da=1.1, db=0.9, ai=0.3, bi=0.7, SO=90.0,
+                               SD=150.0)
+    else:
+        g = Geometry.from_user(40, 40, 32, 20, 16, da=4.0, db=4.0, SO=100.0, SD=160.0)
+    V = 24
+    ang = uniform_angles(V)
+    vox = Voxel(2.0, 2.0)
+    ph = ellipsoid_phantom(g.nx, g.ny, g.nz, vox, 31)
+    vol, flo = breathing_trace(V, 32, dt_s=0.1)
+    fields = list(motion_fields(g.nx, g.ny, g.nz, vox, 33)) + \
+        list(motion_fields(g.nx, g.ny, g.nz, vox, 34, scale=0.3))
+    import oracle
+    proj = np.stack([oracle.projection(oracle.deform(ph, vol[i], flo[i], fields), g, ang[i])
+                     for i in range(V)]).astype(np.float32)
+    _check_modes(ctx, g, ang, vol, flo, [1, 0, 1], proj, fields)
+
+
 def test_culling_changes_nothing_static_overload(ctx):
     """Overload 1 (no warps; its motion sweep steps views by 10, so each motion step prepares a
     view ten ahead)."""
