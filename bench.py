@@ -661,6 +661,14 @@ def main():
         "host_boundary_ms_per_scan": round(host_ms, 1),
         "concurrent": conc,
         "launch": "one HIP graph per sweep",
+        "culling": {
+            "on": True,
+            "note": "warp, warp-add and motion step skip the voxels no later kernel sees; "
+                    "results identical bit for bit (mcsart_job_set_culling)",
+            "mean_fraction_of_voxels": {
+                "read_set": round(float(np.mean(voxels[:, 0])) / g.nvox, 4),
+                "read_set_with_shell": round(float(np.mean(voxels[:, 1])) / g.nvox, 4),
+                "u_nonzero_possible": round(float(np.mean(voxels[:, 2])) / g.nvox, 4)}},
         "tuning": tuning,
         "output_finite_fraction": finite,
     }
