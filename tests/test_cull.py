@@ -58,6 +58,10 @@ GEOMETRIES = {
     "three slices": Geometry.from_user(64, 64, 3, 96, 80, da=1.0, db=1.0, SO=200, SD=300),
     "source close to the volume": Geometry.from_user(64, 64, 32, 200, 150, da=1.5, db=1.5, SO=50,
                                                      SD=120),
+    # the orbit just outside the volume's corners (the least SO allows) and a detector just past
+    # the source: the fan reaches a third of the volume, planes near a corner lie behind the source
+    "source at the corners' circle": Geometry.from_user(64, 64, 32, 64, 80, da=0.2, db=1.0,
+                                                        SO=45.3, SD=47.0),
 }
 
 
