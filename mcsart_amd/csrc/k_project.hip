@@ -267,21 +267,21 @@ __device__ inline float clampf(float x, float lo, float hi) {
 }
 
 // Weights of the W-voxel window over [lo, hi], 0 <= lo <= hi <= n (bounds already clamped into
-// the volume; lo == hi, all weights 0, for a plane outside it).
+// the volume; lo == hi, all weights 0, for a plane outside it).  The window covers the
+// footprint, hi <= f0 + W (W is the most cells any footprint of the view spans, view_info; at
+// the volume's top edge f0 + W = n >= hi), so the last boundary min(f0 + W, hi) is hi itself.
 template <int W>
 __device__ inline void window(float lo, float hi, float top, float (&w)[W], float &sw, float &f0) {
     f0 = clampf(floorf(lo), 0.f, top);   // window start, inside [0, n - W]
     float F = lo;   // F(f0) = max(f0, lo) = lo: 0 <= lo, so f0 <= floor(lo) <= lo
-    const float F0 = F;
 #pragma unroll
     for (int k = 0; k < W; ++k) {
-        // interior boundaries clamp to [lo, hi]; the last one is >= lo (f0 + W > lo, or = n)
-        const float Fn = (k + 1 < W) ? clampf(f0 + (float)(k + 1), lo, hi)
-                                     : fminf(f0 + (float)W, hi);
+        // interior boundaries clamp to [lo, hi]; the last one is hi
+        const float Fn = (k + 1 < W) ? clampf(f0 + (float)(k + 1), lo, hi) : hi;
         w[k] = Fn - F;
         F = Fn;
     }
-    sw = F - F0;
+    sw = hi - lo;
 }
 
 // Footprint of one plane for the R rays of a lane.  The bounds are clamped into the volume

@@ -41,13 +41,15 @@ struct Marks {
 float clampf(float x, float lo, float hi) { return std::fmin(std::fmax(x, lo), hi); }
 
 // The rows kernel's window weights (k_project.hip, window()): F(b) = clamp(b, lo, hi) with the
-// window's interior boundaries, the last one min(f0 + W, hi).
+// window's interior boundaries, the last one hi: the window covers the footprint, which is
+// counted here where it does not (a window whose last boundary f0 + W falls below hi).
+long long window_misses = 0;
 void window_weights(float lo, float hi, float top, int W, float *w, float &f0) {
     f0 = clampf(std::floor(lo), 0.f, top);
+    if (f0 + (float)W < hi) ++window_misses;
     float F = lo;
     for (int k = 0; k < W; ++k) {
-        const float Fn = (k + 1 < W) ? clampf(f0 + (float)(k + 1), lo, hi)
-                                     : std::fmin(f0 + (float)W, hi);
+        const float Fn = (k + 1 < W) ? clampf(f0 + (float)(k + 1), lo, hi) : hi;
         w[k] = Fn - F;
         F = Fn;
     }
@@ -248,8 +250,8 @@ int main(int argc, char **argv) {
                         (double)ang, viol, vsup, vi.wt, vi.wz);
     }
     if (view > 0) print_counts(view - 1, reads.back(), preps.back(), prev_nhit, prev_nhit);
-    std::printf("views %zu kept %.4f support %.4f bad_read %lld bad_support %lld\n",
-                angles.size(), kept / angles.size(), support / angles.size(), bad_read,
-                bad_support);
-    return (bad_read || bad_support) ? 1 : 0;
+    std::printf("views %zu kept %.4f support %.4f bad_read %lld bad_support %lld "
+                "window_misses %lld\n", angles.size(), kept / angles.size(),
+                support / angles.size(), bad_read, bad_support, window_misses);
+    return (bad_read || bad_support || window_misses) ? 1 : 0;
 }

@@ -3,7 +3,8 @@
 tests/cull_check.cpp replays every cell the three projector kernels read, from the kernels' own
 per-ray constants, and checks that each lies in the view's read set (the voxels the warp and the
 motion step still write); it also checks that every voxel whose backprojection can be nonzero lies
-in the view's u support (where the warp-add still samples).  A set narrowed by one voxel must fail
+in the view's u support (where the warp-add still samples), and that every window of the rows
+projector covers its footprint (its last weight boundary is the footprint's end).  A set narrowed by one voxel must fail
 the same replay, so the replay does see the sets' edges.  The GPU side of the same property (the
 sweeps give the same results bit for bit with culling on, off, and with every skipped write made
 as a NaN) is in tests/test_gpu_cull.py.
@@ -42,6 +43,12 @@ def _run(binary, g, n_uniform, shrink=0.0, extra=()):
     return r.returncode, last
 
 
+def _fields(line):
+    """The summary line's "name value" pairs."""
+    tok = line.split()
+    return {tok[i]: float(tok[i + 1]) for i in range(0, len(tok) - 1, 2)}
+
+
 # exact multiples of 45 degrees (branch ties) and their float32 neighbours
 TIES = ("0.7853982", "0.78539824", "2.3561945", "3.1415925", "1e-6", "4.712389")
 
@@ -75,8 +82,7 @@ def test_read_set_at_256x256x128(cull_check):
     g = configs.c3_clinical().geometry
     rc, line = _run(cull_check, g, 0, 0.0, ("0.3", "0.7853982", "2.0"))
     assert rc == 0, line
-    kept = float(line.split()[3])
-    assert 0.55 < kept < 0.75, line   # the detector's fan misses a third of this volume
+    assert 0.55 < _fields(line)["kept"] < 0.75, line   # the fan misses a third of this volume
 
 
 @pytest.mark.parametrize("name", ["128^3", "offset truncated detector", "odd sizes",
@@ -103,5 +109,5 @@ def test_set_sizes_match_brute_force(cull_check, name):
 def test_replay_catches_a_set_one_voxel_too_narrow(cull_check, name):
     rc, line = _run(cull_check, GEOMETRIES[name], 16, 1.0)
     assert rc == 1, line
-    bad_read, bad_support = int(line.split()[-3]), int(line.split()[-1])
-    assert bad_read > 0 and bad_support > 0, line
+    f = _fields(line)
+    assert f["bad_read"] > 0 and f["bad_support"] > 0 and f["window_misses"] == 0, line
