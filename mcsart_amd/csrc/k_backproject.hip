@@ -277,6 +277,11 @@ __global__ void __launch_bounds__(BX *BY) k_backproject(GeomArgs g, double c, do
             }
             return make_float2(s0, s1);
         };
+        // The row sums depend on the column and the row only, and a voxel's first row is often
+        // the previous voxel's last one: that row's sum is carried (prow, psum) instead of read
+        // again.  The accumulators see the same products in the same row order.
+        int prow = -2;
+        float2 psum = make_float2(0.f, 0.f);
         for (int iz = iz0; iz < iz1; ++iz) {
             const int id = ix + iy * g.nx + iz * plane;
             // axial bounds (:131-132,:137-138) in anchor-relative FP32
@@ -300,12 +305,19 @@ __global__ void __launch_bounds__(BX *BY) k_backproject(GeomArgs g, double c, do
             auto F = [&](int b) { return fminf(fmaxf((float)b, bminL), bhi); };
             float acc = 0.f, acc1 = 0.f;
             if (!wide) {
+                // row 0 from the carry when it is the previous voxel's last row (n > 0 only:
+                // the carried sum may be non-finite and must not meet a zero weight)
+                const bool carry = n > 0 && ib_lo == prow;
+                const float F0 = F(0), F1 = F(1);
+                const float cx = fmaf(F1 - F0, psum.x, acc), cy = fmaf(F1 - F0, psum.y, acc1);
+                acc = carry ? cx : acc;
+                acc1 = carry ? cy : acc1;
                 // two rows per step, all their loads in flight; a second row past the end reads
                 // the first again with weight F(lb+2) - F(lb+1) = 0.  F at the step's upper
                 // boundary is the next step's lower one (lbf counts rows in float: exact).
-                float Fa = F(0), lbf = 0.f;
+                float Fa = carry ? F1 : F0, lbf = carry ? 1.f : 0.f;
 #pragma unroll 1   // one or two steps per voxel: unrolling only adds remainder code
-                for (int lb = 0; lb < n; lb += 2) {
+                for (int lb = carry ? 1 : 0; lb < n; lb += 2) {
                     const int r0 = ib_lo + lb - rb0;
                     const int r1 = ib_lo + min(lb + 1, n - 1) - rb0;
                     const float Fb = fminf(fmaxf(lbf + 1.f, bminL), bhi);
@@ -318,7 +330,9 @@ __global__ void __launch_bounds__(BX *BY) k_backproject(GeomArgs g, double c, do
                     acc1 = fmaf(w1, s1.y, acc1);
                     Fa = Fc;
                     lbf += 2.f;
+                    psum = s1;   // row min(lb + 1, n - 1): the voxel's last row after its last step
                 }
+                if (n > 0) prow = ib_lo + n - 1;
             } else {
                 for (int lb = 0; lb < n; ++lb) {
                     const float wb = F(lb + 1) - F(lb);
