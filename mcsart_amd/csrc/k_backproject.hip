@@ -246,6 +246,13 @@ __global__ void __launch_bounds__(BX *BY) k_backproject(GeomArgs g, double c, do
         }
         __syncthreads();
     }
+    // the wavefront's widest footprint, while every lane is active (lanes past the volume and
+    // columns without cells count 0)
+    int am = acount;
+    if (STAGE) {
+        for (int off = 32; off > 0; off >>= 1) am = max(am, __shfl_xor(am, off, 64));
+        am = __builtin_amdgcn_readfirstlane(am);
+    }
     if (!live) return;
 
     // Every value the voxels read lies in the staged rectangle when `staged` (uniform over the
@@ -254,8 +261,12 @@ __global__ void __launch_bounds__(BX *BY) k_backproject(GeomArgs g, double c, do
     const void *gsrc = MODE == BACK_PLAIN ? (const void *)proj : (const void *)rp;
     constexpr unsigned ES = MODE == BACK_PLAIN ? 4u : 8u;   // bytes per detector element
     const int plane = g.nx * g.ny;
-    auto run = [&](auto from_lds) {
+    // W: the cells per row the loops read, WA or (from the LDS stage) the widest footprint of
+    // the wavefront when that is narrower; the slots past it carry weight 0 on a cell the
+    // footprint already counts, so leaving them out adds nothing
+    auto run = [&](auto from_lds, auto width) {
         constexpr bool LDSRC = decltype(from_lds)::value;
+        constexpr int W = decltype(width)::value;
         const int rs = LDSRC ? RA : g.na, rb0 = LDSRC ? B0 : 0, ra0 = LDSRC ? A0 : 0;
         int cof[WA];
 #pragma unroll
@@ -265,13 +276,13 @@ __global__ void __launch_bounds__(BX *BY) k_backproject(GeomArgs g, double c, do
             return det_at<MODE>(gsrc, (unsigned)e * ES);
         };
         auto row_sum = [&](int r) -> float2 {   // sum_k wa[k] * value(row r, cell col[k])
-            float2 v[WA];
+            float2 v[W];
             const int rowe = r * rs;
 #pragma unroll
-            for (int k = 0; k < WA; ++k) v[k] = value(rowe + cof[k]);
+            for (int k = 0; k < W; ++k) v[k] = value(rowe + cof[k]);
             float s0 = 0.f, s1 = 0.f;
 #pragma unroll
-            for (int k = 0; k < WA; ++k) {
+            for (int k = 0; k < W; ++k) {
                 s0 = fmaf(wa[k], v[k].x, s0);
                 s1 = fmaf(wa[k], v[k].y, s1);
             }
@@ -365,10 +376,18 @@ __global__ void __launch_bounds__(BX *BY) k_backproject(GeomArgs g, double c, do
             }
         }
     };
-    if (STAGE && __builtin_amdgcn_readfirstlane((int)staged))
-        run(std::true_type{});
-    else
-        run(std::false_type{});
+    using Full = std::integral_constant<int, WA>;
+    if (STAGE && __builtin_amdgcn_readfirstlane((int)staged)) {
+        constexpr int W2 = WA - 2 > 2 ? WA - 2 : 2, W1 = WA - 1 > 2 ? WA - 1 : 2;
+        if (am <= W2)
+            run(std::true_type{}, std::integral_constant<int, W2>{});
+        else if (am <= W1)
+            run(std::true_type{}, std::integral_constant<int, W1>{});
+        else
+            run(std::true_type{}, Full{});
+    } else {
+        run(std::false_type{}, Full{});
+    }
 }
 
 // Voxels per lane along z: the longest slab (up to 8) that still leaves >= 1024 workgroups.
