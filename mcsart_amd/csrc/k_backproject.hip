@@ -275,11 +275,21 @@ __global__ void __launch_bounds__(BX *BY) k_backproject(GeomArgs g, double c, do
             if (LDSRC) return stage[e];
             return det_at<MODE>(gsrc, (unsigned)e * ES);
         };
-        auto row_sum = [&](int r) -> float2 {   // sum_k wa[k] * value(row r, cell col[k])
-            float2 v[W];
-            const int rowe = r * rs;
+        // the same values by byte offset: EB bytes per element of the source, rows rsB apart
+        constexpr unsigned EB = LDSRC ? 8u : ES;
+        const unsigned rsB = (unsigned)rs * EB;
+        unsigned cofB[WA];
 #pragma unroll
-            for (int k = 0; k < W; ++k) v[k] = value(rowe + cof[k]);
+        for (int k = 0; k < WA; ++k) cofB[k] = (unsigned)cof[k] * EB;
+        auto value_b = [&](unsigned b) -> float2 {
+            if (LDSRC) return *(const float2 *)((const char *)stage + b);
+            return det_at<MODE>(gsrc, b);
+        };
+        // sum_k wa[k] * value(row, cell col[k]) for the row that starts at byte offset rowb
+        auto row_sum = [&](unsigned rowb) -> float2 {
+            float2 v[W];
+#pragma unroll
+            for (int k = 0; k < W; ++k) v[k] = value_b(rowb + cofB[k]);
             float s0 = 0.f, s1 = 0.f;
 #pragma unroll
             for (int k = 0; k < W; ++k) {
@@ -311,9 +321,12 @@ __global__ void __launch_bounds__(BX *BY) k_backproject(GeomArgs g, double c, do
             float x_old = 0.f;
             if (EPI == EPI_MOTION) min_ = motion_load(mot, ncol, id, iz, n > 0);
             if (EPI == EPI_ADD) x_old = add.x[id];
-            // row weight (:164) as a difference of the clamped boundary F(b) = min(max(b, lo), hi)
+            // row weight (:164) as a difference of the clamped boundary F(b) = min(max(b, lo), hi),
+            // one median: with rows (n > 0) lo <= hi (bminL < 1 <= top and bminL <= bmaxL), where
+            // the median of (b, lo, hi) is that clamp exactly; F is used only then
             const float bhi = fminf(bmaxL, top);
-            auto F = [&](int b) { return fminf(fmaxf((float)b, bminL), bhi); };
+            auto Ff = [&](float b) { return __builtin_amdgcn_fmed3f(b, bminL, bhi); };
+            auto F = [&](int b) { return Ff((float)b); };
             float acc = 0.f, acc1 = 0.f;
             if (!wide) {
                 // row 0 from the carry when it is the previous voxel's last row (n > 0 only:
@@ -326,21 +339,28 @@ __global__ void __launch_bounds__(BX *BY) k_backproject(GeomArgs g, double c, do
                 // two rows per step, all their loads in flight; a second row past the end reads
                 // the first again with weight F(lb+2) - F(lb+1) = 0.  F at the step's upper
                 // boundary is the next step's lower one (lbf counts rows in float: exact).
-                float Fa = carry ? F1 : F0, lbf = carry ? 1.f : 0.f;
+                // a step reads rows lb and min(lb + 1, n - 1): byte offsets e0 and
+                // min(e0 + rsB, el), el the voxel's last row (rows at or above the stage's first)
+                float Fa = carry ? F1 : F0;
+                const int lb0 = carry ? 1 : 0;
+                unsigned e0 = (unsigned)(ib_lo + lb0 - rb0) * rsB;
+                const unsigned el = (unsigned)(ib_lo + n - 1 - rb0) * rsB;
+                const float nf = (float)n;
 #pragma unroll 1   // one or two steps per voxel: unrolling only adds remainder code
-                for (int lb = carry ? 1 : 0; lb < n; lb += 2) {
-                    const int r0 = ib_lo + lb - rb0;
-                    const int r1 = ib_lo + min(lb + 1, n - 1) - rb0;
-                    const float Fb = fminf(fmaxf(lbf + 1.f, bminL), bhi);
-                    const float Fc = fminf(fmaxf(lbf + 2.f, bminL), bhi);
-                    const float2 s0 = row_sum(r0), s1 = row_sum(r1);
+                for (float lbf = (float)lb0; lbf < nf; lbf += 2.f, e0 += 2u * rsB) {
+                    // e0 opaque to loop strength reduction (an empty asm statement, no
+                    // instruction): one offset per row plus a cell offset per load, instead of
+                    // one induction variable per cell
+                    asm("" : "+v"(e0));
+                    const unsigned e1 = min(e0 + rsB, el);
+                    const float Fb = Ff(lbf + 1.f), Fc = Ff(lbf + 2.f);
+                    const float2 s0 = row_sum(e0), s1 = row_sum(e1);
                     const float w0 = Fb - Fa, w1 = Fc - Fb;
                     acc = fmaf(w0, s0.x, acc);
                     acc1 = fmaf(w0, s0.y, acc1);
                     acc = fmaf(w1, s1.x, acc);
                     acc1 = fmaf(w1, s1.y, acc1);
                     Fa = Fc;
-                    lbf += 2.f;
                     psum = s1;   // row min(lb + 1, n - 1): the voxel's last row after its last step
                 }
                 if (n > 0) prow = ib_lo + n - 1;
