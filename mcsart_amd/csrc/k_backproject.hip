@@ -303,10 +303,11 @@ __global__ void __launch_bounds__(BX *BY) k_backproject(GeomArgs g, double c, do
         // again.  The accumulators see the same products in the same row order.
         int prow = -2;
         float2 psum = make_float2(0.f, 0.f);
-        for (int iz = iz0; iz < iz1; ++iz) {
-            const int id = ix + iy * g.nx + iz * plane;
+        // voxel index and slab-relative z carried from voxel to voxel (no multiply per voxel)
+        int id = ix + iy * g.nx + iz0 * plane;
+        float d = 0.f;
+        for (int iz = iz0; iz < iz1; ++iz, id += plane, d += 1.f) {
             // axial bounds (:131-132,:137-138) in anchor-relative FP32
-            const float d = (float)(iz - iz0);
             const float L0 = fmaf(bl.q0, d, bl.l0), L1 = fmaf(bl.q1, d, bl.l1);
             const float L2 = fmaf(bl.q2, d, bl.l2), L3 = fmaf(bl.q2, d, bl.l3);
             const float lBmin = fminf(fminf(L0, L1), L3), lBmax = fmaxf(fmaxf(L0, L1), L2);
@@ -343,8 +344,14 @@ __global__ void __launch_bounds__(BX *BY) k_backproject(GeomArgs g, double c, do
                 // min(e0 + rsB, el), el the voxel's last row (rows at or above the stage's first)
                 float Fa = carry ? F1 : F0;
                 const int lb0 = carry ? 1 : 0;
-                unsigned e0 = (unsigned)(ib_lo + lb0 - rb0) * rsB;
-                const unsigned el = (unsigned)(ib_lo + n - 1 - rb0) * rsB;
+                // (from the stage rows and strides are below 2^12 and 2^15: 24-bit products)
+                auto mulB = [&](int r) {
+                    return LDSRC ? __umul24((unsigned)r, rsB) : (unsigned)r * rsB;
+                };
+                unsigned rowB = mulB(ib_lo - rb0);
+                asm("" : "+v"(rowB));   // keeps el one 24-bit multiply-add off rowB
+                unsigned e0 = rowB + (carry ? rsB : 0u);
+                const unsigned el = rowB + mulB(n - 1);
                 const float nf = (float)n;
 #pragma unroll 1   // one or two steps per voxel: unrolling only adds remainder code
                 for (float lbf = (float)lb0; lbf < nf; lbf += 2.f, e0 += 2u * rsB) {
