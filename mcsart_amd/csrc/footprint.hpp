@@ -299,10 +299,20 @@ struct ViewInfo {
     int wa;          // backprojector transverse window (cells)
     double t_box;    // how far (voxels) a projector footprint's transverse bounds reach past the
                      // ray's own pyramid: the four-ratio box against the two edge ratios
+    double zrow;     // the largest z extent of a projector footprint in any plane, which is also
+                     // how far apart the footprints of adjacent detector rows lie (z voxels)
 };
 
+// The rays of R adjacent detector rows (one detector column) can read one window of WZ + 1
+// z-rows together when their footprints' lower bounds lie less than one voxel apart in every
+// plane, so that their windows start at most one row apart: (R - 1) zrow below 1, with a margin
+// for the FP32 rounding of the per-plane bounds.
+inline bool rows_share(const ViewInfo &v, int R) {
+    return R >= 2 && (double)(R - 1) * v.zrow < 1.0 - 1e-3;
+}
+
 inline ViewInfo view_info(const GeomArgs &g, double c, double s) {
-    ViewInfo v{0, 0, 0, 0, 0, 0.0};
+    ViewInfo v{0, 0, 0, 0, 0, 0.0, 0.0};
     double wt_max = 0.0, wz_max = 0.0;
     const double SO = g.SO, SD = g.SD, da = g.da, db = g.db;
     for (int ia = 0; ia < g.na; ++ia) {
@@ -343,6 +353,7 @@ inline ViewInfo view_info(const GeomArgs &g, double c, double s) {
     // a small margin absorbs the FP32 rounding of the per-plane bounds
     v.wt = (int)floor(wt_max * (1.0 + 1e-5) + 1e-5) + 2;
     v.wz = (int)floor(wz_max * (1.0 + 1e-5) + 1e-5) + 2;
+    v.zrow = wz_max;
     // backprojector: the widest transverse footprint of a voxel column, in cells.  The width
     // grows toward the source and with |y'|, so over the square of columns it is largest on the
     // boundary, and the boundary columns are scanned; a column that still exceeds the window is

@@ -294,6 +294,8 @@ struct PlaneFoot {
     float wt[WT], sw;
     float wz[R][WZ], swz[R], sc[R];
     unsigned off[R];   // byte offset of ray r's window corner
+    unsigned g[R];     // ray r's first z-row
+    unsigned base;     // byte offset of the window's first cell in z-row 0
 };
 
 template <int WT, int WZ, int R, int LOGES>
@@ -311,8 +313,10 @@ __device__ inline void plane_foot(const RayT &T, const RayZ (&Z)[R], float p, un
         float g0;
         window<WZ>(clampf(z1, 0.f, nzf), clampf(z2, 0.f, nzf), ztop, P.wz[r], P.swz[r], g0);
         P.sc[r] = __builtin_amdgcn_rcpf(width * (z2 - z1));
-        P.off[r] = ((__umul24((unsigned)(int)g0, sz) + t0) << LOGES) + pofs;
+        P.g[r] = (unsigned)(int)g0;
+        P.off[r] = ((__umul24(P.g[r], sz) + t0) << LOGES) + pofs;
     }
+    P.base = (t0 << LOGES) + pofs;
 }
 
 // The window's values: WT consecutive elements on each of WZ z-rows, from a wave-uniform base
@@ -329,8 +333,13 @@ __device__ inline void window_values(const char *__restrict__ vb, unsigned off, 
 }
 
 // PL planes from plane coordinate p: all footprints, then all gathers (in flight together),
-// then the weighted sums.
-template <int MODE, int WT, int WZ, int R, int PL>
+// then the weighted sums.  SH (image mode, R >= 2, when rows_share holds for the view): the R rays'
+// windows start at most one z-row apart, so rows gmin .. gmin + WZ - 1 and gmax + WZ - 1 hold every
+// ray's window: WZ + 1 gathers for the lane instead of WZ per ray.  Ray r applies its weights
+// shifted by g_r - gmin (0 or 1), a zero weight on the one row outside its window; for the finite
+// values of the image sweep that adds exactly nothing, so the sums equal the per-ray windows' bit
+// for bit, and every cell read is one of some ray's own window.
+template <int MODE, int WT, int WZ, int R, int PL, bool SH = false>
 __device__ inline void rows_step(const char *__restrict__ vb, const RayT &T, const RayZ (&Z)[R],
                                  float p, unsigned pofs, unsigned spB, unsigned sz, float ntf,
                                  float nzf, float ttop, float ztop, float (&acc0)[R],
@@ -367,6 +376,49 @@ __device__ inline void rows_step(const char *__restrict__ vb, const RayT &T, con
                 acc0[r] = fmaf(P[q].sc[r], s0, acc0[r]);
                 acc1[r] = fmaf(P[q].sc[r], s1, acc1[r]);
             }
+    } else if (SH && MODE == PROJ_IMAGE && R >= 2) {
+        constexpr int NZ = WZ + 1;
+        float val[PL][WT][NZ];
+        unsigned gmin[PL];
+#pragma unroll
+        for (int q = 0; q < PL; ++q) {
+            // g_r is monotone in r (the rows' footprints shift by one row spacing each)
+            gmin[q] = min(P[q].g[0], P[q].g[R - 1]);
+            const unsigned gmax = max(P[q].g[0], P[q].g[R - 1]);
+            unsigned orow[NZ];
+            orow[0] = (__umul24(gmin[q], sz) << LOGES) + P[q].base;
+#pragma unroll
+            for (int j = 1; j < WZ; ++j) orow[j] = orow[j - 1] + szB;
+            orow[WZ] = orow[WZ - 1] + (gmax != gmin[q] ? szB : 0u);
+#pragma unroll
+            for (int j = 0; j < NZ; ++j)
+#pragma unroll
+                for (int k = 0; k < WT; ++k)
+                    val[q][k][j] = *(const float *)(vb + orow[j] + (unsigned)k * 4u);
+        }
+#pragma unroll
+        for (int q = 0; q < PL; ++q)
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const bool up = P[q].g[r] != gmin[q];   // window one row above gmin
+                float wzs[NZ];
+#pragma unroll
+                for (int j = 0; j < NZ; ++j) {
+                    const float lo = j > 0 ? P[q].wz[r][j - 1] : 0.f;
+                    const float hi = j < WZ ? P[q].wz[r][j] : 0.f;
+                    wzs[j] = up ? lo : hi;
+                }
+                float s0 = 0.f;
+#pragma unroll
+                for (int k = 0; k < WT; ++k) {
+                    float u0 = 0.f;
+#pragma unroll
+                    for (int j = 0; j < NZ; ++j) u0 = fmaf(wzs[j], val[q][k][j], u0);
+                    s0 = fmaf(P[q].wt[k], u0, s0);
+                }
+                acc0[r] = fmaf(P[q].sc[r], s0, acc0[r]);
+                acc1[r] = fmaf(P[q].sc[r], P[q].sw * P[q].swz[r], acc1[r]);
+            }
     } else {
         float val[PL][R][WT][WZ];
 #pragma unroll
@@ -392,7 +444,7 @@ __device__ inline void rows_step(const char *__restrict__ vb, const RayT &T, con
     }
 }
 
-template <int MODE, int WT, int WZ, int R, bool SQ>
+template <int MODE, int WT, int WZ, int R, bool SQ, bool SH>
 __global__ void __launch_bounds__(RAYS * 16) k_project_rows(GeomArgs g, double c, double s,
                                                            ProjArgs a, int nsplit) {
     const int tx = threadIdx.x, ty = threadIdx.y;
@@ -449,10 +501,10 @@ __global__ void __launch_bounds__(RAYS * 16) k_project_rows(GeomArgs g, double c
         float p = (float)ip_lo + p0;
         unsigned pofs = (xb ? a.roffT : a.roffN) + (unsigned)ip_lo * spB;
         for (; ip + PL <= ip_hi; ip += PL, p += (float)PL, pofs += PL * spB)
-            rows_step<MODE, WT, WZ, R, PL>(a.rbase, T, Z, p, pofs, spB, sz, ntf, nzf, ttop, ztop,
+            rows_step<MODE, WT, WZ, R, PL, SH>(a.rbase, T, Z, p, pofs, spB, sz, ntf, nzf, ttop, ztop,
                                           acc0, acc1);
         for (; PL > 1 && ip < ip_hi; ++ip, p += 1.f, pofs += spB)
-            rows_step<MODE, WT, WZ, R, 1>(a.rbase, T, Z, p, pofs, spB, sz, ntf, nzf, ttop, ztop,
+            rows_step<MODE, WT, WZ, R, 1, SH>(a.rbase, T, Z, p, pofs, spB, sz, ntf, nzf, ttop, ztop,
                                           acc0, acc1);
     }
 
@@ -568,26 +620,30 @@ void pick_rows(const Geom &g, int rows, int split, int &R, int &S) {
     S = split > 0 ? split : split_for(R);
 }
 
-template <int MODE, int WT, int WZ, int R>
+template <int MODE, int WT, int WZ, int R, bool SH>
 void launch_rows(const GeomArgs &ga, const ViewTrig &t, const ProjArgs &a, hipStream_t st, int S) {
     const dim3 block(RAYS, S);
     const dim3 grid((ga.na + RAYS - 1) / RAYS, (ga.nb + R - 1) / R);
     const size_t lds = S > 1 ? sizeof(float) * 2 * S * R * RAYS : 0;
     if (ga.nx == ga.ny)
-        hipLaunchKernelGGL((k_project_rows<MODE, WT, WZ, R, true>), grid, block, lds, st, ga, t.c,
-                           t.s, a, S);
+        hipLaunchKernelGGL((k_project_rows<MODE, WT, WZ, R, true, SH>), grid, block, lds, st, ga,
+                           t.c, t.s, a, S);
     else
-        hipLaunchKernelGGL((k_project_rows<MODE, WT, WZ, R, false>), grid, block, lds, st, ga, t.c,
-                           t.s, a, S);
+        hipLaunchKernelGGL((k_project_rows<MODE, WT, WZ, R, false, SH>), grid, block, lds, st, ga,
+                           t.c, t.s, a, S);
 }
 
+// sh: the image mode's shared z-rows (rows_share holds for the view at this R)
 template <int MODE, int WT, int WZ>
 void launch_rows_r(const GeomArgs &ga, const ViewTrig &t, const ProjArgs &a, hipStream_t st,
-                   int R, int S) {
+                   int R, int S, bool sh) {
+    constexpr bool IMG = MODE == PROJ_IMAGE;
     if (MODE == PROJ_MOTION || R == 1)   // the motion mode runs one ray per lane
-        launch_rows<MODE, WT, WZ, 1>(ga, t, a, st, S);
-    else if (R == 2) launch_rows<MODE, WT, WZ, (MODE == PROJ_MOTION ? 1 : 2)>(ga, t, a, st, S);
-    else launch_rows<MODE, WT, WZ, (MODE == PROJ_MOTION ? 1 : 4)>(ga, t, a, st, S);
+        launch_rows<MODE, WT, WZ, 1, false>(ga, t, a, st, S);
+    else if (R == 2 && IMG && sh) launch_rows<MODE, WT, WZ, (IMG ? 2 : 1), IMG>(ga, t, a, st, S);
+    else if (R == 2) launch_rows<MODE, WT, WZ, (MODE == PROJ_MOTION ? 1 : 2), false>(ga, t, a, st, S);
+    else if (IMG && sh) launch_rows<MODE, WT, WZ, (IMG ? 4 : 1), IMG>(ga, t, a, st, S);
+    else launch_rows<MODE, WT, WZ, (MODE == PROJ_MOTION ? 1 : 4), false>(ga, t, a, st, S);
 }
 
 template <int MODE>
@@ -595,11 +651,12 @@ int launch_mode_rows(const Geom &g, const ViewTrig &t, const ViewInfo &vi, const
                      hipStream_t st, int R, int S) {
     const GeomArgs ga{g.nx, g.ny, g.nz, g.na, g.nb, g.da, g.db, g.ai, g.bi, g.SO, g.SD};
     const int code = (vi.wt <= 2 ? 0 : 1) * 2 + (vi.wz <= 2 ? 0 : 1);
+    const bool sh = rows_share(vi, R);
     switch (code) {
-        case 0: launch_rows_r<MODE, 2, 2>(ga, t, a, st, R, S); break;
-        case 1: launch_rows_r<MODE, 2, 3>(ga, t, a, st, R, S); break;
-        case 2: launch_rows_r<MODE, 3, 2>(ga, t, a, st, R, S); break;
-        default: launch_rows_r<MODE, 3, 3>(ga, t, a, st, R, S); break;
+        case 0: launch_rows_r<MODE, 2, 2>(ga, t, a, st, R, S, sh); break;
+        case 1: launch_rows_r<MODE, 2, 3>(ga, t, a, st, R, S, sh); break;
+        case 2: launch_rows_r<MODE, 3, 2>(ga, t, a, st, R, S, sh); break;
+        default: launch_rows_r<MODE, 3, 3>(ga, t, a, st, R, S, sh); break;
     }
     MCSART_CHECK_LAUNCH();
     return MCSART_OK;
