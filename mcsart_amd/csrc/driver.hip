@@ -870,10 +870,12 @@ extern "C" int mcsart_job_autotune(mcsart_job *j, mcsart_tuning *chosen) {
     const Geom &g = j->g;
     const long long D = g.ndet();
     hipStream_t st = ctx->stream;
-    // up to four views spread over the scan: footprint shapes change with the angle
+    // up to sixteen views spread over the scan at the midpoints of equal shares, so that a full
+    // circle is not sampled at the axis-aligned angles alone: footprint shapes, window sizes and
+    // the rays' branches change with the angle
     std::vector<int> views;
-    const int nv = std::min(j->V, 4);
-    for (int k = 0; k < nv; ++k) views.push_back((int)((long long)k * j->V / nv));
+    const int nv = std::min(j->V, 16);
+    for (int k = 0; k < nv; ++k) views.push_back((int)((2LL * k + 1) * j->V / (2LL * nv)));
     // transposed image for x-branch rays, into the job's scratch
     MCSART_HIP_TRY(hipStreamSynchronize(st));
     rc = launch_transpose(j->d_w1T, j->d_img, g.nx, g.ny, g.nz, st);
