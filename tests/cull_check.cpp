@@ -15,12 +15,14 @@
 // in FP64 from column_setup / column_b.
 //
 // Usage: cull_check nx ny nz na nb da db ai bi SO SD n_angles shrink [angle ...]
+// (environment: CULL_CHECK_FORCE_SHARE=1, see share_check)
 // shrink > 0 narrows both sets' margins by that many voxels (a check that the replay catches a
 // set that is too small).  Prints the kept fractions and the number of violations; exit status
 // 1 on any violation.
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <algorithm>
 #include <vector>
 
 #include "footprint.hpp"
@@ -55,8 +57,33 @@ void window_weights(float lo, float hi, float top, int W, float *w, float &f0) {
     }
 }
 
+// The image projector's shared z-rows (rows_share): a lane's R rays (detector rows ib0 ..
+// ib0 + R - 1, the last ones repeated past the detector) read rows gmin .. gmin + WZ - 1 and
+// gmax + WZ - 1 with gmin / gmax the smaller / larger window start of its first and last ray.
+// Counted here: planes where that misses a ray's window (starts more than one row apart, or the
+// end rays not holding the extremes) for R = 2 and 4 wherever rows_share allows them.
+long long share_misses = 0, shared_views = 0;
+// CULL_CHECK_FORCE_SHARE=1 checks R = 2 and 4 on every view, allowed or not (a check that the
+// replay catches rays whose windows lie too far apart to share)
+const bool force_share = std::getenv("CULL_CHECK_FORCE_SHARE") != nullptr;
+void share_check(const std::vector<int> &g0, int nb, int np, int R) {
+    for (int ib0 = 0; ib0 < nb; ib0 += R)
+        for (int ip = 0; ip < np; ++ip) {
+            int lo = 1 << 30, hi = -(1 << 30);
+            for (int r = 0; r < R; ++r) {
+                const int v = g0[(size_t)std::min(ib0 + r, nb - 1) * np + ip];
+                lo = std::min(lo, v);
+                hi = std::max(hi, v);
+            }
+            const int a = g0[(size_t)ib0 * np + ip];
+            const int b = g0[(size_t)std::min(ib0 + R - 1, nb - 1) * np + ip];
+            if (std::min(a, b) != lo || std::max(a, b) != hi || hi - lo > 1) ++share_misses;
+        }
+}
+
 // k_project_rows: windows W x WZ from the clamped lower bounds
-void rows_reads(const GeomArgs &g, double c, double s, int W, int WZ, Marks &M) {
+void rows_reads(const GeomArgs &g, const ViewInfo &vi, double c, double s, int W, int WZ,
+                Marks &M) {
     for (int ia = 0; ia < g.na; ++ia) {
         const RayDir d = ray_dir(g, c, s, ia);
         RayT T;
@@ -66,6 +93,7 @@ void rows_reads(const GeomArgs &g, double c, double s, int W, int WZ, Marks &M) 
         const float p0 = 0.5f - 0.5f * (float)np;
         const float ntf = (float)nt, nzf = (float)g.nz;
         const float ttop = (float)(nt - W), ztop = (float)(g.nz - WZ);
+        std::vector<int> gz((size_t)g.nb * np);   // window start row per (ib, ip)
         for (int ib = 0; ib < g.nb; ++ib) {
             RayZ Z;
             ray_z_setup(g, d, ib, Z);
@@ -76,6 +104,7 @@ void rows_reads(const GeomArgs &g, double c, double s, int W, int WZ, Marks &M) 
                 float wt[3], wz[3], f0, g0;
                 window_weights(clampf(t1, 0.f, ntf), clampf(t2, 0.f, ntf), ttop, W, wt, f0);
                 window_weights(clampf(z1, 0.f, nzf), clampf(z2, 0.f, nzf), ztop, WZ, wz, g0);
+                gz[(size_t)ib * np + ip] = (int)g0;
                 for (int k = 0; k < W; ++k)
                     for (int j = 0; j < WZ; ++j) {
                         const int it = (int)f0 + k, iz = (int)g0 + j;
@@ -85,6 +114,9 @@ void rows_reads(const GeomArgs &g, double c, double s, int W, int WZ, Marks &M) 
                     }
             }
         }
+        for (int R = 2; R <= 4; R *= 2)
+            if (force_share || rows_share(vi, R)) share_check(gz, g.nb, np, R);
+        if (ia == 0 && rows_share(vi, 2)) ++shared_views;
     }
 }
 
@@ -190,7 +222,7 @@ int main(int argc, char **argv) {
         }
         Marks M{g.nx, g.ny, g.nz, std::vector<unsigned char>(N, 0)};
         if (vi.wt <= 3 && vi.wz <= 3) {
-            rows_reads(g, c, s, vi.wt <= 2 ? 2 : 3, vi.wz <= 2 ? 2 : 3, M);
+            rows_reads(g, vi, c, s, vi.wt <= 2 ? 2 : 3, vi.wz <= 2 ? 2 : 3, M);
             masked_reads(g, c, s, vi.wt <= 2 ? 2 : 3, vi.wz <= 2 ? 2 : 3, M);
         } else {
             wide_reads(g, c, s, M);
@@ -251,7 +283,8 @@ int main(int argc, char **argv) {
     }
     if (view > 0) print_counts(view - 1, reads.back(), preps.back(), prev_nhit, prev_nhit);
     std::printf("views %zu kept %.4f support %.4f bad_read %lld bad_support %lld "
-                "window_misses %lld\n", angles.size(), kept / angles.size(),
-                support / angles.size(), bad_read, bad_support, window_misses);
-    return (bad_read || bad_support || window_misses) ? 1 : 0;
+                "window_misses %lld share_misses %lld shared_views %lld\n", angles.size(),
+                kept / angles.size(), support / angles.size(), bad_read, bad_support,
+                window_misses, share_misses, shared_views);
+    return (bad_read || bad_support || window_misses || share_misses) ? 1 : 0;
 }

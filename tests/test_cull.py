@@ -83,6 +83,21 @@ def test_read_set_at_256x256x128(cull_check):
     rc, line = _run(cull_check, g, 0, 0.0, ("0.3", "0.7853982", "2.0"))
     assert rc == 0, line
     assert 0.55 < _fields(line)["kept"] < 0.75, line   # the fan misses a third of this volume
+    # the image projector's rays share their z-rows on every view here (share_misses == 0 is
+    # part of the exit status)
+    assert _fields(line)["shared_views"] == 3, line
+
+
+def test_replay_catches_rows_too_far_apart_to_share(cull_check, monkeypatch):
+    """With odd sizes adjacent detector rows' footprints lie ~0.7 voxel apart: two rays may share
+    their z-rows (rows_share holds for R = 2), four may not.  Forced to share at R = 4 as well,
+    the replay finds rays whose windows start more than one row apart."""
+    g = GEOMETRIES["odd sizes"]
+    rc, line = _run(cull_check, g, 4, 0.0)
+    assert rc == 0 and _fields(line)["shared_views"] == 4, line
+    monkeypatch.setenv("CULL_CHECK_FORCE_SHARE", "1")
+    rc, line = _run(cull_check, g, 4, 0.0)
+    assert rc == 1 and _fields(line)["share_misses"] > 0, line
 
 
 @pytest.mark.parametrize("name", ["128^3", "offset truncated detector", "odd sizes",
