@@ -332,13 +332,45 @@ __device__ inline void window_values(const char *__restrict__ vb, unsigned off, 
     }
 }
 
+// The values of the R rays' windows, val[r][k][j] = cell k of z-row j of ray r's window.
+// Without SH every ray gathers its own window.  With SH (rows_share holds for the view) the
+// rays' windows start at most one z-row apart, so rows gmin .. gmin + WZ - 1 and gmax + WZ - 1
+// hold all of them: the lane gathers those WZ + 1 rows once (instead of WZ per ray) and each ray
+// selects its WZ rows, shifted by one where its window starts above gmin.  The values are the
+// same either way, so are the sums, bit for bit, whatever the values are.
+template <int WT, int WZ, int R, bool SH, typename E>
+__device__ inline void lane_values(const char *__restrict__ vb, const PlaneFoot<WT, WZ, R> &P,
+                                   unsigned sz, unsigned szB, E (&val)[R][WT][WZ]) {
+    if (!SH || R == 1) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) window_values(vb, P.off[r], szB, val[r]);
+        return;
+    }
+    constexpr int LOGES = sizeof(E) == 8 ? 3 : 2;
+    // g_r is monotone in r (adjacent rows' footprints are one row spacing apart)
+    const unsigned gmin = min(P.g[0], P.g[R - 1]), gmax = max(P.g[0], P.g[R - 1]);
+    E rows[WT][WZ + 1];
+    const unsigned o0 = (__umul24(gmin, sz) << LOGES) + P.base;   // row gmin
+#pragma unroll
+    for (int j = 0; j <= WZ; ++j) {
+        // rows gmin + j, then gmax + WZ - 1 (gmax is gmin or gmin + 1)
+        const unsigned o = j < WZ ? o0 + (unsigned)j * szB
+                                  : o0 + (unsigned)(WZ - 1) * szB + (gmax != gmin ? szB : 0u);
+#pragma unroll
+        for (int k = 0; k < WT; ++k) rows[k][j] = *(const E *)(vb + o + (unsigned)(k * sizeof(E)));
+    }
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const bool up = P.g[r] != gmin;
+#pragma unroll
+        for (int k = 0; k < WT; ++k)
+#pragma unroll
+            for (int j = 0; j < WZ; ++j) val[r][k][j] = up ? rows[k][j + 1] : rows[k][j];
+    }
+}
+
 // PL planes from plane coordinate p: all footprints, then all gathers (in flight together),
-// then the weighted sums.  SH (image mode, R >= 2, when rows_share holds for the view): the R rays'
-// windows start at most one z-row apart, so rows gmin .. gmin + WZ - 1 and gmax + WZ - 1 hold every
-// ray's window: WZ + 1 gathers for the lane instead of WZ per ray.  Ray r applies its weights
-// shifted by g_r - gmin (0 or 1), a zero weight on the one row outside its window; for the finite
-// values of the image sweep that adds exactly nothing, so the sums equal the per-ray windows' bit
-// for bit, and every cell read is one of some ray's own window.
+// then the weighted sums.  SH: the lane's rays share their z-rows (lane_values).
 template <int MODE, int WT, int WZ, int R, int PL, bool SH = false>
 __device__ inline void rows_step(const char *__restrict__ vb, const RayT &T, const RayZ (&Z)[R],
                                  float p, unsigned pofs, unsigned spB, unsigned sz, float ntf,
@@ -354,9 +386,7 @@ __device__ inline void rows_step(const char *__restrict__ vb, const RayT &T, con
     if (MODE == PROJ_MOTION) {
         float2 val[PL][R][WT][WZ];
 #pragma unroll
-        for (int q = 0; q < PL; ++q)
-#pragma unroll
-            for (int r = 0; r < R; ++r) window_values(vb, P[q].off[r], szB, val[q][r]);
+        for (int q = 0; q < PL; ++q) lane_values<WT, WZ, R, SH>(vb, P[q], sz, szB, val[q]);
 #pragma unroll
         for (int q = 0; q < PL; ++q)
 #pragma unroll
@@ -376,55 +406,10 @@ __device__ inline void rows_step(const char *__restrict__ vb, const RayT &T, con
                 acc0[r] = fmaf(P[q].sc[r], s0, acc0[r]);
                 acc1[r] = fmaf(P[q].sc[r], s1, acc1[r]);
             }
-    } else if (SH && MODE == PROJ_IMAGE && R >= 2) {
-        constexpr int NZ = WZ + 1;
-        float val[PL][WT][NZ];
-        unsigned gmin[PL];
-#pragma unroll
-        for (int q = 0; q < PL; ++q) {
-            // g_r is monotone in r (the rows' footprints shift by one row spacing each)
-            gmin[q] = min(P[q].g[0], P[q].g[R - 1]);
-            const unsigned gmax = max(P[q].g[0], P[q].g[R - 1]);
-            unsigned orow[NZ];
-            orow[0] = (__umul24(gmin[q], sz) << LOGES) + P[q].base;
-#pragma unroll
-            for (int j = 1; j < WZ; ++j) orow[j] = orow[j - 1] + szB;
-            orow[WZ] = orow[WZ - 1] + (gmax != gmin[q] ? szB : 0u);
-#pragma unroll
-            for (int j = 0; j < NZ; ++j)
-#pragma unroll
-                for (int k = 0; k < WT; ++k)
-                    val[q][k][j] = *(const float *)(vb + orow[j] + (unsigned)k * 4u);
-        }
-#pragma unroll
-        for (int q = 0; q < PL; ++q)
-#pragma unroll
-            for (int r = 0; r < R; ++r) {
-                const bool up = P[q].g[r] != gmin[q];   // window one row above gmin
-                float wzs[NZ];
-#pragma unroll
-                for (int j = 0; j < NZ; ++j) {
-                    const float lo = j > 0 ? P[q].wz[r][j - 1] : 0.f;
-                    const float hi = j < WZ ? P[q].wz[r][j] : 0.f;
-                    wzs[j] = up ? lo : hi;
-                }
-                float s0 = 0.f;
-#pragma unroll
-                for (int k = 0; k < WT; ++k) {
-                    float u0 = 0.f;
-#pragma unroll
-                    for (int j = 0; j < NZ; ++j) u0 = fmaf(wzs[j], val[q][k][j], u0);
-                    s0 = fmaf(P[q].wt[k], u0, s0);
-                }
-                acc0[r] = fmaf(P[q].sc[r], s0, acc0[r]);
-                acc1[r] = fmaf(P[q].sc[r], P[q].sw * P[q].swz[r], acc1[r]);
-            }
     } else {
         float val[PL][R][WT][WZ];
 #pragma unroll
-        for (int q = 0; q < PL; ++q)
-#pragma unroll
-            for (int r = 0; r < R; ++r) window_values(vb, P[q].off[r], szB, val[q][r]);
+        for (int q = 0; q < PL; ++q) lane_values<WT, WZ, R, SH>(vb, P[q], sz, szB, val[q]);
 #pragma unroll
         for (int q = 0; q < PL; ++q)
 #pragma unroll
@@ -633,17 +618,21 @@ void launch_rows(const GeomArgs &ga, const ViewTrig &t, const ProjArgs &a, hipSt
                            t.c, t.s, a, S);
 }
 
-// sh: the image mode's shared z-rows (rows_share holds for the view at this R)
+// R rays per lane, sharing their z-rows where rows_share holds for the view (vi)
 template <int MODE, int WT, int WZ>
-void launch_rows_r(const GeomArgs &ga, const ViewTrig &t, const ProjArgs &a, hipStream_t st,
-                   int R, int S, bool sh) {
-    constexpr bool IMG = MODE == PROJ_IMAGE;
-    if (MODE == PROJ_MOTION || R == 1)   // the motion mode runs one ray per lane
+void launch_rows_r(const GeomArgs &ga, const ViewTrig &t, const ViewInfo &vi, const ProjArgs &a,
+                   hipStream_t st, int R, int S) {
+    if constexpr (MODE == PROJ_MOTION) {
+        // the motion mode gathers (g, Q) pairs, twice the registers per footprint: one ray per
+        // lane (two sharing their z-rows measured the same at 256x256x128, 51.5 us)
         launch_rows<MODE, WT, WZ, 1, false>(ga, t, a, st, S);
-    else if (R == 2 && IMG && sh) launch_rows<MODE, WT, WZ, (IMG ? 2 : 1), IMG>(ga, t, a, st, S);
-    else if (R == 2) launch_rows<MODE, WT, WZ, (MODE == PROJ_MOTION ? 1 : 2), false>(ga, t, a, st, S);
-    else if (IMG && sh) launch_rows<MODE, WT, WZ, (IMG ? 4 : 1), IMG>(ga, t, a, st, S);
-    else launch_rows<MODE, WT, WZ, (MODE == PROJ_MOTION ? 1 : 4), false>(ga, t, a, st, S);
+    } else {
+        if (R == 1) launch_rows<MODE, WT, WZ, 1, false>(ga, t, a, st, S);
+        else if (R == 2 && rows_share(vi, 2)) launch_rows<MODE, WT, WZ, 2, true>(ga, t, a, st, S);
+        else if (R == 2) launch_rows<MODE, WT, WZ, 2, false>(ga, t, a, st, S);
+        else if (rows_share(vi, 4)) launch_rows<MODE, WT, WZ, 4, true>(ga, t, a, st, S);
+        else launch_rows<MODE, WT, WZ, 4, false>(ga, t, a, st, S);
+    }
 }
 
 template <int MODE>
@@ -651,12 +640,11 @@ int launch_mode_rows(const Geom &g, const ViewTrig &t, const ViewInfo &vi, const
                      hipStream_t st, int R, int S) {
     const GeomArgs ga{g.nx, g.ny, g.nz, g.na, g.nb, g.da, g.db, g.ai, g.bi, g.SO, g.SD};
     const int code = (vi.wt <= 2 ? 0 : 1) * 2 + (vi.wz <= 2 ? 0 : 1);
-    const bool sh = rows_share(vi, R);
     switch (code) {
-        case 0: launch_rows_r<MODE, 2, 2>(ga, t, a, st, R, S, sh); break;
-        case 1: launch_rows_r<MODE, 2, 3>(ga, t, a, st, R, S, sh); break;
-        case 2: launch_rows_r<MODE, 3, 2>(ga, t, a, st, R, S, sh); break;
-        default: launch_rows_r<MODE, 3, 3>(ga, t, a, st, R, S, sh); break;
+        case 0: launch_rows_r<MODE, 2, 2>(ga, t, vi, a, st, R, S); break;
+        case 1: launch_rows_r<MODE, 2, 3>(ga, t, vi, a, st, R, S); break;
+        case 2: launch_rows_r<MODE, 3, 2>(ga, t, vi, a, st, R, S); break;
+        default: launch_rows_r<MODE, 3, 3>(ga, t, vi, a, st, R, S); break;
     }
     MCSART_CHECK_LAUNCH();
     return MCSART_OK;
