@@ -216,3 +216,37 @@ def test_staged_equals_global_reads(which):
         assert np.array_equal(a, b, equal_nan=True)
     if which == "mcsart128_overflow":   # the case this test is for
         assert not np.all(np.isfinite(nrm1))
+
+
+@pytest.mark.parametrize("which", ["clinical256_48views", "mcsart128"])
+def test_projector_rows_agree_bit_for_bit(which):
+    """A ray's projection is the same arithmetic whether its lane walks one, two or four rays and
+    whether those rays share their z-rows (rows_share: at 256x256x128 two and four rays share,
+    at 128^3 only two), so with the split fixed whole schedules agree bit for bit."""
+    from mcsart_amd.configs import make_scan_inputs, scaled
+    from mcsart_amd.simulate import measure
+    if which == "clinical256_48views":
+        spec = scaled(configs.c3_clinical(), n_views=48, op_iter=[1, 0, 1])
+    else:
+        spec = scaled(configs.c2_mcsart128(), n_views=60, op_iter=[1, 0, 1])
+    data = make_scan_inputs(spec)
+    proj = measure(spec, data, spec.seed + 9)
+    print(f"projector rows 1/2/4: {spec.name}, {spec.n_views} views", flush=True)
+    ctx = Context(0)
+    out = {}
+    for rows in (1, 2, 4):
+        job = Job(ctx, spec.geometry, spec.angles, data.alpha, data.beta, spec.op_iter, lam=1.0,
+                  motion=True, nonneg=True)
+        job.set_tuning(proj_split=8, proj_rows=rows)
+        job.set_projections(proj)
+        job.set_image(None)
+        job.set_fields(data.fields)
+        job.run()
+        out[rows] = (job.image(), job.norms(), job.fields())
+        job.close()
+    ctx.close()
+    for rows in (2, 4):
+        assert np.array_equal(out[rows][0], out[1][0]), rows
+        assert np.array_equal(out[rows][1], out[1][1], equal_nan=True), rows
+        for a, b in zip(out[rows][2], out[1][2]):
+            assert np.array_equal(a, b, equal_nan=True), rows
