@@ -260,7 +260,9 @@ __global__ void __launch_bounds__(RAYS *MAXSPLIT) k_project(GeomArgs g, double c
 // w_k = F(T0+k+1) - F(T0+k) = min(T0+k+1, hi) - max(T0+k, lo) whenever the cell meets the
 // footprint (src/cu_projection.cu:83,86), and 0 otherwise.  Plane offsets use 24-bit multiplies
 // and gathers 32-bit byte offsets from one base (launch requires nx*ny < 2^24 and both layouts
-// within 4 GiB of the lower one).  Requires the xy-transposed layout for x-branch rays.
+// within 4 GiB of the lower one).  Requires the xy-transposed layout for x-branch rays.  Where
+// adjacent detector rows' footprints lie less than a voxel apart (rows_share), the R rays also
+// share their z-rows: WZ + 1 gathers per plane for the lane (lane_values).
 // med3(x, lo, hi) = min(max(x, lo), hi) when lo <= hi, in one instruction (v_med3_f32).
 __device__ inline float clampf(float x, float lo, float hi) {
     return __builtin_amdgcn_fmed3f(x, lo, hi);
@@ -357,7 +359,8 @@ __device__ inline void lane_values(const char *__restrict__ vb, const PlaneFoot<
         const unsigned o = j < WZ ? o0 + (unsigned)j * szB
                                   : o0 + (unsigned)(WZ - 1) * szB + (gmax != gmin ? szB : 0u);
 #pragma unroll
-        for (int k = 0; k < WT; ++k) rows[k][j] = *(const E *)(vb + o + (unsigned)(k * sizeof(E)));
+        for (int k = 0; k < WT; ++k)
+            rows[k][j] = *(const E *)(vb + o + (unsigned)(k * sizeof(E)));
     }
 #pragma unroll
     for (int r = 0; r < R; ++r) {
