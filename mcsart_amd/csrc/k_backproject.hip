@@ -19,6 +19,7 @@
 //
 // The SART mode reads (r, P1) interleaved as float2 and writes B(r)/B(P1) with NaN/Inf -> 0,
 // fusing src/AUMISART.cu:212-218 (two backprojections and a division) into one pass.
+#include <cmath>
 #include <type_traits>
 
 #include "footprint.hpp"
@@ -128,7 +129,7 @@ __global__ void __launch_bounds__(BX *BY) k_backproject(GeomArgs g, double c, do
                                                        const float2 *__restrict__ rp,
                                                        float *__restrict__ out, int zslab,
                                                        double ida, double idb, NormFinish nf,
-                                                       MotionStep mot, AddStep add) {
+                                                       MotionStep mot, AddStep add, int xbands) {
     const int tid = threadIdx.y * BX + threadIdx.x;
     // the preceding projector's per-view norm (its partials are complete: kernel boundary); the
     // whole first wavefront (threads 0..63) of the first workgroup does it
@@ -141,8 +142,12 @@ __global__ void __launch_bounds__(BX *BY) k_backproject(GeomArgs g, double c, do
     // (image sweep 47.8 -> 49.1 us, motion sweep 59.8 -> 60.3 us)
     const int tile = xcd_order(blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z),
                                gridDim.x * gridDim.y * gridDim.z);
+    // xbands (rays mostly along x: the view's fan is a band across y, and out-of-fan voxels cost
+    // little): each XCD takes a band of tile columns instead of tile rows, so every XCD has its
+    // share of the fan
     const int bzl = tile % gridDim.z, rest = tile / gridDim.z;
-    const int bxl = rest % gridDim.x, byl = rest / gridDim.x;
+    const int bxl = xbands ? rest / gridDim.y : rest % gridDim.x;
+    const int byl = xbands ? rest % gridDim.y : rest / gridDim.x;
     const int ix = bxl * BX + threadIdx.x;
     const int iy = byl * BY + threadIdx.y;
     const bool live = ix < g.nx && iy < g.ny;
@@ -432,9 +437,10 @@ void launch_wa(dim3 grid, dim3 block, hipStream_t st, const GeomArgs &ga, const 
     const double ida = 1.0 / (double)ga.da, idb = 1.0 / (double)ga.db;
     const MotionStep m = mot ? *mot : MotionStep{};
     const AddStep ad = add ? *add : AddStep{};
+    const int xb = std::fabs(t.c) > std::fabs(t.s) ? 1 : 0;
 #define MCSART_BACK(ST, EP)                                                                      \
     hipLaunchKernelGGL((k_backproject<MODE, WA, ST, EP>), grid, block, 0, st, ga, t.c, t.s, proj, \
-                       rp, out, zs, ida, idb, nf, m, ad)
+                       rp, out, zs, ida, idb, nf, m, ad, xb)
     constexpr int MOTION = MODE == BACK_RATIO ? EPI_MOTION : EPI_STORE;
     constexpr int ADD = MODE == BACK_RATIO ? EPI_ADD : EPI_STORE;
     if (MODE == BACK_RATIO && mot) {
