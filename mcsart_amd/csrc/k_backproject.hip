@@ -437,7 +437,16 @@ void launch_wa(dim3 grid, dim3 block, hipStream_t st, const GeomArgs &ga, const 
     const double ida = 1.0 / (double)ga.da, idb = 1.0 / (double)ga.db;
     const MotionStep m = mot ? *mot : MotionStep{};
     const AddStep ad = add ? *add : AddStep{};
-    const int xb = std::fabs(t.c) > std::fabs(t.s) ? 1 : 0;
+    // bands of tile columns for views whose rays run mostly along x, when the fan at the
+    // isocentre covers less than 85 % of the volume's width (a fan as wide as the volume leaves
+    // no band empty, and column bands cost L2 locality: every XCD then reads the whole detector)
+    const double mag = ga.SO / ga.SD, half = 0.5 * (double)ga.ny;
+    const double lat0 = ((double)ga.ai - 0.5) * ga.da * mag;
+    const double lat1 = ((double)ga.na + (double)ga.ai - 0.5) * ga.da * mag;
+    const double inside = std::fmin(std::fmax(lat0, lat1), half) -
+                          std::fmax(std::fmin(lat0, lat1), -half);
+    const double covered = std::fmax(0.0, inside) / (double)ga.ny;
+    const int xb = std::fabs(t.c) > std::fabs(t.s) && covered < 0.85 ? 1 : 0;
 #define MCSART_BACK(ST, EP)                                                                      \
     hipLaunchKernelGGL((k_backproject<MODE, WA, ST, EP>), grid, block, 0, st, ga, t.c, t.s, proj, \
                        rp, out, zs, ida, idb, nf, m, ad, xb)
