@@ -371,15 +371,18 @@ typedef struct {
     int back_zslab;   /* backprojector voxels per lane along z: 0 or 1..64 */
     int back_stage;   /* backprojector detector reads: 0 automatic (LDS-staged rectangle),
                          1 staged in LDS, 2 straight from global memory */
-    int proj_rows;    /* detector rows (rays of one column) per projector lane: 0, 1, 2 or 4 */
+    int proj_rows;    /* detector rows (rays of one column) per projector lane: 0, 1, 2 or 4;
+                         the rays share their z-row gathers where adjacent rows' footprints lie
+                         less than a voxel apart (bit-identical either way) */
     int back_zslab_motion; /* voxels per lane along z of the motion sweep's backprojector (with
                               the fused field update): 0 (automatic) or 1..64 */
 } mcsart_tuning;
 int mcsart_job_set_tuning(mcsart_job *job, const mcsart_tuning *t);
 int mcsart_job_get_tuning(mcsart_job *job, mcsart_tuning *t);
-/* Time the candidate launch shapes on a few of the job's views (projector in residual mode on the
- * current image, backprojector in ratio mode into scratch; the job's image, fields and norms are
- * untouched), keep the fastest and report it in *chosen (may be NULL).  Synchronous. */
+/* Time the candidate launch shapes on up to sixteen of the job's views spread over the scan
+ * (projector in residual mode on the current image, backprojector in ratio mode into scratch;
+ * the job's image, fields and norms are untouched), keep the fastest and report it in *chosen
+ * (may be NULL).  Synchronous. */
 int mcsart_job_autotune(mcsart_job *job, mcsart_tuning *chosen);
 /* Average duration (microseconds) of one launch of a kernel of the sweeps, with the job's launch
  * shapes: 0 projector (residual mode), 1 backprojector (ratio mode), 2 warp of the image into the
