@@ -482,9 +482,11 @@ __global__ void __launch_bounds__(RAYS * 16) k_project_rows(GeomArgs g, double c
         // PL planes per step, all their gathers in flight together, the last ones one at a time.
         // One plane when a lane walks two or more rays: their gathers already overlap, and a
         // second plane's registers cost occupancy (R = 2 image mode: 74 VGPRs, 6 waves per SIMD,
-        // against 53 and 8 with one plane; measured 52.0 -> 50.6 us at 256x256x128).  p and the
-        // plane's byte offset are carried: p is a small integer + 0.5, exact in float.
-        constexpr int PL = R >= 2 ? 1 : 2;
+        // against 53 and 8 with one plane; measured 52.0 -> 50.6 us at 256x256x128) -- except
+        // for two rays sharing their z-rows, which hold fewer values: 68 VGPRs and 7 waves with
+        // two planes, 46.2 -> 44.1 us.  p and the plane's byte offset are carried: p is a small
+        // integer + 0.5, exact in float.
+        constexpr int PL = (R == 1 || (SH && R == 2)) ? 2 : 1;
         int ip = ip_lo;
         float p = (float)ip_lo + p0;
         unsigned pofs = (xb ? a.roffT : a.roffN) + (unsigned)ip_lo * spB;
