@@ -80,12 +80,12 @@ def test_read_set_and_support_cover_every_read(cull_check, name):
 
 def test_read_set_at_256x256x128(cull_check):
     g = configs.c3_clinical().geometry
-    rc, line = _run(cull_check, g, 0, 0.0, ("0.3", "0.7853982", "2.0"))
+    rc, line = _run(cull_check, g, 8, 0.0, ("0.3", "0.7853982", "2.0"))
     assert rc == 0, line
     assert 0.55 < _fields(line)["kept"] < 0.75, line   # the fan misses a third of this volume
     # the image projector's rays share their z-rows on every view here (share_misses == 0 is
     # part of the exit status)
-    assert _fields(line)["shared_views"] == 3, line
+    assert _fields(line)["shared_views"] == 11, line
 
 
 def test_replay_catches_rows_too_far_apart_to_share(cull_check, monkeypatch):
