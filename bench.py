@@ -551,6 +551,9 @@ def main():
         ach = nbytes / (us * 1e-6) / 1e9
         kernels_roofline[k] = {
             "avg_launch_us": round(us, 2), "bytes_per_launch": int(nbytes),
+            # the same launch's bytes if it processed every voxel (the culled kernels: warp,
+            # warp-add, motion step); frac and achieved count only the voxels it processes
+            "bytes_per_launch_uncut": int(algorithmic_bytes(spec, byte_kind[k], layouts, None)),
             "achieved": round(ach, 1), "frac": round(ach / HBM_PEAK_GBS, 4),
             "launches_per_scan": per_scan[k],
             "ms_per_scan": round(us * per_scan[k] * 1e-3, 2),
@@ -565,6 +568,10 @@ def main():
         "bound": "hbm", "kernel": dom, "achieved": d["achieved"], "peak": HBM_PEAK_GBS,
         "unit": "GB/s", "frac": d["frac"], "traffic": traffic,
         "avg_launch_us": d["avg_launch_us"], "bytes_per_launch": d["bytes_per_launch"],
+        "bytes_per_launch_uncut": d["bytes_per_launch_uncut"],
+        "bytes_note": ("bytes_per_launch counts the voxels the culled launch processes (achieved "
+                       "and frac use it); bytes_per_launch_uncut, every voxel, is what the kernel "
+                       "moved before culling, not a bound on this launch"),
         "timing": "HIP events around all views launched back to back, 3 repetitions",
         "traffic_source": (f"{src} (rocprofv3 FETCH_SIZE + WRITE_SIZE, calibrated on a 64 MiB "
                            "copy)") if traffic else None,
