@@ -83,6 +83,9 @@ IRREGULAR = {
                                            SO=70.0, SD=120.0),
     "odd_sizes": Geometry.from_user(37, 37, 19, 53, 31, da=1.3, db=1.1, ai=0.35, bi=-0.2,
                                     SO=55.0, SD=97.0),
+    # nx != ny with a fan narrower than the volume (78 % of its width at the isocentre): views
+    # whose rays run along x band the backprojector's tiles by column over unequal tile counts
+    "non_square": Geometry.from_user(48, 32, 20, 40, 24, da=1.0, db=1.0, SO=100.0, SD=160.0),
 }
 
 
