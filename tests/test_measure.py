@@ -169,3 +169,47 @@ def test_occupancy_table(tmp_path):
     rows = [r for r in md.splitlines() if r.startswith("| `")]
     assert rows == ["| `k_warp_tile<true, 32>` | 2097152 | 1024 | 32768 | 2 | 15.0 |",
                     "| `k_warp_tile<true, 32>` | 8388608 | 1024 | 131072 | 1 | 20.5 |"]
+
+
+def test_per_view_trace_grouping(tmp_path):
+    """mcsart_amd.profile_views puts each launch of a sweep trace on the view it serves (the
+    image sweep's kernels on views 0 .. V-1, the motion sweep's on 1 .. V-1) and groups the motion
+    step by the layouts of the view after it: with every launch's duration set to its view's
+    index, each group's mean is the mean of the indices in it."""
+    import numpy as np
+    from mcsart_amd import profile_views as pv
+    from mcsart_amd.driver import view_voxels
+    spec = configs.c2_mcsart128()
+    V = spec.n_views
+    names = {"warp": "k_warp_tile<true, 32>", "projector": "k_project_rows<1, 2, 2, 2, true, true>",
+             "backprojector": "k_backproject<1, 6, true, 0>", "warp_add": "k_warp_add_tile<true, 32>",
+             "projector_motion": "k_project_rows<2, 2, 2, 1, true, false>",
+             "motion": "k_backproject<1, 5, true, 1>"}
+    offset = {k: 1000 * (i + 1) for i, k in enumerate(names)}
+    order = [(k, v) for v in range(V) for k in ("warp", "projector", "backprojector", "warp_add")]
+    order += [(k, v) for v in range(1, V) for k in ("projector_motion", "motion")]
+    with open(tmp_path / "run_kernel_trace.csv", "w", newline="") as f:
+        w = csv.writer(f)
+        w.writerow(["Kernel_Name", "Start_Timestamp", "End_Timestamp"])
+        t = 0
+        for k, v in order:
+            ns = 1000 * (offset[k] + v)
+            w.writerow([f"void mcsart::(anonymous namespace)::{names[k]}(float const*, int)", t,
+                        t + ns])
+            t += ns + 500
+    res = pv.summarise(spec, pv.read_trace(str(tmp_path)))
+    lay = view_voxels(spec.geometry, spec.angles)[:, 4]
+    deg = np.degrees(spec.angles.astype(np.float64)) % 180.0
+    for k in names:
+        views = np.arange(0 if k in ("warp", "projector", "backprojector", "warp_add") else 1, V)
+        served = np.minimum(views + 1, V - 1) if k == "motion" else views
+        assert res[k]["launches"] == len(views)
+        assert res[k]["mean_us"] == pytest.approx(offset[k] + views.mean())
+        for code, name in pv.LAYOUTS.items():
+            sel = views[lay[served] == code]
+            if sel.size:
+                assert res[k]["by_layout"][name] == (pytest.approx(offset[k] + sel.mean()), sel.size)
+        lo = views[deg[views] < 15]
+        assert res[k]["by_angle_15deg"][0] == pytest.approx(offset[k] + lo.mean())
+    pv.write_md(spec, res, str(tmp_path / "views.md"), ["cmd"])
+    assert "| motion | %d |" % (V - 1) in (tmp_path / "views.md").read_text()
